@@ -1,0 +1,189 @@
+"""Benchmark: particle-substeps/s of the HIP MLS-MPM substep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multi_material|elastic_block]
+
+A step is one ``Solver.advance(dt)`` of the workload, which is exactly one substep (dt below the CFL
+step): collider pose upload, CFL speed reduction, P2G, grid update, collider contact, G2P, particle
+projection. Inputs are resident in HBM before the timed region; the timed region is bracketed by
+device synchronisation. Per-kernel device time comes from HIP events recorded by the library on its
+own stream around every launch.
+
+The printed JSON line carries:
+  roofline      for the kernel with the largest device time: algorithmic bytes per launch (bytes per
+                particle x particles + bytes per active grid node x active nodes) over its mean launch
+                time, against 8 TB/s HBM
+  cpu_baseline  the CPU oracle (a restatement of the reference Warp kernels in C++, OpenMP, FP32) timed
+                on a bounded number of substeps of the same workload on this host's cores
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per particle and per active node, reference data flow (each array touched once)
+P2G_BYTES_PARTICLE = 104  # x12 v12 C36 F36 mat4 p_mass4
+P2G_BYTES_SNOW = 4  # Jp
+G2P_BYTES_PARTICLE = 148  # read x12 F36 mat4, write x12 v12 C36 F36
+G2P_BYTES_SNOW = 8  # Jp read + write
+P2G_BYTES_NODE = 16
+G2P_BYTES_NODE = 12
+GRID_BYTES_NODE = 28 + 16  # read accum 16, write velocity 12 (+ zeroing 16 folded into the update)
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse_args(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="multi_material", choices=["multi_material", "elastic_block"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-collider", action="store_true")
+    return ap.parse_args(argv)
+
+
+def build_workload(name: str, with_collider: bool):
+    from mpm_plugin_amd import workloads
+
+    if name == "elastic_block":
+        return workloads.elastic_block()
+    return workloads.multi_material(with_collider=with_collider)
+
+
+def active_nodes(solver) -> int:
+    _, gm = solver.get_grid(1)
+    return int((gm > 0).sum())
+
+
+def cpu_baseline(w, seconds: float) -> dict:
+    """Time the FP32 CPU oracle on the same workload for about ``seconds`` of work."""
+    from oracle import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    ora = O.OracleSolver(w.domain, w.materials, w.gravity, threads=threads)
+    ora.add_particles(w.positions, w.velocities, w.material_ids, w.emitter_ids)
+    if w.colliders:
+        ora.set_colliders(w.colliders)
+    t_sim, steps, elapsed = 0.0, 0, 0.0
+    t0 = time.perf_counter()
+    while elapsed < seconds and steps < w.substeps:
+        if w.colliders:
+            ora.set_collider_transforms(*w.collider_transforms(t_sim, t_sim + w.dt))
+        ora.advance(w.dt)
+        t_sim += w.dt
+        steps += 1
+        elapsed = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": w.n * steps / elapsed, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} substeps of the same {w.n}-particle workload (FP32 C++ restatement of the "
+                      f"reference Warp kernels, OpenMP, {threads} threads on {cpu}), {elapsed:.1f} s"}
+
+
+def main(argv=None) -> int:
+    args = parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (independent scenes shard "
+              f"across GPUs without communication, so the per-GPU figure is the one to multiply)",
+              file=sys.stderr)
+        return 2
+    w = build_workload(args.workload, with_collider=not args.no_collider)
+    log(f"workload {w.name}: {w.n} particles, grid {w.domain.grid_shape}, dt {w.dt}")
+    solver = w.make_solver(device="cuda:0")
+    t_sim = 0.0
+
+    def step():
+        nonlocal t_sim
+        if w.colliders:
+            solver.set_collider_transforms(*w.collider_transforms(t_sim, t_sim + w.dt))
+        st = solver.advance(w.dt)
+        if st.substeps != 1:
+            raise RuntimeError(f"advance({w.dt}) took {st.substeps} substeps; the workload assumes one")
+        t_sim += w.dt
+
+    for _ in range(args.warmup):
+        step()
+    solver.synchronize()
+    n_active = active_nodes(solver)
+    n_snow = int(sum((w.material_ids == i).sum() for i, m in enumerate(w.materials) if m.model == "snow"))
+    log(f"warmup done ({args.warmup} steps); active nodes {n_active}")
+
+    solver.reset_timing()
+    solver.set_timing(True)
+    solver.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    solver.synchronize()
+    elapsed = time.perf_counter() - t0
+    solver.set_timing(False)
+    times = solver.kernel_times()
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; kernel ms: "
+        + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in times.items()))
+
+    n = solver.particle_count
+    bytes_per_launch = {
+        "p2g": P2G_BYTES_PARTICLE * n + P2G_BYTES_SNOW * n_snow + P2G_BYTES_NODE * n_active,
+        "g2p": G2P_BYTES_PARTICLE * n + G2P_BYTES_SNOW * n_snow + G2P_BYTES_NODE * n_active,
+    }
+    dominant = max(("p2g", "g2p"), key=lambda k: times[k][0])
+    ms, launches = times[dominant]
+    mean_s = ms / 1000.0 / max(launches, 1)
+    achieved = bytes_per_launch[dominant] / mean_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
+                "mean_launch_us": round(mean_s * 1e6, 2), "algorithmic_bytes": int(bytes_per_launch[dominant])}
+    kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in times.items() if v[1]}
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        baseline = cpu_baseline(w, args.cpu_seconds)
+        log(f"cpu baseline {baseline['value']:.3e} particle-substeps/s")
+
+    value = n * args.steps / elapsed
+    line = {
+        "metric": "particle-substeps/s at 1M pts/128^3",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1000.0 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "particles": n, "grid": list(w.domain.grid_shape), "dt": w.dt,
+                   "materials": [m.model for m in w.materials], "colliders": len(w.colliders),
+                   "substeps_per_step": 1},
+        "roofline": roofline,
+        "cpu_baseline": baseline,
+        "kernels": kernels,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,158 @@
+/* mpm_hip.h: C-ABI of libmpm_hip.so, the MI355X (gfx950) MLS-MPM substep library.
+ *
+ * The reference's boundary for this path is the Python ABC mpm_core.solver.Solver
+ * (src/mpm_core/solver.py:37-82) implemented by WarpSolver (src/mpm_core/backends/warp/solver.py:26-255).
+ * Its FFI is Python: mpm_core/backends/hip/solver.py binds every entry point below with ctypes. Each
+ * declaration names the reference method or kernel it replaces.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only; host arrays use the reference's numpy layouts: positions (N,3)
+ *     float32, C and F (N,3,3) float32 row major, Jp (N,) float32, material/emitter ids (N,) int32,
+ *     grids (D0,D1,D2,3) / (D0,D1,D2) float32 in C order with D = grid_shape + 6 (PAD = 3, origin =
+ *     domain min - 3 dx).
+ *   - Every call is synchronous with respect to host-visible outputs and returns MPM_OK or a negative
+ *     code; mpm_last_error() gives the message of the last failure on the calling thread.
+ *   - One HIP stream per context. Contexts share no state, so several may live on different threads.
+ */
+#ifndef MPM_HIP_H
+#define MPM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+
+#define MPM_OK 0
+#define MPM_ERR_VALUE (-1)           /* bad argument: ValueError in the reference */
+#define MPM_ERR_NOT_IMPLEMENTED (-2) /* unsupported material model: NotImplementedError */
+#define MPM_ERR_MEMORY (-3)          /* device allocation failed: MemoryError */
+#define MPM_ERR_RUNTIME (-4)         /* HIP runtime failure: RuntimeError */
+
+#define MPM_MODEL_ELASTIC 0
+#define MPM_MODEL_VON_MISES 1
+#define MPM_MODEL_DRUCKER_PRAGER 2
+#define MPM_MODEL_SNOW 3
+
+typedef struct mpm_ctx mpm_ctx;
+
+/* Domain constants, derived on the host in FP64 and rounded to FP32 as in
+ * backends/warp/solver.py:37-54. */
+typedef struct mpm_domain_desc {
+  int32_t dims[3];         /* grid_shape + 2 * PAD */
+  float origin[3];         /* domain.min - PAD * dx */
+  float dx, inv_dx, p_vol;
+  float gravity[3];
+  int32_t wall_lo_node[3]; /* floor((min - origin) / dx + 1e-3) */
+  int32_t wall_hi_node[3]; /* ceil((max - origin) / dx - 1e-3) */
+  int32_t lo_on[3], hi_on[3];
+  float wall_lo[3], wall_hi[3];
+  float clamp_lo[3], clamp_hi[3];
+  float wall_friction;
+  float cell_vol;          /* dx^3, for is_occupied */
+  float remove_lo[3];      /* float32(min - dx) per open lo side, -inf when walled (solver.py:222-240) */
+  float remove_hi[3];      /* float32(max + dx) per open hi side, +inf when walled */
+  double dx_f64;           /* dx in double precision for the CFL step count (solver.py:176-185) */
+} mpm_domain_desc;
+
+/* One material (solver.py:56-71; materials.py:48-51; _dp_alpha solver.py:263-265). */
+typedef struct mpm_material_desc {
+  int32_t model;
+  float mu, lam, yield_stress, alpha;
+  float hardening, max_hardening, crit_c, crit_s;
+  float p_mass;            /* float32(density) * float32(p_vol) (solver.py:105) */
+  double wave_speed;       /* sqrt((lam + 2 mu) * h / rho), h = max_hardening for snow (solver.py:60-62) */
+} mpm_material_desc;
+
+/* Start/end pose of one collider over the next advance (colliders.py:20-28), decomposed on the host
+ * (solver.py:160-172): M = [R(q) S | t], q as (x, y, z, w), q1 on q0's hemisphere. */
+typedef struct mpm_collider_pose {
+  float t0[3], t1[3];
+  float q0[4], q1[4];
+  float S0[9], S1[9];
+  float margin;            /* AABB early-out margin in local units: 3 dx / smallest singular value */
+} mpm_collider_pose;
+
+typedef struct mpm_advance_stats {
+  int64_t substeps;
+  double dt;
+  int64_t removed;
+} mpm_advance_stats;
+
+/* ---- lifecycle ---------------------------------------------------------------------------- */
+int32_t mpm_abi_version(void);
+const char* mpm_last_error(void);
+/* replaces WarpSolver.__init__ (solver.py:27-78) */
+int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_material_desc* materials,
+                   int32_t n_materials, mpm_ctx** out);
+int32_t mpm_destroy(mpm_ctx* ctx);
+
+/* ---- particles ---------------------------------------------------------------------------- */
+/* replaces WarpSolver.add_particles (solver.py:100-125): C = 0, F = I, Jp = 1 */
+int32_t mpm_add_particles(mpm_ctx* ctx, int64_t n, const float* x, const float* v, const int32_t* mat,
+                          const int32_t* emitter);
+/* replaces WarpSolver.load_state (solver.py:137-138) */
+int32_t mpm_load_state(mpm_ctx* ctx, int64_t n, const float* x, const float* v, const float* C, const float* F,
+                       const float* Jp, const int32_t* mat, const int32_t* emitter);
+/* replaces WarpSolver.particle_count (solver.py:127-129) */
+int64_t mpm_particle_count(const mpm_ctx* ctx);
+/* replaces get_particles / save_state (solver.py:131-135, 96-98); NULL skips a field.
+ * Particles come back in insertion order whatever order the device keeps them in. */
+int32_t mpm_get_state(mpm_ctx* ctx, float* x, float* v, float* C, float* F, float* Jp, int32_t* mat,
+                      int32_t* emitter);
+
+/* ---- colliders ---------------------------------------------------------------------------- */
+/* replaces WarpSolver.set_colliders (solver.py:142-152): meshes concatenated, local space */
+int32_t mpm_set_colliders(mpm_ctx* ctx, int32_t n, const int32_t* n_verts, const int32_t* n_tris,
+                          const float* verts, const int32_t* tris, const int32_t* closed, const float* friction);
+/* replaces WarpSolver.set_collider_transforms (solver.py:154-172) */
+int32_t mpm_set_collider_poses(mpm_ctx* ctx, int32_t n, const mpm_collider_pose* poses);
+
+/* ---- stepping ----------------------------------------------------------------------------- */
+/* replaces WarpSolver.advance (solver.py:213-220) with _choose_dt (:176-185) and _remove_escaped (:222-240) */
+int32_t mpm_advance(mpm_ctx* ctx, double frame_dt, mpm_advance_stats* out);
+/* one fixed-dt substep (solver.py:187-211) at frame fraction s of a frame lasting frame_dt */
+int32_t mpm_substep(mpm_ctx* ctx, double dt, double s, double frame_dt);
+
+/* ---- queries ------------------------------------------------------------------------------ */
+/* replaces WarpSolver.is_occupied (solver.py:244-255; kernels.py:272-298) */
+int32_t mpm_is_occupied(mpm_ctx* ctx, int64_t m, const float* points, uint8_t* out);
+
+/* ---- phase hooks for parity tests (one reference kernel each) ------------------------------ */
+/* grid zero + p2g (kernels.py:92-140) */
+int32_t mpm_debug_p2g(mpm_ctx* ctx, double dt);
+/* grid_update (kernels.py:155-180) then grid_colliders (colliders.py:77-117) */
+int32_t mpm_debug_grid_update(mpm_ctx* ctx, double dt, double s, double frame_dt);
+/* g2p (kernels.py:183-263) then project_particles (colliders.py:120-149) */
+int32_t mpm_debug_g2p(mpm_ctx* ctx, double dt, double s, double frame_dt);
+/* which = 0: P2G output (momentum, mass); which = 1: grid velocity after the update (velocity, mass) */
+int32_t mpm_get_grid(mpm_ctx* ctx, int32_t which, float* grid_v, float* grid_m);
+/* overwrite the grid velocity and mass (which = 1 layout) before mpm_debug_g2p */
+int32_t mpm_set_grid(mpm_ctx* ctx, const float* grid_v, const float* grid_m);
+/* particle binning of the last substep: per-particle brick key, brick offsets, permutation */
+int32_t mpm_get_binning(mpm_ctx* ctx, int32_t* keys, int32_t* perm, int32_t* offsets, int64_t* n_bricks);
+
+/* ---- measurement -------------------------------------------------------------------------- */
+/* kernel ids for mpm_get_kernel_time */
+#define MPM_K_P2G 0
+#define MPM_K_GRID 1
+#define MPM_K_G2P 2
+#define MPM_K_BIN 3
+#define MPM_K_COLLIDERS 4
+#define MPM_K_OTHER 5
+#define MPM_K_COUNT 6
+/* record HIP events around every launch on the context's stream (off by default) */
+int32_t mpm_set_timing(mpm_ctx* ctx, int32_t enable);
+/* accumulated device time and launch count of one kernel class since the last reset */
+int32_t mpm_get_kernel_time(mpm_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
+int32_t mpm_reset_timing(mpm_ctx* ctx);
+/* wait for all work on the context's stream */
+int32_t mpm_synchronize(mpm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM_HIP_H */

@@ -1,0 +1,408 @@
+// Device kernels of one MLS-MPM substep on gfx950 (wave64).
+//
+// Data layout in HBM
+//   particles  structure of arrays, each component a plane of `cap` floats: x[a * cap + p],
+//              C[(3 i + j) * cap + p], so a wave reads every component with one coalesced 256-byte load.
+//              Slot p holds the particle whose insertion-order index is pid[p].
+//   grid       bricks of 4x4x4 nodes, 64 nodes (1 KiB of float4) contiguous per brick, bricks in
+//              (bx, by, bz) row-major order. Two float4 arrays: `accum` receives P2G (momentum xyz,
+//              mass) and is all-zero outside P2G; `vel` holds the updated velocity (xyz) and the mass (w)
+//              that G2P and the collider pass read. The grid update turns accum into vel and writes the
+//              zeros back, so no separate clearing pass is needed.
+//
+// Kernel semantics follow the reference Warp kernels (src/mpm_core/backends/warp/kernels.py and
+// colliders.py); each kernel names the function it restates.
+#pragma once
+
+#include "device_math.hpp"
+
+namespace mpm {
+
+struct MatDev {
+  int32_t model;
+  float mu, lam, yield_stress, alpha, hardening, max_hardening, crit_c, crit_s, p_mass;
+};
+
+struct Particles {
+  float* x;
+  float* v;
+  float* C;
+  float* F;
+  float* Jp;
+  int32_t* mat;
+  int32_t* pid;
+  int64_t cap;
+};
+
+struct GridGeom {
+  int dims[3];    // nodes per axis (grid_shape + 6)
+  int bdims[3];   // bricks per axis
+  float origin[3];
+  float dx, inv_dx;
+};
+
+struct StepConsts {
+  float p_vol;
+  float gravity[3];
+  int wall_lo_node[3], wall_hi_node[3];
+  int lo_on[3], hi_on[3];
+  float wall_lo[3], wall_hi[3];
+  float clamp_lo[3], clamp_hi[3];
+  float wall_friction;
+};
+
+__device__ __forceinline__ int node_addr(const GridGeom& g, int i, int j, int k) {
+  return (((i >> 2) * g.bdims[1] + (j >> 2)) * g.bdims[2] + (k >> 2)) * 64 + ((i & 3) << 4) + ((j & 3) << 2) + (k & 3);
+}
+
+__device__ __forceinline__ void node_ijk(const GridGeom& g, int addr, int& i, int& j, int& k) {
+  const int brick = addr >> 6, local = addr & 63;
+  const int bz = brick % g.bdims[2];
+  const int by = (brick / g.bdims[2]) % g.bdims[1];
+  const int bx = brick / (g.bdims[2] * g.bdims[1]);
+  i = (bx << 2) + (local >> 4);
+  j = (by << 2) + ((local >> 2) & 3);
+  k = (bz << 2) + (local & 3);
+}
+
+__device__ __forceinline__ void load_m3(const float* base, int64_t cap, int64_t p, M3<float>& m) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) m.a[e / 3][e % 3] = base[e * cap + p];
+}
+
+__device__ __forceinline__ void store_m3(float* base, int64_t cap, int64_t p, const M3<float>& m) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) base[e * cap + p] = m.a[e / 3][e % 3];
+}
+
+// Kirchhoff stress of one particle (kernels.py:115-124): Hencky for von Mises and Drucker-Prager,
+// fixed corotated for snow (with hardening h = min(exp(xi (1 - Jp)), h_max)) and elastic.
+__device__ __forceinline__ M3<float> particle_tau(const MatDev& m, const M3<float>& F, float Jp) {
+  M3<float> U, V;
+  float sig[3];
+  svd3(F, U, sig, V);
+  if (m.model == kVonMises || m.model == kDruckerPrager) return hencky_tau(U, sig, m.mu, m.lam);
+  float mu = m.mu, lam = m.lam;
+  if (m.model == kSnow) {
+    const float h = fminf(expf(m.hardening * (1.0f - Jp)), m.max_hardening);
+    mu *= h;
+    lam *= h;
+  }
+  return fixed_corotated_tau(F, U, sig, V, mu, lam);
+}
+
+// ---------------------------------------------------------------------------------------------
+// P2G, reference data flow: one thread per particle, 27 x 4 FP32 global atomics (kernels.py:92-140).
+__global__ void __launch_bounds__(256) k_p2g_atomic(Particles P, const MatDev* __restrict__ mats, GridGeom g,
+                                                    StepConsts sc, int n, float dt, float4* __restrict__ accum) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t cap = P.cap;
+  const MatDev m = mats[P.mat[p]];
+  M3<float> F, Cm;
+  load_m3(P.F, cap, p, F);
+  load_m3(P.C, cap, p, Cm);
+  const M3<float> tau = particle_tau(m, F, P.Jp[p]);
+  const float pm = m.p_mass;
+  const float k = -dt * sc.p_vol * 4.0f * g.inv_dx * g.inv_dx;
+  M3<float> aff;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) aff.a[e / 3][e % 3] = k * tau.a[e / 3][e % 3] + pm * Cm.a[e / 3][e % 3];
+  float fx[3];
+  int base[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float xp = (P.x[a * cap + p] - g.origin[a]) * g.inv_dx;
+    base[a] = (int)floorf(xp - 0.5f);
+    fx[a] = xp - (float)base[a];
+  }
+  const float mv[3] = {pm * P.v[p], pm * P.v[cap + p], pm * P.v[2 * cap + p]};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float wi = bspline_w(fx[0], i);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wj = bspline_w(fx[1], j);
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const float w = wi * wj * bspline_w(fx[2], kk);
+        const float d0 = ((float)i - fx[0]) * g.dx, d1 = ((float)j - fx[1]) * g.dx, d2 = ((float)kk - fx[2]) * g.dx;
+        float4* node = accum + node_addr(g, base[0] + i, base[1] + j, base[2] + kk);
+        atomicAdd(&node->x, w * (mv[0] + (aff.a[0][0] * d0 + aff.a[0][1] * d1 + aff.a[0][2] * d2)));
+        atomicAdd(&node->y, w * (mv[1] + (aff.a[1][0] * d0 + aff.a[1][1] * d1 + aff.a[1][2] * d2)));
+        atomicAdd(&node->z, w * (mv[2] + (aff.a[2][0] * d0 + aff.a[2][1] * d1 + aff.a[2][2] * d2)));
+        atomicAdd(&node->w, w * pm);
+      }
+    }
+  }
+}
+
+// wall_contact (kernels.py:143-152): drop the into-wall component, Coulomb friction on the rest
+__device__ __forceinline__ void wall_contact(float (&vel)[3], int axis, float friction) {
+  const float vn = fabsf(vel[axis]);
+  vel[axis] = 0.0f;
+  const float len = sqrtf(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+  if (len > 0.0f) {
+    const float f = fmaxf(0.0f, 1.0f - friction * vn / len);
+    vel[0] *= f;
+    vel[1] *= f;
+    vel[2] *= f;
+  }
+}
+
+// grid_update (kernels.py:155-180) over one node: momentum -> velocity, gravity, walls by node index.
+__device__ __forceinline__ float4 update_node(const GridGeom& g, const StepConsts& sc, float dt, int i, int j, int k,
+                                              float4 a) {
+  const float m = a.w;
+  if (!(m > 0.0f)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float vel[3] = {a.x / m + dt * sc.gravity[0], a.y / m + dt * sc.gravity[1], a.z / m + dt * sc.gravity[2]};
+  const int node[3] = {i, j, k};
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    if (sc.lo_on[ax] != 0 && node[ax] <= sc.wall_lo_node[ax] && vel[ax] < 0.0f) wall_contact(vel, ax, sc.wall_friction);
+    if (sc.hi_on[ax] != 0 && node[ax] >= sc.wall_hi_node[ax] && vel[ax] > 0.0f) wall_contact(vel, ax, sc.wall_friction);
+  }
+  return make_float4(vel[0], vel[1], vel[2], m);
+}
+
+// Dense grid update over every node of the bricked grid; clears accum behind itself.
+__global__ void __launch_bounds__(256) k_grid_update_dense(GridGeom g, StepConsts sc, float dt, int n_nodes,
+                                                           float4* __restrict__ accum, float4* __restrict__ vel) {
+  const int addr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (addr >= n_nodes) return;
+  const float4 a = accum[addr];
+  if (a.w == 0.0f && a.x == 0.0f && a.y == 0.0f && a.z == 0.0f) {
+    vel[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  int i, j, k;
+  node_ijk(g, addr, i, j, k);
+  vel[addr] = update_node(g, sc, dt, i, j, k, a);
+  accum[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------------------------
+// G2P (kernels.py:183-263): gather v and C, advect with wall projection and finite-only clamp,
+// F <- (I + dt C) F, per-model return map.
+__device__ __forceinline__ void g2p_particle(Particles& P, const MatDev* __restrict__ mats, const GridGeom& g,
+                                             const StepConsts& sc, float dt, const float4* __restrict__ vel,
+                                             int64_t p) {
+  const int64_t cap = P.cap;
+  float xo[3], fx[3];
+  int base[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    xo[a] = P.x[a * cap + p];
+    const float xp = (xo[a] - g.origin[a]) * g.inv_dx;
+    base[a] = (int)floorf(xp - 0.5f);
+    fx[a] = xp - (float)base[a];
+  }
+  float nv[3] = {0.0f, 0.0f, 0.0f};
+  M3<float> nC;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) nC.a[e / 3][e % 3] = 0.0f;
+  const float c4 = 4.0f * g.inv_dx * g.inv_dx;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float wi = bspline_w(fx[0], i);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wj = bspline_w(fx[1], j);
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const float w = wi * wj * bspline_w(fx[2], kk);
+        const float dpos[3] = {((float)i - fx[0]) * g.dx, ((float)j - fx[1]) * g.dx, ((float)kk - fx[2]) * g.dx};
+        const float4 gv4 = vel[node_addr(g, base[0] + i, base[1] + j, base[2] + kk)];
+        const float gv[3] = {gv4.x, gv4.y, gv4.z};
+        const float wc = c4 * w;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          nv[a] += w * gv[a];
+#pragma unroll
+          for (int b = 0; b < 3; ++b) nC.a[a][b] += wc * gv[a] * dpos[b];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) P.v[a * cap + p] = nv[a];
+  store_m3(P.C, cap, p, nC);
+  float nx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nx[a] = xo[a] + dt * nv[a];
+    if (sc.lo_on[a] != 0 && nx[a] < sc.wall_lo[a]) nx[a] = sc.wall_lo[a];
+    if (sc.hi_on[a] != 0 && nx[a] > sc.wall_hi[a]) nx[a] = sc.wall_hi[a];
+  }
+  // a non-finite position is stored as it is so the NaN check of the runner sees it (kernels.py:236-241)
+  if (isfinite(nx[0]) && isfinite(nx[1]) && isfinite(nx[2])) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) nx[a] = fminf(fmaxf(nx[a], sc.clamp_lo[a]), sc.clamp_hi[a]);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) P.x[a * cap + p] = nx[a];
+  M3<float> Fo;
+  load_m3(P.F, cap, p, Fo);
+  M3<float> G;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) G.a[e / 3][e % 3] = (e % 4 == 0 ? 1.0f : 0.0f) + dt * nC.a[e / 3][e % 3];
+  const M3<float> Fn = mul(G, Fo);
+  const MatDev m = mats[P.mat[p]];
+  if (m.model == kElastic) {
+    store_m3(P.F, cap, p, Fn);
+    return;
+  }
+  M3<float> U, V;
+  float sig[3], ns[3];
+  svd3(Fn, U, sig, V);
+  if (m.model == kVonMises) {
+    project_von_mises(sig, m.mu, m.yield_stress, ns);
+  } else if (m.model == kDruckerPrager) {
+    project_drucker_prager(sig, m.mu, m.lam, m.alpha, ns);
+  } else {  // snow (Stomakhin et al. 2013)
+    const float lo = 1.0f - m.crit_c, hi = 1.0f + m.crit_s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ns[a] = fminf(fmaxf(sig[a], lo), hi);
+    P.Jp[p] = P.Jp[p] * (sig[0] * sig[1] * sig[2]) / (ns[0] * ns[1] * ns[2]);
+  }
+  store_m3(P.F, cap, p, usv(U, ns, V));
+}
+
+__global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restrict__ mats, GridGeom g,
+                                             StepConsts sc, int n, float dt, const float4* __restrict__ vel) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  g2p_particle(P, mats, g, sc, dt, vel, p);
+}
+
+// ---------------------------------------------------------------------------------------------
+// max |v| for the CFL step (kernels.py:266-269). Speeds are >= 0, so their IEEE bit patterns order
+// like unsigned integers; a NaN or inf speed yields a non-finite maximum, which the host treats as 0.
+__global__ void __launch_bounds__(256) k_max_speed(Particles P, int n, unsigned int* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  float s = 0.0f;
+  if (p < n) {
+    const float a = P.v[p], b = P.v[P.cap + p], c = P.v[2 * P.cap + p];
+    s = sqrtf(a * a + b * b + c * c);
+    if (!(s == s)) s = __int_as_float(0x7fc00000);
+  }
+  unsigned int u = __float_as_uint(s);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) u = max(u, (unsigned int)__shfl_xor((int)u, off, 64));
+  __shared__ unsigned int wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int b = wmax[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = max(b, wmax[w]);
+    atomicMax(out, b);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// State transfer between host layouts (array of structs in insertion order) and device planes.
+
+// dst[pid[p] * comps + c] = src[c * cap + p]
+__global__ void k_pack(const float* __restrict__ src, int64_t cap, const int32_t* __restrict__ pid, int comps, int n,
+                       float* __restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t q = pid ? pid[p] : p;
+  for (int c = 0; c < comps; ++c) dst[q * comps + c] = src[c * cap + p];
+}
+
+// dst[c * cap + (p0 + p)] = src[p * comps + c]
+__global__ void k_unpack(const float* __restrict__ src, int comps, int n, int64_t cap, int64_t p0,
+                         float* __restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int c = 0; c < comps; ++c) dst[c * cap + p0 + p] = src[(int64_t)p * comps + c];
+}
+
+// new particles: C = 0, F = I, Jp = 1, pid = insertion index
+__global__ void k_init_new(Particles P, int64_t p0, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = p0 + i;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    P.C[e * P.cap + p] = 0.0f;
+    P.F[e * P.cap + p] = (e % 4 == 0) ? 1.0f : 0.0f;
+  }
+  P.Jp[p] = 1.0f;
+  P.pid[p] = (int32_t)p;
+}
+
+// bricked grid <-> reference C-order (D0, D1, D2) arrays
+__global__ void k_grid_to_ref(GridGeom g, const float4* __restrict__ src, float* __restrict__ v3,
+                              float* __restrict__ m) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = g.dims[0] * g.dims[1] * g.dims[2];
+  if (idx >= total) return;
+  const int k = idx % g.dims[2], j = (idx / g.dims[2]) % g.dims[1], i = idx / (g.dims[2] * g.dims[1]);
+  const float4 a = src[node_addr(g, i, j, k)];
+  v3[3 * (int64_t)idx] = a.x;
+  v3[3 * (int64_t)idx + 1] = a.y;
+  v3[3 * (int64_t)idx + 2] = a.z;
+  m[idx] = a.w;
+}
+
+__global__ void k_grid_from_ref(GridGeom g, const float* __restrict__ v3, const float* __restrict__ m,
+                                float4* __restrict__ dst) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = g.dims[0] * g.dims[1] * g.dims[2];
+  if (idx >= total) return;
+  const int k = idx % g.dims[2], j = (idx / g.dims[2]) % g.dims[1], i = idx / (g.dims[2] * g.dims[1]);
+  dst[node_addr(g, i, j, k)] = make_float4(v3[3 * (int64_t)idx], v3[3 * (int64_t)idx + 1], v3[3 * (int64_t)idx + 2], m[idx]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// is_occupied (kernels.py:272-298): splat particle volume with B-spline weights, then test the
+// node nearest to each query point against half a cell volume.
+__global__ void __launch_bounds__(256) k_scatter_volume(Particles P, GridGeom g, float p_vol, int n,
+                                                        float* __restrict__ vol) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float fx[3];
+  int base[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float xp = (P.x[a * P.cap + p] - g.origin[a]) * g.inv_dx;
+    base[a] = (int)floorf(xp - 0.5f);
+    fx[a] = xp - (float)base[a];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        atomicAdd(vol + node_addr(g, base[0] + i, base[1] + j, base[2] + k),
+                  bspline_w(fx[0], i) * bspline_w(fx[1], j) * bspline_w(fx[2], k) * p_vol);
+}
+
+__global__ void k_query_fill(GridGeom g, const float* __restrict__ pts, int m, float cell_vol,
+                             const float* __restrict__ vol, uint8_t* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  int c[3];
+  bool inside = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    c[a] = (int)roundf((pts[3 * q + a] - g.origin[a]) * g.inv_dx);
+    inside = inside && c[a] >= 0 && c[a] < g.dims[a];
+  }
+  out[q] = (inside && vol[node_addr(g, c[0], c[1], c[2])] >= 0.5f * cell_vol) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Removal of particles past open sides (solver.py:222-240), in three passes: flag, scan, compact.
+__global__ void k_escape_flags(Particles P, int n, float3 lo, float3 hi, int32_t* __restrict__ keep_by_pid) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float x0 = P.x[p], x1 = P.x[P.cap + p], x2 = P.x[2 * P.cap + p];
+  const bool out = x0 < lo.x || x1 < lo.y || x2 < lo.z || x0 > hi.x || x1 > hi.y || x2 > hi.z;
+  keep_by_pid[P.pid[p]] = out ? 0 : 1;
+}
+
+}  // namespace mpm

@@ -1,0 +1,164 @@
+"""Synthetic workloads: the benchmark and parity scenes, built through the public interface.
+
+* ``elastic_block``: unit-cube domain at 64^3 (dx = 1/64), walls with friction 1e6 (sticky), a 0.2 m
+  elastic cube centred at (0.5, 0.6, 0.5) sampled at 8 particles per cell (26^3 = 17,576 particles,
+  jitter +-0.25 of the particle spacing), E = 1e4, nu = 0.3, rho = 1000, gravity (0, -9.8, 0),
+  dt = 1e-4 for 1000 substeps.
+* ``multi_material``: unit cube at 128^3, gravity (0, 0, -9.81), walls on; a von Mises sphere
+  (r 0.15 at (0.22, 0.5, 0.30), E 5e3, nu 0.45, yield 200, rho 1000), a Drucker-Prager column
+  ([0.4, 0.6]^2 x [0, 0.72], E 3.5e4, nu 0.3, 30 deg, rho 1600) and a snow sphere (r 0.16 at
+  (0.80, 0.5, 0.35), E 1.4e5, nu 0.2, xi 10, theta_c 2.5e-2, theta_s 7.5e-3, h_max 5, rho 400) thrown at
+  (-3, 0, 0) m/s: about 1.0 M particles at 8 per cell; one closed icosphere collider (2562 vertices,
+  r 0.1, friction 0.5) moving from (0.5, 0.5, 0.88) to (0.5, 0.5, 0.70) over 0.1 s; dt = 5e-5 for 2000
+  substeps (one substep per advance: the CFL step is about 6.6e-5).
+
+Particles are sampled with mpm_core.geometry.sample_mesh_volume, as the runner emits them.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from mpm_core.geometry import sample_mesh_volume, transform_points
+from mpm_core.materials import MaterialParams
+from mpm_core.scene import Domain, MeshData
+from mpm_core.solver import ColliderGeometry
+
+
+def _translate(x, y, z) -> np.ndarray:
+    m = np.eye(4, dtype=np.float32)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def _scale(x, y=None, z=None) -> np.ndarray:
+    return np.diag([x, x if y is None else y, x if z is None else z, 1.0]).astype(np.float32)
+
+
+def box_mesh() -> tuple[np.ndarray, np.ndarray]:
+    """Unit cube [0,1]^3 with outward winding."""
+    v = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], np.float32)
+    t = np.array([[0, 2, 1], [1, 2, 3], [4, 5, 6], [5, 7, 6], [0, 1, 4], [1, 5, 4],
+                  [2, 6, 3], [3, 6, 7], [0, 4, 2], [2, 4, 6], [1, 3, 5], [3, 7, 5]], np.int32)
+    return v, t
+
+
+def icosphere(subdivisions: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere: 10 * 4^s + 2 vertices, 20 * 4^s triangles."""
+    g = (1 + 5 ** 0.5) / 2
+    verts = [np.array(p, float) for p in
+             ([-1, g, 0], [1, g, 0], [-1, -g, 0], [1, -g, 0], [0, -1, g], [0, 1, g],
+              [0, -1, -g], [0, 1, -g], [g, 0, -1], [g, 0, 1], [-g, 0, -1], [-g, 0, 1])]
+    verts = [p / np.linalg.norm(p) for p in verts]
+    faces = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+             [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+             [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    for _ in range(subdivisions):
+        mids: dict[tuple[int, int], int] = {}
+
+        def mid(a: int, b: int) -> int:
+            key = (a, b) if a < b else (b, a)
+            if key not in mids:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                mids[key] = len(verts) - 1
+            return mids[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        faces = nf
+    return np.array(verts, np.float32), np.array(faces, np.int32)
+
+
+def box_points(lo, hi, spacing: float, seed) -> np.ndarray:
+    v, t = box_mesh()
+    m = _translate(*lo) @ _scale(*np.subtract(hi, lo))
+    return sample_mesh_volume(transform_points(m, v), t, spacing, seed)
+
+
+def sphere_points(center, radius: float, spacing: float, seed, subdivisions: int = 4) -> np.ndarray:
+    v, t = icosphere(subdivisions)
+    return sample_mesh_volume(v * np.float32(radius) + np.asarray(center, np.float32), t, spacing, seed)
+
+
+@dataclass
+class Workload:
+    """One scene: what to pass to create_solver, the particles to add, and the stepping."""
+
+    name: str
+    domain: Domain
+    materials: list[MaterialParams]
+    gravity: tuple[float, float, float]
+    positions: np.ndarray
+    velocities: np.ndarray
+    material_ids: np.ndarray
+    emitter_ids: np.ndarray
+    dt: float
+    substeps: int
+    colliders: list[ColliderGeometry] = field(default_factory=list)
+    collider_path: list[tuple[np.ndarray, np.ndarray]] = field(default_factory=list)  # (start, end) per collider
+    duration: float = 0.0
+
+    @property
+    def n(self) -> int:
+        return len(self.positions)
+
+    def collider_transforms(self, t0: float, t1: float) -> tuple[np.ndarray, np.ndarray]:
+        """(C,4,4) start/end world matrices at times t0 and t1 of a linear translation path."""
+        starts, ends = [], []
+        for (a, b), geom in zip(self.collider_path, self.colliders):
+            del geom
+            for t, out in ((t0, starts), (t1, ends)):
+                u = min(max(t / self.duration, 0.0), 1.0) if self.duration > 0 else 0.0
+                out.append(_translate(*(a + u * (b - a))) @ _scale(0.1))
+        return np.stack(starts), np.stack(ends)
+
+    def make_solver(self, backend: str = "hip", device: str | None = None):
+        from mpm_core.backends import create_solver
+
+        s = create_solver(backend, domain=self.domain, materials=self.materials, gravity=self.gravity,
+                          device=device)
+        s.add_particles(self.positions, self.velocities, self.material_ids, self.emitter_ids)
+        if self.colliders:
+            s.set_colliders(self.colliders)
+        return s
+
+
+def elastic_block(seed: int = 0) -> Workload:
+    domain = Domain((0, 0, 0), (1, 1, 1), 64, wall_friction=1.0e6)
+    mat = MaterialParams("elastic_block", "elastic", 1.0e4, 0.3, 1000.0)
+    x = box_points((0.4, 0.5, 0.4), (0.6, 0.7, 0.6), domain.dx / 2, seed)
+    n = len(x)
+    return Workload("elastic_block_64", domain, [mat], (0.0, -9.8, 0.0), x, np.zeros_like(x),
+                    np.zeros(n, np.int32), np.zeros(n, np.int32), dt=1.0e-4, substeps=1000)
+
+
+def multi_material(seed: int = 1, with_collider: bool = True) -> Workload:
+    domain = Domain((0, 0, 0), (1, 1, 1), 128)
+    mats = [
+        MaterialParams("plastic", "von_mises", 5.0e3, 0.45, 1000.0, yield_stress=200.0),
+        MaterialParams("sand", "drucker_prager", 3.5e4, 0.3, 1600.0, friction_angle_deg=30.0),
+        MaterialParams("snow", "snow", 1.4e5, 0.2, 400.0, hardening=10.0, critical_compression=2.5e-2,
+                       critical_stretch=7.5e-3, max_hardening=5.0),
+    ]
+    h = domain.dx / 2
+    bodies = [
+        sphere_points((0.22, 0.5, 0.30), 0.15, h, seed + 0),
+        box_points((0.4, 0.4, 0.0), (0.6, 0.6, 0.72), h, seed + 1),
+        sphere_points((0.80, 0.5, 0.35), 0.16, h, seed + 2),
+    ]
+    vels = [(0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (-3.0, 0.0, 0.0)]
+    x = np.concatenate(bodies)
+    v = np.concatenate([np.tile(np.asarray(u, np.float32), (len(b), 1)) for b, u in zip(bodies, vels)])
+    ids = np.concatenate([np.full(len(b), i, np.int32) for i, b in enumerate(bodies)])
+    w = Workload("multi_material_128", domain, mats, (0.0, 0.0, -9.81), x, v, ids, ids.copy(), dt=5.0e-5,
+                 substeps=2000, duration=0.1)
+    if with_collider:
+        verts, tris = icosphere(4)
+        w.colliders = [ColliderGeometry(MeshData(verts, tris), closed=True, friction=0.5)]
+        w.collider_path = [(np.array([0.5, 0.5, 0.88]), np.array([0.5, 0.5, 0.70]))]
+    return w
