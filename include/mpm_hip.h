@@ -134,6 +134,17 @@ int32_t mpm_set_grid(mpm_ctx* ctx, const float* grid_v, const float* grid_m);
 /* particle binning of the last substep: per-particle brick key, brick offsets, permutation */
 int32_t mpm_get_binning(mpm_ctx* ctx, int32_t* keys, int32_t* perm, int32_t* offsets, int64_t* n_bricks);
 
+/* run the brick binning (stable sort + reorder) now, as the next substep would (parity tests) */
+int32_t mpm_debug_sort(mpm_ctx* ctx);
+
+/* ---- options ------------------------------------------------------------------------------ */
+#define MPM_OPT_P2G_MODE 0       /* 0: binned LDS-window P2G (default); 1: one thread per particle,
+                                    27 x 4 global atomics (the reference's data flow) */
+#define MPM_OPT_SORT_INTERVAL 1  /* substeps between brick re-sorts (default 8; >= 1) */
+#define MPM_OPT_P2G_BLOCKS 2    /* workgroups of the binned P2G launch (0: automatic) */
+int32_t mpm_set_option(mpm_ctx* ctx, int32_t option, int64_t value);
+int64_t mpm_get_option(const mpm_ctx* ctx, int32_t option);
+
 /* ---- measurement -------------------------------------------------------------------------- */
 /* kernel ids for mpm_get_kernel_time */
 #define MPM_K_P2G 0

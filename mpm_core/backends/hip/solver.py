@@ -265,6 +265,17 @@ class HipSolver(Solver):
         N.check(self._lib.mpm_get_binning(self._ctx, N.iptr(keys), N.iptr(perm), N.iptr(offsets), C.byref(nb)))
         return keys, perm, offsets
 
+    def debug_sort(self) -> None:
+        """Bin the particles by brick now (stable radix sort + reorder), as the next substep would."""
+        N.check(self._lib.mpm_debug_sort(self._ctx))
+
+    def set_option(self, option: int, value: int) -> None:
+        """Tuning knobs of the library (mpm_plugin_amd._native.OPT_*)."""
+        N.check(self._lib.mpm_set_option(self._ctx, int(option), int(value)))
+
+    def get_option(self, option: int) -> int:
+        return int(self._lib.mpm_get_option(self._ctx, int(option)))
+
     def set_timing(self, enabled: bool) -> None:
         N.check(self._lib.mpm_set_timing(self._ctx, int(bool(enabled))))
 

@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "libmpm_hip.so")
 ABI_VERSION = 1
 OK, ERR_VALUE, ERR_NOT_IMPLEMENTED, ERR_MEMORY, ERR_RUNTIME = 0, -1, -2, -3, -4
 K_P2G, K_GRID, K_G2P, K_BIN, K_COLLIDERS, K_OTHER = range(6)
+OPT_P2G_MODE, OPT_SORT_INTERVAL, OPT_P2G_BLOCKS = range(3)
+P2G_BINNED, P2G_ATOMIC = 0, 1
 KERNEL_NAMES = ("p2g", "grid", "g2p", "bin", "colliders", "other")
 
 
@@ -73,6 +75,9 @@ def _declare(L: C.CDLL) -> None:
         "mpm_get_grid": ([ctx, i32, fp, fp], i32),
         "mpm_set_grid": ([ctx, fp, fp], i32),
         "mpm_get_binning": ([ctx, ip, ip, ip, i64p], i32),
+        "mpm_debug_sort": ([ctx], i32),
+        "mpm_set_option": ([ctx, i32, i64], i32),
+        "mpm_get_option": ([ctx, i32], i64),
         "mpm_set_timing": ([ctx, i32], i32),
         "mpm_get_kernel_time": ([ctx, i32, C.POINTER(C.c_double), i64p], i32),
         "mpm_reset_timing": ([ctx], i32),

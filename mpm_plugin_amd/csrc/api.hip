@@ -2,7 +2,14 @@
 //
 // One context = one simulation domain on one GPU with its own HIP stream. The host side of the
 // reference solver (src/mpm_core/backends/warp/solver.py) maps onto the entry points as documented in
-// the header; the kernels live in kernels.hpp.
+// the header; the kernels live in kernels.hpp, binning.hpp, colliders.hpp and substep.hpp.
+//
+// Substep on the default (binned) path:
+//   [every sort_interval substeps, or after any state change]
+//     k_brick_keys -> radix passes (k_radix_hist, scan, k_radix_scatter) -> k_brick_bounds -> k_reorder
+//   k_p2g_binned   (LDS windows per occupied brick)
+//   k_grid_bricks  (dirty bricks: update, walls, colliders)
+//   k_g2p          (gather, advection, collider projection, return map, stress, max speed)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,7 +21,8 @@
 #include <vector>
 
 #include "../../include/mpm_hip.h"
-#include "kernels.hpp"
+#include "binning.hpp"
+#include "substep.hpp"
 
 using namespace mpm;
 
@@ -53,9 +61,9 @@ void dev_free(void* p) {
 
 constexpr int kBlock = 256;
 
-inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
 
-// device scratch that grows on demand
+// device buffer that grows on demand (contents are not kept)
 struct Scratch {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -69,7 +77,11 @@ struct Scratch {
     }
     return ptr;
   }
-  ~Scratch() { dev_free(ptr); }
+  void release() {
+    dev_free(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
 };
 
 struct Timing {
@@ -84,6 +96,35 @@ struct Timing {
   int64_t launches[MPM_K_COUNT] = {};
 };
 
+// all per-slot particle planes of one buffer set, in one allocation:
+// x 3, v 3, C 9, F 9, Jp 1, tau 6 float planes, then mat, pid int planes
+constexpr int kFloatPlanes = 31;
+constexpr int kIntPlanes = 2;
+
+struct ParticleSet {
+  char* base = nullptr;
+  int64_t cap = 0;
+  float* x() const { return reinterpret_cast<float*>(base); }
+  float* v() const { return x() + 3 * cap; }
+  float* C() const { return x() + 6 * cap; }
+  float* F() const { return x() + 15 * cap; }
+  float* Jp() const { return x() + 24 * cap; }
+  float* tau() const { return x() + 25 * cap; }
+  int32_t* mat() const { return reinterpret_cast<int32_t*>(x() + kFloatPlanes * cap); }
+  int32_t* pid() const { return mat() + cap; }
+  Particles view() const { return Particles{x(), v(), C(), F(), Jp(), mat(), pid(), cap}; }
+  static size_t bytes(int64_t cap) { return (size_t)cap * 4 * (kFloatPlanes + kIntPlanes); }
+};
+
+struct ColliderHost {
+  std::vector<TriGeom> tris;
+  std::vector<TriNormals> normals;
+  ColliderDev dev{};
+  float built_pad = -1.0f;
+  std::vector<uint32_t> cell_start;  // relative to this collider's candidate list
+  std::vector<uint32_t> cell_tris;
+};
+
 }  // namespace
 
 struct mpm_ctx {
@@ -93,24 +134,59 @@ struct mpm_ctx {
   std::vector<mpm_material_desc> mats;
   MatDev* d_mats = nullptr;
 
-  // particles
+  // particles: set A is live, set B is the reorder target; emitter ids stay in insertion order
   int64_t n = 0, cap = 0;
-  float *x = nullptr, *v = nullptr, *C = nullptr, *F = nullptr, *Jp = nullptr;
-  int32_t *mat = nullptr, *emitter = nullptr, *pid = nullptr;
-  std::vector<char> present;  // material index -> any particle of it exists
+  ParticleSet A, B;
+  int32_t* emitter = nullptr;
+  std::vector<char> present;
 
   // grid
   GridGeom geom{};
   StepConsts sc{};
-  int n_nodes = 0;  // nodes in the bricked allocation
+  int n_nodes = 0, n_bricks = 0, key_bits = 1;
   float4 *accum = nullptr, *vel = nullptr;
+  uint32_t* dirty = nullptr;
+  uint8_t* live = nullptr;
+
+  // binning
+  uint32_t *keys0 = nullptr, *keys1 = nullptr, *keys2 = nullptr, *vals1 = nullptr, *vals2 = nullptr;
+  uint32_t *offsets = nullptr, *active = nullptr, *n_active = nullptr;
+  Scratch hist, scan_sums;
+  bool sorted = false;
+  int64_t steps_since_sort = 0;
+  uint32_t* last_perm = nullptr;  // the sorted values of the last sort (points into vals1/vals2)
+
+  // options
+  int p2g_mode = 0;
+  int64_t sort_interval = 8;
+  int64_t p2g_blocks = 0;
+
+  // colliders
+  std::vector<ColliderHost> colliders;
+  ColliderDev* d_cols = nullptr;
+  TriGeom* d_tris = nullptr;
+  TriNormals* d_normals = nullptr;
+  uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
 
   unsigned int* d_vmax = nullptr;
-  Scratch scratch, scratch2;
+  bool vmax_valid = false;
+  Scratch scratch;
   Timing timing;
   double frame_dt = 1.0;
 
-  Particles parts() const { return Particles{x, v, C, F, Jp, mat, pid, cap}; }
+  Particles parts() const { return A.view(); }
+  ColliderSet collider_set() const {
+    ColliderSet cs{};
+    cs.count = (int)colliders.size();
+    cs.cols = d_cols;
+    cs.tris = d_tris;
+    cs.normals = d_normals;
+    cs.cell_start = d_cell_start;
+    cs.cell_tris = d_cell_tris;
+    cs.thickness = (float)(2.0 * dom.dx_f64);
+    cs.band = (float)dom.dx_f64;
+    return cs;
+  }
 };
 
 namespace {
@@ -173,11 +249,16 @@ void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
 
 // ---- particle storage ------------------------------------------------------------------------
 void free_particles(mpm_ctx* c) {
-  for (void* p : {(void*)c->x, (void*)c->v, (void*)c->C, (void*)c->F, (void*)c->Jp, (void*)c->mat,
-                  (void*)c->emitter, (void*)c->pid})
+  dev_free(c->A.base);
+  dev_free(c->B.base);
+  for (void* p : {(void*)c->emitter, (void*)c->keys0, (void*)c->keys1, (void*)c->keys2, (void*)c->vals1,
+                  (void*)c->vals2})
     dev_free(p);
-  c->x = c->v = c->C = c->F = c->Jp = nullptr;
-  c->mat = c->emitter = c->pid = nullptr;
+  c->A = ParticleSet{};
+  c->B = ParticleSet{};
+  c->emitter = nullptr;
+  c->keys0 = c->keys1 = c->keys2 = c->vals1 = c->vals2 = nullptr;
+  c->last_perm = nullptr;
   c->cap = 0;
 }
 
@@ -186,39 +267,38 @@ void reserve(mpm_ctx* c, int64_t need) {
   if (need <= c->cap) return;
   int64_t cap = std::max<int64_t>(need, c->cap + c->cap / 2);
   cap = (cap + 255) & ~int64_t(255);
-  float* x = dev_alloc<float>(3 * cap);
-  float* v = dev_alloc<float>(3 * cap);
-  float* C = dev_alloc<float>(9 * cap);
-  float* F = dev_alloc<float>(9 * cap);
-  float* Jp = dev_alloc<float>(cap);
-  int32_t* mat = dev_alloc<int32_t>(cap);
+  ParticleSet A, B;
+  A.cap = B.cap = cap;
+  A.base = dev_alloc<char>(ParticleSet::bytes(cap));
+  B.base = dev_alloc<char>(ParticleSet::bytes(cap));
   int32_t* em = dev_alloc<int32_t>(cap);
-  int32_t* pid = dev_alloc<int32_t>(cap);
+  uint32_t* k0 = dev_alloc<uint32_t>(cap);
+  uint32_t* k1 = dev_alloc<uint32_t>(cap);
+  uint32_t* k2 = dev_alloc<uint32_t>(cap);
+  uint32_t* v1 = dev_alloc<uint32_t>(cap);
+  uint32_t* v2 = dev_alloc<uint32_t>(cap);
   if (c->n) {
-    auto planes = [&](float* dst, const float* src, int comps) {
-      for (int k = 0; k < comps; ++k)
-        HIP_CHECK(hipMemcpyAsync(dst + k * cap, src + k * c->cap, c->n * sizeof(float), hipMemcpyDeviceToDevice,
-                                 c->stream));
-    };
-    planes(x, c->x, 3);
-    planes(v, c->v, 3);
-    planes(C, c->C, 9);
-    planes(F, c->F, 9);
-    planes(Jp, c->Jp, 1);
-    HIP_CHECK(hipMemcpyAsync(mat, c->mat, c->n * 4, hipMemcpyDeviceToDevice, c->stream));
+    for (int k = 0; k < kFloatPlanes + kIntPlanes; ++k)
+      HIP_CHECK(hipMemcpyAsync(A.base + (size_t)k * cap * 4, c->A.base + (size_t)k * c->cap * 4, c->n * 4,
+                               hipMemcpyDeviceToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(em, c->emitter, c->n * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIP_CHECK(hipMemcpyAsync(pid, c->pid, c->n * 4, hipMemcpyDeviceToDevice, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
   const int64_t n = c->n;
   free_particles(c);
-  c->x = x; c->v = v; c->C = C; c->F = F; c->Jp = Jp;
-  c->mat = mat; c->emitter = em; c->pid = pid;
+  c->A = A;
+  c->B = B;
+  c->emitter = em;
+  c->keys0 = k0;
+  c->keys1 = k1;
+  c->keys2 = k2;
+  c->vals1 = v1;
+  c->vals2 = v2;
   c->cap = cap;
   c->n = n;
 }
 
-// upload `n` host AoS rows of `comps` floats into planes starting at slot p0
+// upload `n` host rows of `comps` floats into planes starting at slot p0
 void upload_planes(mpm_ctx* c, float* planes, const float* host, int comps, int64_t n, int64_t p0) {
   if (n == 0) return;
   float* stage = static_cast<float*>(c->scratch.get(n * comps * sizeof(float)));
@@ -227,10 +307,11 @@ void upload_planes(mpm_ctx* c, float* planes, const float* host, int comps, int6
   check_launch("k_unpack");
 }
 
+// download planes into host rows in insertion order (through pid)
 void download_planes(mpm_ctx* c, const float* planes, float* host, int comps) {
   if (c->n == 0) return;
   float* stage = static_cast<float*>(c->scratch.get(c->n * comps * sizeof(float)));
-  k_pack<<<grid_for(c->n), kBlock, 0, c->stream>>>(planes, c->cap, c->pid, comps, (int)c->n, stage);
+  k_pack<<<grid_for(c->n), kBlock, 0, c->stream>>>(planes, c->cap, c->A.pid(), comps, (int)c->n, stage);
   check_launch("k_pack");
   HIP_CHECK(hipMemcpyAsync(host, stage, c->n * comps * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -241,44 +322,127 @@ void check_materials(mpm_ctx* c, const int32_t* mat, int64_t n) {
     if (mat[i] < 0 || mat[i] >= (int32_t)c->mats.size()) fail(MPM_ERR_VALUE, "material id out of range");
 }
 
+void state_changed(mpm_ctx* c) {
+  c->sorted = false;
+  c->vmax_valid = false;
+}
+
+// ---- device-wide exclusive scan --------------------------------------------------------------
+void scan_u32(mpm_ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles > kScanTile) fail(MPM_ERR_VALUE, "scan too large");
+  uint32_t* sums = static_cast<uint32_t*>(c->scan_sums.get((tiles + 1) * sizeof(uint32_t)));
+  k_scan_tiles<<<(unsigned)tiles, kScanThreads, 0, c->stream>>>(in, (int)n, out, sums);
+  check_launch("k_scan_tiles");
+  if (tiles > 1) {
+    k_scan_top<<<1, kScanThreads, 0, c->stream>>>(sums, (int)tiles);
+    check_launch("k_scan_top");
+    k_scan_add<<<(unsigned)tiles, kScanThreads, 0, c->stream>>>(out, (int)n, sums);
+    check_launch("k_scan_add");
+  }
+}
+
+// ---- binning ---------------------------------------------------------------------------------
+void sort_particles(mpm_ctx* c) {
+  const int64_t n = c->n;
+  Timed t(c, MPM_K_BIN);
+  HIP_CHECK(hipMemsetAsync(c->n_active, 0, sizeof(uint32_t), c->stream));
+  if (n == 0) {
+    k_brick_bounds<<<1, kBlock, 0, c->stream>>>(c->keys1, 0, c->n_bricks, c->offsets, c->active, c->n_active);
+    check_launch("k_brick_bounds");
+    c->sorted = true;
+    c->steps_since_sort = 0;
+    return;
+  }
+  k_brick_keys<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)n, c->keys0, c->vals2);
+  check_launch("k_brick_keys");
+  const int passes = (c->key_bits + 7) / 8;
+  const int tiles = (int)((n + kSortTile - 1) / kSortTile);
+  uint32_t* hist = static_cast<uint32_t*>(c->hist.get((size_t)256 * tiles * sizeof(uint32_t)));
+  const uint32_t* kin = c->keys0;
+  const uint32_t* vin = c->vals2;
+  for (int pass = 0; pass < passes; ++pass) {
+    uint32_t* kout = (pass % 2 == 0) ? c->keys1 : c->keys2;
+    uint32_t* vout = (pass % 2 == 0) ? c->vals1 : c->vals2;
+    k_radix_hist<<<tiles, kSortThreads, 0, c->stream>>>(kin, (int)n, 8 * pass, tiles, hist);
+    check_launch("k_radix_hist");
+    scan_u32(c, hist, hist, (int64_t)256 * tiles);
+    k_radix_scatter<<<tiles, kSortThreads, 0, c->stream>>>(kin, vin, (int)n, 8 * pass, tiles, hist, kout, vout);
+    check_launch("k_radix_scatter");
+    kin = kout;
+    vin = vout;
+  }
+  k_brick_bounds<<<grid_for(n + 1), kBlock, 0, c->stream>>>(kin, (int)n, c->n_bricks, c->offsets, c->active,
+                                                            c->n_active);
+  check_launch("k_brick_bounds");
+  k_reorder<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.view(), c->B.view(), c->A.tau(), c->B.tau(), vin, (int)n);
+  check_launch("k_reorder");
+  std::swap(c->A, c->B);
+  c->last_perm = const_cast<uint32_t*>(vin);
+  c->sorted = true;
+  c->steps_since_sort = 0;
+}
+
 // ---- substep phases --------------------------------------------------------------------------
 void run_p2g(mpm_ctx* c, float dt) {
+  if (c->p2g_mode == 1) {
+    if (c->n == 0) return;
+    Timed t(c, MPM_K_P2G);
+    k_p2g_atomic<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt,
+                                                           c->accum);
+    check_launch("k_p2g_atomic");
+    return;
+  }
+  if (!c->sorted || c->steps_since_sort >= c->sort_interval) sort_particles(c);
   if (c->n == 0) return;
   Timed t(c, MPM_K_P2G);
-  k_p2g_atomic<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt,
-                                                         c->accum);
-  check_launch("k_p2g_atomic");
+  int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 4096;
+  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, std::min<int64_t>(c->n_bricks, c->n)));
+  k_p2g_binned<<<(unsigned)blocks, kP2GThreads, 0, c->stream>>>(c->parts(), c->A.tau(), c->d_mats, c->geom,
+                                                                c->sc.p_vol, dt, c->offsets, c->active, c->n_active,
+                                                                c->accum, c->dirty);
+  check_launch("k_p2g_binned");
 }
 
-void run_grid(mpm_ctx* c, float dt) {
+void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   Timed t(c, MPM_K_GRID);
-  k_grid_update_dense<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->sc, dt, c->n_nodes, c->accum,
-                                                                        c->vel);
-  check_launch("k_grid_update_dense");
+  if (c->p2g_mode == 1) {
+    k_grid_dense<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->sc, dt, c->n_nodes, c->accum, c->vel,
+                                                                  c->live, c->collider_set(), s, inv_frame_dt,
+                                                                  c->d_vmax);
+    check_launch("k_grid_dense");
+  } else {
+    const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
+    k_grid_bricks<<<blocks, 256, 0, c->stream>>>(c->geom, c->sc, dt, c->n_bricks, c->dirty, c->live, c->accum,
+                                                 c->vel, c->collider_set(), s, inv_frame_dt, c->d_vmax);
+    check_launch("k_grid_bricks");
+  }
 }
 
-void run_g2p(mpm_ctx* c, float dt) {
+void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   if (c->n == 0) return;
   Timed t(c, MPM_K_G2P);
-  k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel);
+  k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel,
+                                                  c->A.tau(), c->d_vmax, c->collider_set(), s, inv_frame_dt);
   check_launch("k_g2p");
+  c->vmax_valid = true;
 }
 
 void substep(mpm_ctx* c, double dt, double s, double frame_dt) {
-  (void)s;
-  (void)frame_dt;
-  const float dtf = (float)dt;
+  const float dtf = (float)dt, sf = (float)s, inv_fdt = (float)(1.0 / frame_dt);
   run_p2g(c, dtf);
-  run_grid(c, dtf);
-  run_g2p(c, dtf);
+  run_grid(c, dtf, sf, inv_fdt);
+  run_g2p(c, dtf, sf, inv_fdt);
+  c->steps_since_sort += 1;
 }
 
 float max_speed(mpm_ctx* c) {
-  HIP_CHECK(hipMemsetAsync(c->d_vmax, 0, sizeof(unsigned int), c->stream));
-  {
+  if (!c->vmax_valid) {
+    HIP_CHECK(hipMemsetAsync(c->d_vmax, 0, sizeof(unsigned int), c->stream));
     Timed t(c, MPM_K_OTHER);
     k_max_speed<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), (int)c->n, c->d_vmax);
     check_launch("k_max_speed");
+    c->vmax_valid = true;
   }
   unsigned int bits = 0;
   HIP_CHECK(hipMemcpyAsync(&bits, c->d_vmax, sizeof bits, hipMemcpyDeviceToHost, c->stream));
@@ -286,15 +450,6 @@ float max_speed(mpm_ctx* c) {
   float f;
   std::memcpy(&f, &bits, sizeof f);
   return f;
-}
-
-void refresh_present(mpm_ctx* c) {
-  std::fill(c->present.begin(), c->present.end(), 0);
-  if (c->n == 0) return;
-  std::vector<int32_t> m(c->n);
-  HIP_CHECK(hipMemcpyAsync(m.data(), c->mat, c->n * 4, hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipStreamSynchronize(c->stream));
-  for (int32_t id : m) c->present[id] = 1;
 }
 
 void load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const float* Cm, const float* F,
@@ -305,22 +460,33 @@ void load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const flo
   if (n > (int64_t)INT32_MAX) fail(MPM_ERR_VALUE, "too many particles");
   c->n = 0;
   reserve(c, n);
-  upload_planes(c, c->x, x, 3, n, 0);
-  upload_planes(c, c->v, v, 3, n, 0);
-  upload_planes(c, c->C, Cm, 9, n, 0);
-  upload_planes(c, c->F, F, 9, n, 0);
-  upload_planes(c, c->Jp, Jp, 1, n, 0);
+  upload_planes(c, c->A.x(), x, 3, n, 0);
+  upload_planes(c, c->A.v(), v, 3, n, 0);
+  upload_planes(c, c->A.C(), Cm, 9, n, 0);
+  upload_planes(c, c->A.F(), F, 9, n, 0);
+  upload_planes(c, c->A.Jp(), Jp, 1, n, 0);
   if (n) {
-    HIP_CHECK(hipMemcpyAsync(c->mat, mat, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->A.mat(), mat, n * 4, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(c->emitter, emitter, n * 4, hipMemcpyHostToDevice, c->stream));
-    std::vector<int32_t> ids(n);
-    for (int64_t i = 0; i < n; ++i) ids[i] = (int32_t)i;
-    HIP_CHECK(hipMemcpyAsync(c->pid, ids.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    k_iota<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.pid(), (int)n);
+    check_launch("k_iota");
+    k_stress<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, (int)n, c->A.tau());
+    check_launch("k_stress");
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
   c->n = n;
   std::fill(c->present.begin(), c->present.end(), 0);
   for (int64_t i = 0; i < n; ++i) c->present[mat[i]] = 1;
+  state_changed(c);
+}
+
+void refresh_present(mpm_ctx* c) {
+  std::fill(c->present.begin(), c->present.end(), 0);
+  if (c->n == 0) return;
+  std::vector<int32_t> m(c->n);
+  HIP_CHECK(hipMemcpyAsync(m.data(), c->A.mat(), c->n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (int32_t id : m) c->present[id] = 1;
 }
 
 // drop particles more than one cell past an open side; keeps insertion order of the survivors
@@ -330,7 +496,7 @@ int64_t remove_escaped(mpm_ctx* c) {
   for (int a = 0; a < 3; ++a) any_open = any_open || !d.lo_on[a] || !d.hi_on[a];
   if (!any_open || c->n == 0) return 0;
   const int64_t n = c->n;
-  int32_t* keep = static_cast<int32_t*>(c->scratch2.get(n * sizeof(int32_t)));
+  int32_t* keep = static_cast<int32_t*>(c->scratch.get(n * sizeof(int32_t)));
   k_escape_flags<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), (int)n,
                                                         make_float3(d.remove_lo[0], d.remove_lo[1], d.remove_lo[2]),
                                                         make_float3(d.remove_hi[0], d.remove_hi[1], d.remove_hi[2]),
@@ -347,13 +513,14 @@ int64_t remove_escaped(mpm_ctx* c) {
   // also round-trips the state here (solver.py:237-239)
   std::vector<float> x(3 * n), v(3 * n), C(9 * n), F(9 * n), Jp(n);
   std::vector<int32_t> mat(n), em(n);
-  download_planes(c, c->x, x.data(), 3);
-  download_planes(c, c->v, v.data(), 3);
-  download_planes(c, c->C, C.data(), 9);
-  download_planes(c, c->F, F.data(), 9);
-  download_planes(c, c->Jp, Jp.data(), 1);
-  download_planes(c, reinterpret_cast<float*>(c->mat), reinterpret_cast<float*>(mat.data()), 1);
-  download_planes(c, reinterpret_cast<float*>(c->emitter), reinterpret_cast<float*>(em.data()), 1);
+  download_planes(c, c->A.x(), x.data(), 3);
+  download_planes(c, c->A.v(), v.data(), 3);
+  download_planes(c, c->A.C(), C.data(), 9);
+  download_planes(c, c->A.F(), F.data(), 9);
+  download_planes(c, c->A.Jp(), Jp.data(), 1);
+  download_planes(c, reinterpret_cast<float*>(c->A.mat()), reinterpret_cast<float*>(mat.data()), 1);
+  HIP_CHECK(hipMemcpyAsync(em.data(), c->emitter, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   int64_t w = 0;
   for (int64_t i = 0; i < n; ++i) {
     if (!flags[i]) continue;
@@ -368,6 +535,160 @@ int64_t remove_escaped(mpm_ctx* c) {
   }
   load_state(c, kept, x.data(), v.data(), C.data(), F.data(), Jp.data(), mat.data(), em.data());
   return removed;
+}
+
+// ---- colliders -------------------------------------------------------------------------------
+void upload_colliders(mpm_ctx* c) {
+  for (void* p : {(void*)c->d_cols, (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start,
+                  (void*)c->d_cell_tris})
+    dev_free(p);
+  c->d_cols = nullptr;
+  c->d_tris = nullptr;
+  c->d_normals = nullptr;
+  c->d_cell_start = nullptr;
+  c->d_cell_tris = nullptr;
+  if (c->colliders.empty()) return;
+  std::vector<ColliderDev> cols;
+  std::vector<TriGeom> tris;
+  std::vector<TriNormals> normals;
+  std::vector<uint32_t> starts, cand;
+  for (auto& h : c->colliders) {
+    ColliderDev d = h.dev;
+    d.tri_offset = (int)tris.size();
+    d.cell_offset = (int)starts.size();
+    tris.insert(tris.end(), h.tris.begin(), h.tris.end());
+    normals.insert(normals.end(), h.normals.begin(), h.normals.end());
+    const uint32_t base = (uint32_t)cand.size();
+    const size_t cells = h.cell_start.empty() ? 0 : h.cell_start.size() - 1;
+    for (size_t i = 0; i < cells; ++i) starts.push_back(base + h.cell_start[i]);
+    cand.insert(cand.end(), h.cell_tris.begin(), h.cell_tris.end());
+    cols.push_back(d);
+  }
+  starts.push_back((uint32_t)cand.size());
+  c->d_cols = dev_alloc<ColliderDev>(cols.size());
+  c->d_tris = dev_alloc<TriGeom>(tris.size());
+  c->d_normals = dev_alloc<TriNormals>(normals.size());
+  c->d_cell_start = dev_alloc<uint32_t>(starts.size());
+  c->d_cell_tris = dev_alloc<uint32_t>(cand.size());
+  HIP_CHECK(hipMemcpy(c->d_cols, cols.data(), cols.size() * sizeof(ColliderDev), hipMemcpyHostToDevice));
+  if (!tris.empty()) {
+    HIP_CHECK(hipMemcpy(c->d_tris, tris.data(), tris.size() * sizeof(TriGeom), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->d_normals, normals.data(), normals.size() * sizeof(TriNormals), hipMemcpyHostToDevice));
+  }
+  HIP_CHECK(hipMemcpy(c->d_cell_start, starts.data(), starts.size() * 4, hipMemcpyHostToDevice));
+  if (!cand.empty()) HIP_CHECK(hipMemcpy(c->d_cell_tris, cand.data(), cand.size() * 4, hipMemcpyHostToDevice));
+}
+
+// candidate-triangle voxel grid of one collider over its AABB grown by `pad` (local units)
+void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
+  ColliderDev& d = h.dev;
+  h.cell_start.clear();
+  h.cell_tris.clear();
+  if (d.n_tris == 0) {
+    d.g_res[0] = d.g_res[1] = d.g_res[2] = 0;
+    h.built_pad = pad;
+    return;
+  }
+  float ext[3], emax = 0.0f;
+  for (int a = 0; a < 3; ++a) {
+    ext[a] = (d.aabb_hi[a] - d.aabb_lo[a]) + 2.0f * pad;
+    emax = std::max(emax, ext[a]);
+  }
+  const float hvox = std::max(emax / 32.0f, 1e-12f);
+  int64_t cells = 1;
+  for (int a = 0; a < 3; ++a) {
+    d.g_res[a] = std::min(64, std::max(1, (int)std::ceil(ext[a] / hvox)));
+    d.g_lo[a] = d.aabb_lo[a] - pad;
+    cells *= d.g_res[a];
+  }
+  d.g_inv_h = 1.0f / hvox;
+  TriGeom* dtris = dev_alloc<TriGeom>(h.tris.size());
+  uint32_t* counts = dev_alloc<uint32_t>(cells + 1);
+  uint32_t* starts = dev_alloc<uint32_t>(cells + 1);
+  ColliderDev local = d;
+  local.tri_offset = 0;
+  HIP_CHECK(hipMemcpyAsync(dtris, h.tris.data(), h.tris.size() * sizeof(TriGeom), hipMemcpyHostToDevice, c->stream));
+  k_voxel_candidates<<<grid_for(cells), kBlock, 0, c->stream>>>(local, dtris, (int)cells, 0, counts, nullptr, nullptr);
+  check_launch("k_voxel_candidates");
+  std::vector<uint32_t> cnt(cells);
+  HIP_CHECK(hipMemcpyAsync(cnt.data(), counts, cells * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  h.cell_start.assign(cells + 1, 0);
+  for (int64_t i = 0; i < cells; ++i) h.cell_start[i + 1] = h.cell_start[i] + cnt[i];
+  const uint32_t total = h.cell_start[cells];
+  uint32_t* out = dev_alloc<uint32_t>(std::max<uint32_t>(total, 1));
+  HIP_CHECK(hipMemcpyAsync(starts, h.cell_start.data(), (cells + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  k_voxel_candidates<<<grid_for(cells), kBlock, 0, c->stream>>>(local, dtris, (int)cells, 1, nullptr, starts, out);
+  check_launch("k_voxel_candidates");
+  h.cell_tris.resize(total);
+  if (total) HIP_CHECK(hipMemcpyAsync(h.cell_tris.data(), out, total * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  dev_free(dtris);
+  dev_free(counts);
+  dev_free(starts);
+  dev_free(out);
+  h.built_pad = pad;
+}
+
+// triangle corners and pseudonormals of one mesh (double precision, stored as float)
+void mesh_tables(const float* verts, int nv, const int32_t* tris, int nt, ColliderHost& h) {
+  std::vector<double> fn(3 * (size_t)nt, 0.0), vn(3 * (size_t)nv, 0.0);
+  std::vector<std::pair<uint64_t, int>> edges;
+  edges.reserve(3 * (size_t)nt);
+  for (int t = 0; t < nt; ++t) {
+    const int32_t* tri = tris + 3 * t;
+    double P[3][3];
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < 3; ++i) P[k][i] = verts[3 * tri[k] + i];
+    const double e1[3] = {P[1][0] - P[0][0], P[1][1] - P[0][1], P[1][2] - P[0][2]};
+    const double e2[3] = {P[2][0] - P[0][0], P[2][1] - P[0][1], P[2][2] - P[0][2]};
+    double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (len > 0)
+      for (double& x : n) x /= len;
+    for (int i = 0; i < 3; ++i) fn[3 * t + i] = n[i];
+    for (int k = 0; k < 3; ++k) {
+      const double* p0 = P[k];
+      const double* p1 = P[(k + 1) % 3];
+      const double* p2 = P[(k + 2) % 3];
+      const double u[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+      const double w[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+      const double lu = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      const double lw = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+      double ang = 0.0;
+      if (lu > 0 && lw > 0)
+        ang = std::acos(std::min(1.0, std::max(-1.0, (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / (lu * lw))));
+      for (int i = 0; i < 3; ++i) vn[3 * tri[k] + i] += ang * n[i];
+      const int a = tri[k], b = tri[(k + 1) % 3];
+      edges.push_back({(uint64_t(std::min(a, b)) << 32) | uint64_t(std::max(a, b)), 3 * t + k});
+    }
+  }
+  std::vector<double> en(9 * (size_t)nt, 0.0);
+  std::sort(edges.begin(), edges.end());
+  for (size_t i = 0; i < edges.size();) {
+    size_t j = i;
+    double s[3] = {0, 0, 0};
+    while (j < edges.size() && edges[j].first == edges[i].first) {
+      const int t = edges[j].second / 3;
+      for (int d = 0; d < 3; ++d) s[d] += fn[3 * t + d];
+      ++j;
+    }
+    for (size_t k = i; k < j; ++k)
+      for (int d = 0; d < 3; ++d) en[3 * edges[k].second + d] = s[d];
+    i = j;
+  }
+  h.tris.resize(nt);
+  h.normals.resize(nt);
+  for (int t = 0; t < nt; ++t) {
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < 3; ++i) h.tris[t].v[3 * k + i] = verts[3 * tris[3 * t + k] + i];
+    float* o = h.normals[t].n;
+    for (int i = 0; i < 3; ++i) o[i] = (float)fn[3 * t + i];
+    for (int e = 0; e < 3; ++e)
+      for (int i = 0; i < 3; ++i) o[3 + 3 * e + i] = (float)en[3 * (3 * t + e) + i];
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < 3; ++i) o[12 + 3 * k + i] = (float)vn[3 * tris[3 * t + k] + i];
+  }
 }
 
 template <class Fn> int32_t run(mpm_ctx* c, Fn&& fn) {
@@ -389,6 +710,7 @@ template <class Fn> int32_t run(mpm_ctx* c, Fn&& fn) {
     return MPM_ERR_RUNTIME;
   }
 }
+
 }  // namespace
 
 extern "C" {
@@ -417,6 +739,8 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     for (int i = 0; i < n_materials; ++i)
       if (materials[i].model < 0 || materials[i].model > MPM_MODEL_SNOW)
         fail(MPM_ERR_NOT_IMPLEMENTED, "material model is not supported by the HIP backend");
+    for (int a = 0; a < 3; ++a)
+      if (domain->dims[a] < 3) fail(MPM_ERR_VALUE, "grid dimensions must be at least 3 nodes");
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->dom = *domain;
     c->mats.assign(materials, materials + n_materials);
@@ -451,14 +775,26 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
       s.clamp_hi[a] = domain->clamp_hi[a];
     }
     s.wall_friction = domain->wall_friction;
-    const int64_t nodes = (int64_t)g.bdims[0] * g.bdims[1] * g.bdims[2] * 64;
-    if (nodes > (int64_t)1 << 30) fail(MPM_ERR_VALUE, "grid too large");
-    c->n_nodes = (int)nodes;
-    c->accum = dev_alloc<float4>(nodes);
-    c->vel = dev_alloc<float4>(nodes);
-    HIP_CHECK(hipMemsetAsync(c->accum, 0, nodes * sizeof(float4), c->stream));
-    HIP_CHECK(hipMemsetAsync(c->vel, 0, nodes * sizeof(float4), c->stream));
+    const int64_t bricks = (int64_t)g.bdims[0] * g.bdims[1] * g.bdims[2];
+    if (bricks * 64 > ((int64_t)1 << 30)) fail(MPM_ERR_VALUE, "grid too large");
+    c->n_bricks = (int)bricks;
+    c->n_nodes = (int)(bricks * 64);
+    c->key_bits = 1;
+    while (((int64_t)1 << c->key_bits) < bricks) ++c->key_bits;
+    c->accum = dev_alloc<float4>(c->n_nodes);
+    c->vel = dev_alloc<float4>(c->n_nodes);
+    c->dirty = dev_alloc<uint32_t>(bricks);
+    c->live = dev_alloc<uint8_t>(bricks);
+    c->offsets = dev_alloc<uint32_t>(bricks + 1);
+    c->active = dev_alloc<uint32_t>(bricks);
+    c->n_active = dev_alloc<uint32_t>(1);
+    HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
+    HIP_CHECK(hipMemsetAsync(c->vel, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
+    HIP_CHECK(hipMemsetAsync(c->dirty, 0, bricks * 4, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->live, 0, bricks, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->n_active, 0, 4, c->stream));
     c->d_vmax = dev_alloc<unsigned int>(1);
+    HIP_CHECK(hipMemsetAsync(c->d_vmax, 0, 4, c->stream));
     reserve(c, 256);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -472,28 +808,25 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
 
 int32_t mpm_destroy(mpm_ctx* c) {
   if (!c) return MPM_OK;
-  {
-    int prev = 0;
-    (void)hipGetDevice(&prev);
-    (void)hipSetDevice(c->device);  // best effort: destroy never throws
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    free_particles(c);
-    dev_free(c->accum);
-    dev_free(c->vel);
-    dev_free(c->d_mats);
-    dev_free(c->d_vmax);
-    c->scratch.~Scratch();
-    new (&c->scratch) Scratch();
-    c->scratch2.~Scratch();
-    new (&c->scratch2) Scratch();
-    for (auto& p : c->timing.pending) {
-      (void)hipEventDestroy(p.a);
-      (void)hipEventDestroy(p.b);
-    }
-    for (auto e : c->timing.pool) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    (void)hipSetDevice(prev);
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(c->device);  // best effort: destroy never throws
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_particles(c);
+  for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
+                  (void*)c->active, (void*)c->n_active, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_cols,
+                  (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris})
+    dev_free(p);
+  c->scratch.release();
+  c->hist.release();
+  c->scan_sums.release();
+  for (auto& p : c->timing.pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
   }
+  for (auto e : c->timing.pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  (void)hipSetDevice(prev);
   delete c;
   return MPM_OK;
 }
@@ -509,15 +842,16 @@ int32_t mpm_add_particles(mpm_ctx* c, int64_t n, const float* x, const float* v,
     if (c->n + n > (int64_t)INT32_MAX) fail(MPM_ERR_VALUE, "too many particles");
     reserve(c, c->n + n);
     const int64_t p0 = c->n;
-    upload_planes(c, c->x, x, 3, n, p0);
-    upload_planes(c, c->v, v, 3, n, p0);
-    HIP_CHECK(hipMemcpyAsync(c->mat + p0, mat, n * 4, hipMemcpyHostToDevice, c->stream));
+    upload_planes(c, c->A.x(), x, 3, n, p0);
+    upload_planes(c, c->A.v(), v, 3, n, p0);
+    HIP_CHECK(hipMemcpyAsync(c->A.mat() + p0, mat, n * 4, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(c->emitter + p0, emitter, n * 4, hipMemcpyHostToDevice, c->stream));
-    c->n = p0 + n;
-    k_init_new<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), p0, (int)n);
+    k_init_new<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->A.tau(), p0, (int)n);
     check_launch("k_init_new");
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->n = p0 + n;
     for (int64_t i = 0; i < n; ++i) c->present[mat[i]] = 1;
+    state_changed(c);
   });
 }
 
@@ -533,29 +867,83 @@ int32_t mpm_get_state(mpm_ctx* c, float* x, float* v, float* Cm, float* F, float
                       int32_t* emitter) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
-    if (x) download_planes(c, c->x, x, 3);
-    if (v) download_planes(c, c->v, v, 3);
-    if (Cm) download_planes(c, c->C, Cm, 9);
-    if (F) download_planes(c, c->F, F, 9);
-    if (Jp) download_planes(c, c->Jp, Jp, 1);
-    if (mat) download_planes(c, reinterpret_cast<const float*>(c->mat), reinterpret_cast<float*>(mat), 1);
-    if (emitter) download_planes(c, reinterpret_cast<const float*>(c->emitter), reinterpret_cast<float*>(emitter), 1);
+    if (x) download_planes(c, c->A.x(), x, 3);
+    if (v) download_planes(c, c->A.v(), v, 3);
+    if (Cm) download_planes(c, c->A.C(), Cm, 9);
+    if (F) download_planes(c, c->A.F(), F, 9);
+    if (Jp) download_planes(c, c->A.Jp(), Jp, 1);
+    if (mat) download_planes(c, reinterpret_cast<const float*>(c->A.mat()), reinterpret_cast<float*>(mat), 1);
+    if (emitter && c->n) {
+      HIP_CHECK(hipMemcpyAsync(emitter, c->emitter, c->n * 4, hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
   });
 }
 
-int32_t mpm_set_colliders(mpm_ctx* c, int32_t n, const int32_t*, const int32_t*, const float*, const int32_t*,
-                          const int32_t*, const float*) {
+int32_t mpm_set_colliders(mpm_ctx* c, int32_t n, const int32_t* n_verts, const int32_t* n_tris, const float* verts,
+                          const int32_t* tris, const int32_t* closed, const float* friction) {
   if (!c) return MPM_ERR_VALUE;
-  if (n == 0) return MPM_OK;
-  g_error = "colliders are not available in this build";
-  return MPM_ERR_NOT_IMPLEMENTED;
+  return run(c, [&] {
+    if (n < 0 || n > kMaxColliders) fail(MPM_ERR_VALUE, "the HIP backend takes at most 8 colliders per domain");
+    if (n > 0 && (!n_verts || !n_tris || !closed || !friction)) fail(MPM_ERR_VALUE, "null collider array");
+    std::vector<ColliderHost> hs(n);
+    int64_t vo = 0, to = 0;
+    for (int i = 0; i < n; ++i) {
+      const int nv = n_verts[i], nt = n_tris[i];
+      if (nv < 0 || nt < 0) fail(MPM_ERR_VALUE, "negative mesh size");
+      const float* V = verts + 3 * vo;
+      const int32_t* T = tris + 3 * to;
+      for (int64_t k = 0; k < 3 * (int64_t)nt; ++k)
+        if (T[k] < 0 || T[k] >= nv) fail(MPM_ERR_VALUE, "collider triangle index out of range");
+      ColliderHost& h = hs[i];
+      mesh_tables(V, nv, T, nt, h);
+      ColliderDev& d = h.dev;
+      std::memset(&d, 0, sizeof d);
+      for (int a = 0; a < 3; ++a) {
+        float lo = nv ? V[a] : 0.0f, hi = lo;
+        for (int k = 1; k < nv; ++k) {
+          lo = std::min(lo, V[3 * k + a]);
+          hi = std::max(hi, V[3 * k + a]);
+        }
+        d.aabb_lo[a] = lo;
+        d.aabb_hi[a] = hi;
+      }
+      d.closed = closed[i] ? 1 : 0;
+      d.friction = friction[i];
+      d.n_tris = nt;
+      d.q0[3] = d.q1[3] = 1.0f;  // identity until the first pose upload
+      d.S0[0] = d.S0[4] = d.S0[8] = d.S1[0] = d.S1[4] = d.S1[8] = 1.0f;
+      d.margin = (float)(3.0 * c->dom.dx_f64);
+      vo += nv;
+      to += nt;
+    }
+    for (auto& h : hs) build_candidates(c, h, h.dev.margin * 1.25f);
+    c->colliders = std::move(hs);
+    upload_colliders(c);
+  });
 }
 
-int32_t mpm_set_collider_poses(mpm_ctx* c, int32_t n, const mpm_collider_pose*) {
+int32_t mpm_set_collider_poses(mpm_ctx* c, int32_t n, const mpm_collider_pose* poses) {
   if (!c) return MPM_ERR_VALUE;
-  if (n == 0) return MPM_OK;
-  g_error = "expected 0 collider transforms";
-  return MPM_ERR_VALUE;
+  return run(c, [&] {
+    if (n != (int32_t)c->colliders.size())
+      fail(MPM_ERR_VALUE, "expected " + std::to_string(c->colliders.size()) + " collider transforms");
+    if (n > 0 && !poses) fail(MPM_ERR_VALUE, "null pose array");
+    for (int i = 0; i < n; ++i) {
+      ColliderHost& h = c->colliders[i];
+      ColliderDev& d = h.dev;
+      const mpm_collider_pose& p = poses[i];
+      std::memcpy(d.t0, p.t0, sizeof d.t0);
+      std::memcpy(d.t1, p.t1, sizeof d.t1);
+      std::memcpy(d.q0, p.q0, sizeof d.q0);
+      std::memcpy(d.q1, p.q1, sizeof d.q1);
+      std::memcpy(d.S0, p.S0, sizeof d.S0);
+      std::memcpy(d.S1, p.S1, sizeof d.S1);
+      d.margin = p.margin;
+      if (d.margin > h.built_pad) build_candidates(c, h, d.margin * 1.25f);
+    }
+    if (n > 0) upload_colliders(c);
+  });
 }
 
 int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
@@ -619,6 +1007,7 @@ int32_t mpm_debug_p2g(mpm_ctx* c, double dt) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
     HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
+    c->sorted = false;  // binned path: bin from the current positions, as a first substep does
     run_p2g(c, (float)dt);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -626,20 +1015,18 @@ int32_t mpm_debug_p2g(mpm_ctx* c, double dt) {
 
 int32_t mpm_debug_grid_update(mpm_ctx* c, double dt, double s, double frame_dt) {
   if (!c) return MPM_ERR_VALUE;
-  (void)s;
   return run(c, [&] {
     c->frame_dt = frame_dt;
-    run_grid(c, (float)dt);
+    run_grid(c, (float)dt, (float)s, (float)(1.0 / frame_dt));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
 
 int32_t mpm_debug_g2p(mpm_ctx* c, double dt, double s, double frame_dt) {
   if (!c) return MPM_ERR_VALUE;
-  (void)s;
   return run(c, [&] {
     c->frame_dt = frame_dt;
-    run_g2p(c, (float)dt);
+    run_g2p(c, (float)dt, (float)s, (float)(1.0 / frame_dt));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
@@ -649,8 +1036,8 @@ int32_t mpm_get_grid(mpm_ctx* c, int32_t which, float* grid_v, float* grid_m) {
   return run(c, [&] {
     const int64_t total = (int64_t)c->geom.dims[0] * c->geom.dims[1] * c->geom.dims[2];
     float* stage = static_cast<float*>(c->scratch.get(total * 4 * sizeof(float)));
-    k_grid_to_ref<<<grid_for(total), kBlock, 0, c->stream>>>(c->geom, which == 0 ? c->accum : c->vel, stage,
-                                                             stage + 3 * total);
+    k_grid_to_ref<<<grid_for(total), kBlock, 0, c->stream>>>(c->geom, which == 0 ? c->accum : c->vel,
+                                                             which == 0 ? nullptr : c->live, stage, stage + 3 * total);
     check_launch("k_grid_to_ref");
     HIP_CHECK(hipMemcpyAsync(grid_v, stage, total * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipMemcpyAsync(grid_m, stage + 3 * total, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -667,14 +1054,72 @@ int32_t mpm_set_grid(mpm_ctx* c, const float* grid_v, const float* grid_m) {
     HIP_CHECK(hipMemcpyAsync(stage + 3 * total, grid_m, total * sizeof(float), hipMemcpyHostToDevice, c->stream));
     k_grid_from_ref<<<grid_for(total), kBlock, 0, c->stream>>>(c->geom, stage, stage + 3 * total, c->vel);
     check_launch("k_grid_from_ref");
+    HIP_CHECK(hipMemsetAsync(c->live, 1, c->n_bricks, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
 
-int32_t mpm_get_binning(mpm_ctx* c, int32_t*, int32_t*, int32_t*, int64_t*) {
+int32_t mpm_get_binning(mpm_ctx* c, int32_t* keys, int32_t* perm, int32_t* offsets, int64_t* n_bricks) {
+  if (!c || !n_bricks) return MPM_ERR_VALUE;
+  return run(c, [&] {
+    *n_bricks = c->n_bricks;
+    if (!c->last_perm) {
+      if (keys || perm || offsets) fail(MPM_ERR_VALUE, "no binning has run yet");
+      return;
+    }
+    if (keys && c->n) HIP_CHECK(hipMemcpyAsync(keys, c->keys0, c->n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (perm && c->n) HIP_CHECK(hipMemcpyAsync(perm, c->last_perm, c->n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (offsets)
+      HIP_CHECK(hipMemcpyAsync(offsets, c->offsets, (c->n_bricks + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int32_t mpm_debug_sort(mpm_ctx* c) {
   if (!c) return MPM_ERR_VALUE;
-  g_error = "binning is not available in this build";
-  return MPM_ERR_NOT_IMPLEMENTED;
+  return run(c, [&] {
+    sort_particles(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int32_t mpm_set_option(mpm_ctx* c, int32_t option, int64_t value) {
+  if (!c) return MPM_ERR_VALUE;
+  return run(c, [&] {
+    switch (option) {
+      case MPM_OPT_P2G_MODE:
+        if (value != 0 && value != 1) fail(MPM_ERR_VALUE, "p2g mode must be 0 or 1");
+        if (value != c->p2g_mode) {
+          // the dense path leaves no dirty flags; hand over a clean accumulator either way
+          HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
+          HIP_CHECK(hipMemsetAsync(c->dirty, 0, (size_t)c->n_bricks * 4, c->stream));
+          HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+        c->p2g_mode = (int)value;
+        c->sorted = false;
+        break;
+      case MPM_OPT_SORT_INTERVAL:
+        if (value < 1) fail(MPM_ERR_VALUE, "sort interval must be >= 1");
+        c->sort_interval = value;
+        break;
+      case MPM_OPT_P2G_BLOCKS:
+        if (value < 0) fail(MPM_ERR_VALUE, "block count must be >= 0");
+        c->p2g_blocks = value;
+        break;
+      default:
+        fail(MPM_ERR_VALUE, "unknown option");
+    }
+  });
+}
+
+int64_t mpm_get_option(const mpm_ctx* c, int32_t option) {
+  if (!c) return -1;
+  switch (option) {
+    case MPM_OPT_P2G_MODE: return c->p2g_mode;
+    case MPM_OPT_SORT_INTERVAL: return c->sort_interval;
+    case MPM_OPT_P2G_BLOCKS: return c->p2g_blocks;
+    default: return -1;
+  }
 }
 
 int32_t mpm_set_timing(mpm_ctx* c, int32_t enable) {

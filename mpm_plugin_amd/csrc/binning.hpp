@@ -1,0 +1,217 @@
+// Particle binning by 4x4x4-node brick on gfx950: stable LSD radix sort of (brick key, slot) pairs,
+// brick offsets and the list of occupied bricks.
+//
+// A particle's key is the brick of its stencil base node floor(x / dx - 0.5) (the node the quadratic
+// stencil starts at, kernels.py:26-27), so all 27 stencil nodes of a particle lie in the brick's
+// 6x6x6-node neighbourhood. The sort is stable: particles of one brick keep their previous relative
+// order, and the permutation equals numpy.argsort(keys, kind="stable") bit for bit.
+//
+// Radix sort: 8-bit digits, least significant first. Each pass runs
+//   k_radix_hist     per-tile digit histogram, stored digit-major (hist[d * tiles + t])
+//   exclusive scan   over the digit-major histogram: the output offset of (digit d, tile t)
+//   k_radix_scatter  per-tile stable ranking (wave64 ballots + per-wave digit counts in LDS)
+// A tile is 256 threads x kItems items, walked in item order so the ranking preserves input order.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mpm {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// keys[p] = brick of particle slot p; vals[p] = p
+__global__ void __launch_bounds__(256) k_brick_keys(Particles P, GridGeom g, int n, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float x[3] = {P.x[p], P.x[P.cap + p], P.x[2 * P.cap + p]};
+  int base[3];
+  float fx[3];
+  stencil_base(g, x, base, fx);
+  keys[p] = (uint32_t)brick_of_base(g, base);
+  vals[p] = (uint32_t)p;
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                             int tiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int t = blockIdx.x;
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int i = t * kSortTile + j * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * tiles + t] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* __restrict__ keys_in,
+                                                                const uint32_t* __restrict__ vals_in, int n,
+                                                                int shift, int tiles,
+                                                                const uint32_t* __restrict__ offsets,
+                                                                uint32_t* __restrict__ keys_out,
+                                                                uint32_t* __restrict__ vals_out) {
+  constexpr int kWaves = kSortThreads / 64;
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wave_count[kWaves][256];
+  const int t = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  running[threadIdx.x] = offsets[threadIdx.x * tiles + t];
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int j = 0; j < kSortItems; ++j) {
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wave_count[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int i = t * kSortTile + j * kSortThreads + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t key = valid ? keys_in[i] : 0u;
+    const uint32_t val = valid ? vals_in[i] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank_in_wave = (uint32_t)__popcll(peers & lt_mask);
+    if (valid && rank_in_wave == 0) wave_count[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + rank_in_wave;
+      for (int w = 0; w < wave; ++w) pos += wave_count[w][d];
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) add += wave_count[w][threadIdx.x];
+    running[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// ---- device-wide exclusive scan of uint32 (reduce, scan of block sums, apply) -----------------
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_warp, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_warp[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int w = 0; w < waves; ++w) {
+      const uint32_t s = s_warp[w];
+      s_warp[w] = acc;
+      acc += s;
+    }
+    s_warp[waves] = acc;
+  }
+  __syncthreads();
+  total = s_warp[waves];
+  const uint32_t r = x - v + s_warp[wave];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const uint32_t* __restrict__ in, int n,
+                                                             uint32_t* __restrict__ out,
+                                                             uint32_t* __restrict__ tile_sums) {
+  __shared__ uint32_t s_warp[kScanThreads / 64 + 1];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t acc = block_exclusive_scan(sum, s_warp, total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = acc;
+    acc += v[k];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of up to kScanTile tile sums, in place
+__global__ void __launch_bounds__(kScanThreads) k_scan_top(uint32_t* __restrict__ sums, int n) {
+  __shared__ uint32_t s_warp[kScanThreads / 64 + 1];
+  const int base = threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < n) ? sums[base + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t acc = block_exclusive_scan(sum, s_warp, total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) sums[base + k] = acc;
+    acc += v[k];
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* __restrict__ out, int n,
+                                                           const uint32_t* __restrict__ tile_sums) {
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const uint32_t add = tile_sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) out[base + k] += add;
+}
+
+// offsets[b] = first sorted position with key >= b, for b in [0, n_bricks]; occupied bricks are
+// appended to `active` (their order there is irrelevant to every result)
+__global__ void __launch_bounds__(256) k_brick_bounds(const uint32_t* __restrict__ sorted_keys, int n, int n_bricks,
+                                                      uint32_t* __restrict__ offsets, uint32_t* __restrict__ active,
+                                                      uint32_t* __restrict__ n_active) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int prev = (i == 0) ? -1 : (int)sorted_keys[i - 1];
+  const int cur = (i == n) ? n_bricks : (int)sorted_keys[i];
+  for (int b = prev + 1; b <= cur; ++b) offsets[b] = (uint32_t)i;
+  if (i < n && cur != prev) active[atomicAdd(n_active, 1u)] = (uint32_t)cur;
+}
+
+// gather every per-slot particle array into sorted order: dst[i] = src[perm[i]]
+__global__ void __launch_bounds__(256) k_reorder(Particles src, Particles dst, float* __restrict__ tau_src,
+                                                 float* __restrict__ tau_dst, const uint32_t* __restrict__ perm,
+                                                 int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = perm[i];
+  const int64_t cs = src.cap, cd = dst.cap;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    dst.x[a * cd + i] = src.x[a * cs + s];
+    dst.v[a * cd + i] = src.v[a * cs + s];
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    dst.C[e * cd + i] = src.C[e * cs + s];
+    dst.F[e * cd + i] = src.F[e * cs + s];
+  }
+#pragma unroll
+  for (int e = 0; e < 6; ++e) tau_dst[e * cd + i] = tau_src[e * cs + s];
+  dst.Jp[i] = src.Jp[s];
+  dst.mat[i] = src.mat[s];
+  dst.pid[i] = src.pid[s];
+}
+
+}  // namespace mpm

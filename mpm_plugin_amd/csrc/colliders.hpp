@@ -1,0 +1,364 @@
+// Animated rigid mesh colliders on gfx950 (colliders.py semantics).
+//
+// Pose (colliders.py:20-37): M(s) = [R(slerp(q0, q1, s)) (S0 + s (S1 - S0)) | t0 + s (t1 - t0)],
+// s the substep's position in the frame. Each workgroup derives M(s), its inverse and the central
+// difference velocity matrix dM = (M(s + h) - M(s - h)) / (2 h) * inv_frame_dt (h = 1e-3,
+// colliders.py:68-74) once, into LDS, for every collider.
+//
+// Signed distance (collider_sdf, colliders.py:40-65): the query goes to mesh-local space, is dropped
+// outside the mesh AABB grown by `margin`, then finds the closest point on the mesh. The reference
+// queries a BVH (wp.mesh_query_point_sign_normal); here each collider carries a local-space voxel
+// grid covering the AABB plus the margin, built on the GPU when the mesh is set or the margin grows.
+// Voxel v (centre c, half diagonal h) lists every triangle t with dist(c, t) <= dmin(c) + 2 h, where
+// dmin(c) is the distance from c to the mesh. For any point p in the voxel the closest triangle
+// satisfies dist(c, t) <= dist(p, t) + h <= dist(p, mesh) + h <= dmin(c) + 2 h, so it is in the list
+// and the query is exact, not interpolated. Lists are in ascending triangle order and the first
+// strictly smaller distance wins, as in a linear scan. The sign is that of (p - closest) against the
+// angle-weighted pseudonormal of the closest feature (face, edge or vertex; Baerentzen & Aanaes 2005),
+// which is inside-negative for closed meshes and back-side-negative for open ones. Closest point on a
+// triangle: Ericson, Real-Time Collision Detection, 5.1.5.
+//
+// grid contact (grid_colliders, colliders.py:77-117) runs inside the grid update on every node with
+// mass; particle projection (project_particles, colliders.py:120-149) runs at the end of G2P.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mpm {
+
+constexpr int kMaxColliders = 8;
+
+struct ColliderDev {
+  float t0[3], t1[3], q0[4], q1[4], S0[9], S1[9];
+  float aabb_lo[3], aabb_hi[3];
+  float margin, friction;
+  int closed;
+  int tri_offset, n_tris;
+  // voxel grid of candidate triangles (local space)
+  float g_lo[3];
+  float g_inv_h;
+  int g_res[3];
+  int cell_offset;  // first cell of this collider in cell_start
+};
+
+// triangle corners, then pseudonormals: face, edges (ab, bc, ca), vertices (a, b, c)
+struct TriGeom {
+  float v[9];
+};
+struct TriNormals {
+  float n[21];
+};
+
+struct ColliderSet {
+  int count;
+  const ColliderDev* cols;
+  const TriGeom* tris;
+  const TriNormals* normals;
+  const uint32_t* cell_start;  // per cell: first candidate; cell_start[cell + 1] ends it
+  const uint32_t* cell_tris;   // candidate triangle indices (collider-local)
+  float thickness;             // open meshes are solid this far behind the surface (2 dx)
+  float band;                  // grid contact band in front of the surface (dx)
+};
+
+struct ColliderFrame {
+  float A[9], t[3];        // M(s)
+  float Ai[9], ti[3];      // M(s)^-1
+  float dA[9], dt[3];      // velocity matrix
+};
+
+__device__ __forceinline__ void quat_rot(const float (&q)[4], float (&R)[9]) {
+  const float x = q[0], y = q[1], z = q[2], w = q[3];
+  R[0] = 1.0f - 2.0f * (y * y + z * z); R[1] = 2.0f * (x * y - z * w); R[2] = 2.0f * (x * z + y * w);
+  R[3] = 2.0f * (x * y + z * w); R[4] = 1.0f - 2.0f * (x * x + z * z); R[5] = 2.0f * (y * z - x * w);
+  R[6] = 2.0f * (x * z - y * w); R[7] = 2.0f * (y * z + x * w); R[8] = 1.0f - 2.0f * (x * x + y * y);
+}
+
+// pose at s (colliders.py:30-37): slerp of the rotation, lerp of stretch and translation
+__device__ void collider_matrix(const ColliderDev& c, float s, float (&A)[9], float (&t)[3]) {
+  float d = c.q0[0] * c.q1[0] + c.q0[1] * c.q1[1] + c.q0[2] * c.q1[2] + c.q0[3] * c.q1[3];
+  d = fminf(fmaxf(d, -1.0f), 1.0f);
+  const float ang = acosf(d);
+  float wa, wb;
+  if (ang < 1.0e-6f) {
+    wa = 1.0f - s;
+    wb = s;
+  } else {
+    const float si = sinf(ang);
+    wa = sinf((1.0f - s) * ang) / si;
+    wb = sinf(s * ang) / si;
+  }
+  float q[4], nq = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    q[i] = wa * c.q0[i] + wb * c.q1[i];
+    nq += q[i] * q[i];
+  }
+  nq = 1.0f / sqrtf(nq);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) q[i] *= nq;
+  float R[9];
+  quat_rot(q, R);
+  float S[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = c.S0[i] + s * (c.S1[i] - c.S0[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[3 * i + j] = R[3 * i] * S[j] + R[3 * i + 1] * S[3 + j] + R[3 * i + 2] * S[6 + j];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t[i] = c.t0[i] + s * (c.t1[i] - c.t0[i]);
+}
+
+__device__ void collider_frame(const ColliderDev& c, float s, float inv_frame_dt, ColliderFrame& f) {
+  collider_matrix(c, s, f.A, f.t);
+  const float* a = f.A;
+  const float c00 = a[4] * a[8] - a[5] * a[7];
+  const float c01 = a[5] * a[6] - a[3] * a[8];
+  const float c02 = a[3] * a[7] - a[4] * a[6];
+  const float inv = 1.0f / (a[0] * c00 + a[1] * c01 + a[2] * c02);
+  f.Ai[0] = c00 * inv; f.Ai[1] = (a[2] * a[7] - a[1] * a[8]) * inv; f.Ai[2] = (a[1] * a[5] - a[2] * a[4]) * inv;
+  f.Ai[3] = c01 * inv; f.Ai[4] = (a[0] * a[8] - a[2] * a[6]) * inv; f.Ai[5] = (a[2] * a[3] - a[0] * a[5]) * inv;
+  f.Ai[6] = c02 * inv; f.Ai[7] = (a[1] * a[6] - a[0] * a[7]) * inv; f.Ai[8] = (a[0] * a[4] - a[1] * a[3]) * inv;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) f.ti[i] = -(f.Ai[3 * i] * f.t[0] + f.Ai[3 * i + 1] * f.t[1] + f.Ai[3 * i + 2] * f.t[2]);
+  const float h = 1.0e-3f;
+  float Ap[9], tp[3], Am[9], tm[3];
+  collider_matrix(c, s + h, Ap, tp);
+  collider_matrix(c, s - h, Am, tm);
+  const float k = inv_frame_dt / (2.0f * h);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) f.dA[i] = (Ap[i] - Am[i]) * k;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) f.dt[i] = (tp[i] - tm[i]) * k;
+}
+
+__device__ __forceinline__ void affine_point(const float (&A)[9], const float (&t)[3], const float (&p)[3], float (&o)[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = A[3 * i] * p[0] + A[3 * i + 1] * p[1] + A[3 * i + 2] * p[2] + t[i];
+}
+
+// closest point on triangle abc to p; feature 0 face, 1..3 vertex a/b/c, 4..6 edge ab/bc/ca
+__device__ __forceinline__ void closest_on_triangle(const float (&p)[3], const float* v, float (&q)[3], int& feature) {
+  const float ab[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+  const float ac[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+  const float ap[3] = {p[0] - v[0], p[1] - v[1], p[2] - v[2]};
+  const float d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
+  const float d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
+  if (d1 <= 0.0f && d2 <= 0.0f) { q[0] = v[0]; q[1] = v[1]; q[2] = v[2]; feature = 1; return; }
+  const float bp[3] = {p[0] - v[3], p[1] - v[4], p[2] - v[5]};
+  const float d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
+  const float d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
+  if (d3 >= 0.0f && d4 <= d3) { q[0] = v[3]; q[1] = v[4]; q[2] = v[5]; feature = 2; return; }
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    const float w = d1 / (d1 - d3);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) q[i] = v[i] + w * ab[i];
+    feature = 4; return;
+  }
+  const float cp[3] = {p[0] - v[6], p[1] - v[7], p[2] - v[8]};
+  const float d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
+  const float d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
+  if (d6 >= 0.0f && d5 <= d6) { q[0] = v[6]; q[1] = v[7]; q[2] = v[8]; feature = 3; return; }
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    const float w = d2 / (d2 - d6);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) q[i] = v[i] + w * ac[i];
+    feature = 6; return;
+  }
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) q[i] = v[3 + i] + w * (v[6 + i] - v[3 + i]);
+    feature = 5; return;
+  }
+  const float den = 1.0f / (va + vb + vc);
+  const float bv = vb * den, cw = vc * den;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) q[i] = v[i] + ab[i] * bv + ac[i] * cw;
+  feature = 0;
+}
+
+__device__ __forceinline__ void scan_triangle(const ColliderSet& cs, const ColliderDev& c, int t, const float (&pl)[3],
+                                              float& best, int& best_t, int& best_f, float (&best_q)[3]) {
+  const float* v = cs.tris[c.tri_offset + t].v;
+  float q[3];
+  int f;
+  closest_on_triangle(pl, v, q, f);
+  const float d2 = (pl[0] - q[0]) * (pl[0] - q[0]) + (pl[1] - q[1]) * (pl[1] - q[1]) + (pl[2] - q[2]) * (pl[2] - q[2]);
+  if (d2 < best) {
+    best = d2;
+    best_t = t;
+    best_f = f;
+    best_q[0] = q[0]; best_q[1] = q[1]; best_q[2] = q[2];
+  }
+}
+
+// (outward normal, signed distance); phi = 1e30 when far (colliders.py:40-65)
+__device__ void collider_sdf(const ColliderSet& cs, const ColliderDev& c, const ColliderFrame& f, const float (&p)[3],
+                             float (&n)[3], float& phi) {
+  n[0] = 0.0f; n[1] = 0.0f; n[2] = 1.0f;
+  phi = 1.0e30f;
+  float pl[3];
+  affine_point(f.Ai, f.ti, p, pl);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (pl[a] < c.aabb_lo[a] - c.margin || pl[a] > c.aabb_hi[a] + c.margin) return;
+  if (c.n_tris == 0) return;
+  float best = 3.0e38f, q[3] = {0.0f, 0.0f, 0.0f};
+  int bt = -1, bf = 0;
+  int cell[3];
+  bool in_grid = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float u = (pl[a] - c.g_lo[a]) * c.g_inv_h;
+    cell[a] = (int)floorf(u);
+    in_grid = in_grid && u >= 0.0f && cell[a] < c.g_res[a];
+  }
+  if (in_grid) {
+    const int ci = c.cell_offset + (cell[0] * c.g_res[1] + cell[1]) * c.g_res[2] + cell[2];
+    const uint32_t a0 = cs.cell_start[ci], a1 = cs.cell_start[ci + 1];
+    for (uint32_t k = a0; k < a1; ++k) scan_triangle(cs, c, (int)cs.cell_tris[k], pl, best, bt, bf, q);
+  } else {
+    for (int t = 0; t < c.n_tris; ++t) scan_triangle(cs, c, t, pl, best, bt, bf, q);
+  }
+  const float* nn = cs.normals[c.tri_offset + bt].n;
+  const float* pn = bf == 0 ? nn : (bf <= 3 ? nn + 12 + 3 * (bf - 1) : nn + 3 + 3 * (bf - 4));
+  const float ol[3] = {pl[0] - q[0], pl[1] - q[1], pl[2] - q[2]};
+  const float sign = (ol[0] * pn[0] + ol[1] * pn[1] + ol[2] * pn[2]) < 0.0f ? -1.0f : 1.0f;
+  float d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) d[i] = f.A[3 * i] * ol[0] + f.A[3 * i + 1] * ol[1] + f.A[3 * i + 2] * ol[2];
+  const float dist = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  float nw[3];
+  if (dist > 1.0e-9f) {
+    const float k = sign / dist;
+    nw[0] = d[0] * k; nw[1] = d[1] * k; nw[2] = d[2] * k;
+  } else {
+    // face normal through the inverse transpose
+#pragma unroll
+    for (int i = 0; i < 3; ++i) nw[i] = f.Ai[i] * nn[0] + f.Ai[3 + i] * nn[1] + f.Ai[6 + i] * nn[2];
+    const float l = 1.0f / sqrtf(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+    nw[0] *= l; nw[1] *= l; nw[2] *= l;
+  }
+  if (c.closed == 0 && sign < 0.0f && dist > cs.thickness) return;
+  n[0] = nw[0]; n[1] = nw[1]; n[2] = nw[2];
+  phi = sign * dist;
+}
+
+// velocity of the collider material point at p (colliders.py:68-74)
+__device__ __forceinline__ void collider_velocity(const ColliderFrame& f, const float (&p)[3], float (&vc)[3]) {
+  float pl[3];
+  affine_point(f.Ai, f.ti, p, pl);
+  affine_point(f.dA, f.dt, pl, vc);
+}
+
+// all collider frames of this substep into LDS (thread 0 of the block), then a barrier
+__device__ __forceinline__ void load_frames(const ColliderSet& cs, float s, float inv_frame_dt, ColliderFrame* sf) {
+  if (threadIdx.x < cs.count) collider_frame(cs.cols[threadIdx.x], s, inv_frame_dt, sf[threadIdx.x]);
+  __syncthreads();
+}
+
+// grid_colliders on one node with mass (colliders.py:95-117)
+__device__ void collide_node_frames(const ColliderSet& cs, const ColliderFrame* frames, const GridGeom& g, int i, int j,
+                                    int k, float4& node) {
+  const float pos[3] = {g.origin[0] + (float)i * g.dx, g.origin[1] + (float)j * g.dx, g.origin[2] + (float)k * g.dx};
+  float vel[3] = {node.x, node.y, node.z};
+  for (int c = 0; c < cs.count; ++c) {
+    const ColliderDev col = cs.cols[c];
+    float n[3], phi;
+    collider_sdf(cs, col, frames[c], pos, n, phi);
+    if (phi < cs.band) {
+      float vcol[3];
+      collider_velocity(frames[c], pos, vcol);
+      const float rel[3] = {vel[0] - vcol[0], vel[1] - vcol[1], vel[2] - vcol[2]};
+      const float vn = rel[0] * n[0] + rel[1] * n[1] + rel[2] * n[2];
+      if (vn < 0.0f) {
+        float rt[3] = {rel[0] - vn * n[0], rel[1] - vn * n[1], rel[2] - vn * n[2]};
+        const float rl = sqrtf(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+        if (rl > 0.0f) {
+          const float fr = fmaxf(0.0f, 1.0f - col.friction * (-vn) / rl);
+          rt[0] *= fr; rt[1] *= fr; rt[2] *= fr;
+        }
+        if (phi < 0.0f) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) vel[a] = vcol[a] + rt[a];
+        } else {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) vel[a] = vcol[a] + rt[a] + vn * n[a];
+        }
+      }
+    }
+  }
+  node.x = vel[0];
+  node.y = vel[1];
+  node.z = vel[2];
+}
+
+// project_particles on one particle (colliders.py:134-149)
+__device__ void project_particle(const ColliderSet& cs, const ColliderFrame* frames, float (&pos)[3], float (&vel)[3]) {
+  for (int c = 0; c < cs.count; ++c) {
+    const ColliderDev col = cs.cols[c];
+    float n[3], phi;
+    collider_sdf(cs, col, frames[c], pos, n, phi);
+    if (phi < 0.0f) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) pos[a] -= phi * n[a];
+      float vcol[3];
+      collider_velocity(frames[c], pos, vcol);
+      const float vn = (vel[0] - vcol[0]) * n[0] + (vel[1] - vcol[1]) * n[1] + (vel[2] - vcol[2]) * n[2];
+      if (vn < 0.0f) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) vel[a] -= vn * n[a];
+      }
+    }
+  }
+}
+
+// ---- candidate grid build (one thread per voxel, every triangle) -------------------------------
+
+__device__ __forceinline__ float tri_dist2(const TriGeom* tris, int t, const float (&p)[3]) {
+  float q[3];
+  int f;
+  closest_on_triangle(p, tris[t].v, q, f);
+  return (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]);
+}
+
+__device__ __forceinline__ void voxel_centre(const ColliderDev& c, int cell, float (&p)[3], float& half_diag) {
+  const int k = cell % c.g_res[2], j = (cell / c.g_res[2]) % c.g_res[1], i = cell / (c.g_res[2] * c.g_res[1]);
+  const float h = 1.0f / c.g_inv_h;
+  p[0] = c.g_lo[0] + ((float)i + 0.5f) * h;
+  p[1] = c.g_lo[1] + ((float)j + 0.5f) * h;
+  p[2] = c.g_lo[2] + ((float)k + 0.5f) * h;
+  half_diag = 0.8660254f * h;
+}
+
+// pass 0: candidate count per voxel; pass 1: candidate indices, ascending
+__global__ void __launch_bounds__(256) k_voxel_candidates(ColliderDev c, const TriGeom* __restrict__ tris, int n_cells,
+                                                          int pass, uint32_t* __restrict__ counts,
+                                                          const uint32_t* __restrict__ starts,
+                                                          uint32_t* __restrict__ out) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= n_cells) return;
+  float p[3], hd;
+  voxel_centre(c, cell, p, hd);
+  const TriGeom* T = tris + c.tri_offset;
+  float dmin2 = 3.0e38f;
+  for (int t = 0; t < c.n_tris; ++t) dmin2 = fminf(dmin2, tri_dist2(T, t, p));
+  // 1e-5 relative slack absorbs FP32 rounding of the distances
+  const float lim = (sqrtf(dmin2) + 2.0f * hd) * 1.00001f + 1.0e-6f;
+  const float lim2 = lim * lim;
+  if (pass == 0) {
+    uint32_t cnt = 0;
+    for (int t = 0; t < c.n_tris; ++t) cnt += tri_dist2(T, t, p) <= lim2 ? 1u : 0u;
+    counts[cell] = cnt;
+  } else {
+    uint32_t w = starts[cell];
+    for (int t = 0; t < c.n_tris; ++t)
+      if (tri_dist2(T, t, p) <= lim2) out[w++] = (uint32_t)t;
+  }
+}
+
+}  // namespace mpm

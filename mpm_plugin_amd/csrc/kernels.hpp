@@ -75,6 +75,25 @@ __device__ __forceinline__ void store_m3(float* base, int64_t cap, int64_t p, co
   for (int e = 0; e < 9; ++e) base[e * cap + p] = m.a[e / 3][e % 3];
 }
 
+__device__ __forceinline__ int clamp_i(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Stencil base node floor(x / dx - 0.5) and fractional offset (kernels.py:25-27, 128-130). The base is
+// clamped into the grid so a non-finite position never indexes outside it; such a particle's weights
+// are non-finite and carry the NaN into the grid, where the runner's NaN check finds it.
+__device__ __forceinline__ void stencil_base(const GridGeom& g, const float (&x)[3], int (&base)[3], float (&fx)[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float xp = (x[a] - g.origin[a]) * g.inv_dx;
+    const float fb = floorf(xp - 0.5f);
+    base[a] = clamp_i((int)fminf(fmaxf(fb, -1.0f), (float)g.dims[a]), 0, g.dims[a] - 3);
+    fx[a] = xp - fb;
+  }
+}
+
+__device__ __forceinline__ int brick_of_base(const GridGeom& g, const int (&base)[3]) {
+  return ((base[0] >> 2) * g.bdims[1] + (base[1] >> 2)) * g.bdims[2] + (base[2] >> 2);
+}
+
 // Kirchhoff stress of one particle (kernels.py:115-124): Hencky for von Mises and Drucker-Prager,
 // fixed corotated for snow (with hardening h = min(exp(xi (1 - Jp)), h_max)) and elastic.
 __device__ __forceinline__ M3<float> particle_tau(const MatDev& m, const M3<float>& F, float Jp) {
@@ -89,6 +108,58 @@ __device__ __forceinline__ M3<float> particle_tau(const MatDev& m, const M3<floa
     lam *= h;
   }
   return fixed_corotated_tau(F, U, sig, V, mu, lam);
+}
+
+// Kirchhoff stress from an SVD F = U diag(sig) V^T, as the symmetric U diag(d) U^T (6 entries
+// xx yy zz xy xz yz). Fixed corotated: (F - R) F^T = U (diag(sig) - I) diag(sig) U^T, so
+// d_i = 2 mu (sig_i - 1) sig_i + lam J (J - 1); Hencky: d_i = 2 mu eps_i + lam tr(eps).
+__device__ __forceinline__ void stress_from_svd(const MatDev& m, const M3<float>& U, const float (&sig)[3], float Jp,
+                                                float (&tau)[6]) {
+  float d[3];
+  if (m.model == kVonMises || m.model == kDruckerPrager) {
+    const float e0 = safe_log(sig[0]), e1 = safe_log(sig[1]), e2 = safe_log(sig[2]);
+    const float tr = e0 + e1 + e2;
+    d[0] = 2.0f * m.mu * e0 + m.lam * tr;
+    d[1] = 2.0f * m.mu * e1 + m.lam * tr;
+    d[2] = 2.0f * m.mu * e2 + m.lam * tr;
+  } else {
+    float mu = m.mu, lam = m.lam;
+    if (m.model == kSnow) {
+      const float h = fminf(expf(m.hardening * (1.0f - Jp)), m.max_hardening);
+      mu *= h;
+      lam *= h;
+    }
+    const float J = sig[0] * sig[1] * sig[2];
+    const float vol = lam * J * (J - 1.0f);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) d[a] = 2.0f * mu * (sig[a] - 1.0f) * sig[a] + vol;
+  }
+  const auto& u = U.a;
+  tau[0] = u[0][0] * d[0] * u[0][0] + u[0][1] * d[1] * u[0][1] + u[0][2] * d[2] * u[0][2];
+  tau[1] = u[1][0] * d[0] * u[1][0] + u[1][1] * d[1] * u[1][1] + u[1][2] * d[2] * u[1][2];
+  tau[2] = u[2][0] * d[0] * u[2][0] + u[2][1] * d[1] * u[2][1] + u[2][2] * d[2] * u[2][2];
+  tau[3] = u[0][0] * d[0] * u[1][0] + u[0][1] * d[1] * u[1][1] + u[0][2] * d[2] * u[1][2];
+  tau[4] = u[0][0] * d[0] * u[2][0] + u[0][1] * d[1] * u[2][1] + u[0][2] * d[2] * u[2][2];
+  tau[5] = u[1][0] * d[0] * u[2][0] + u[1][1] * d[1] * u[2][1] + u[1][2] * d[2] * u[2][2];
+}
+
+__device__ __forceinline__ void store_tau(float* tau, int64_t cap, int64_t p, const float (&t)[6]) {
+#pragma unroll
+  for (int e = 0; e < 6; ++e) tau[e * cap + p] = t[e];
+}
+
+// stress of every particle from its F (and Jp): after load_state, when tau is not yet known
+__global__ void __launch_bounds__(256) k_stress(Particles P, const MatDev* __restrict__ mats, int n,
+                                                float* __restrict__ tau) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const MatDev m = mats[P.mat[p]];
+  M3<float> F, U, V;
+  load_m3(P.F, P.cap, p, F);
+  float sig[3], t[6];
+  svd3(F, U, sig, V);
+  stress_from_svd(m, U, sig, P.Jp[p], t);
+  store_tau(tau, P.cap, p, t);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -110,12 +181,8 @@ __global__ void __launch_bounds__(256) k_p2g_atomic(Particles P, const MatDev* _
   for (int e = 0; e < 9; ++e) aff.a[e / 3][e % 3] = k * tau.a[e / 3][e % 3] + pm * Cm.a[e / 3][e % 3];
   float fx[3];
   int base[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float xp = (P.x[a * cap + p] - g.origin[a]) * g.inv_dx;
-    base[a] = (int)floorf(xp - 0.5f);
-    fx[a] = xp - (float)base[a];
-  }
+  const float xo[3] = {P.x[p], P.x[cap + p], P.x[2 * cap + p]};
+  stencil_base(g, xo, base, fx);
   const float mv[3] = {pm * P.v[p], pm * P.v[cap + p], pm * P.v[2 * cap + p]};
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -165,116 +232,6 @@ __device__ __forceinline__ float4 update_node(const GridGeom& g, const StepConst
   return make_float4(vel[0], vel[1], vel[2], m);
 }
 
-// Dense grid update over every node of the bricked grid; clears accum behind itself.
-__global__ void __launch_bounds__(256) k_grid_update_dense(GridGeom g, StepConsts sc, float dt, int n_nodes,
-                                                           float4* __restrict__ accum, float4* __restrict__ vel) {
-  const int addr = blockIdx.x * blockDim.x + threadIdx.x;
-  if (addr >= n_nodes) return;
-  const float4 a = accum[addr];
-  if (a.w == 0.0f && a.x == 0.0f && a.y == 0.0f && a.z == 0.0f) {
-    vel[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return;
-  }
-  int i, j, k;
-  node_ijk(g, addr, i, j, k);
-  vel[addr] = update_node(g, sc, dt, i, j, k, a);
-  accum[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-}
-
-// ---------------------------------------------------------------------------------------------
-// G2P (kernels.py:183-263): gather v and C, advect with wall projection and finite-only clamp,
-// F <- (I + dt C) F, per-model return map.
-__device__ __forceinline__ void g2p_particle(Particles& P, const MatDev* __restrict__ mats, const GridGeom& g,
-                                             const StepConsts& sc, float dt, const float4* __restrict__ vel,
-                                             int64_t p) {
-  const int64_t cap = P.cap;
-  float xo[3], fx[3];
-  int base[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    xo[a] = P.x[a * cap + p];
-    const float xp = (xo[a] - g.origin[a]) * g.inv_dx;
-    base[a] = (int)floorf(xp - 0.5f);
-    fx[a] = xp - (float)base[a];
-  }
-  float nv[3] = {0.0f, 0.0f, 0.0f};
-  M3<float> nC;
-#pragma unroll
-  for (int e = 0; e < 9; ++e) nC.a[e / 3][e % 3] = 0.0f;
-  const float c4 = 4.0f * g.inv_dx * g.inv_dx;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float wi = bspline_w(fx[0], i);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const float wj = bspline_w(fx[1], j);
-#pragma unroll
-      for (int kk = 0; kk < 3; ++kk) {
-        const float w = wi * wj * bspline_w(fx[2], kk);
-        const float dpos[3] = {((float)i - fx[0]) * g.dx, ((float)j - fx[1]) * g.dx, ((float)kk - fx[2]) * g.dx};
-        const float4 gv4 = vel[node_addr(g, base[0] + i, base[1] + j, base[2] + kk)];
-        const float gv[3] = {gv4.x, gv4.y, gv4.z};
-        const float wc = c4 * w;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          nv[a] += w * gv[a];
-#pragma unroll
-          for (int b = 0; b < 3; ++b) nC.a[a][b] += wc * gv[a] * dpos[b];
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) P.v[a * cap + p] = nv[a];
-  store_m3(P.C, cap, p, nC);
-  float nx[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    nx[a] = xo[a] + dt * nv[a];
-    if (sc.lo_on[a] != 0 && nx[a] < sc.wall_lo[a]) nx[a] = sc.wall_lo[a];
-    if (sc.hi_on[a] != 0 && nx[a] > sc.wall_hi[a]) nx[a] = sc.wall_hi[a];
-  }
-  // a non-finite position is stored as it is so the NaN check of the runner sees it (kernels.py:236-241)
-  if (isfinite(nx[0]) && isfinite(nx[1]) && isfinite(nx[2])) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) nx[a] = fminf(fmaxf(nx[a], sc.clamp_lo[a]), sc.clamp_hi[a]);
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) P.x[a * cap + p] = nx[a];
-  M3<float> Fo;
-  load_m3(P.F, cap, p, Fo);
-  M3<float> G;
-#pragma unroll
-  for (int e = 0; e < 9; ++e) G.a[e / 3][e % 3] = (e % 4 == 0 ? 1.0f : 0.0f) + dt * nC.a[e / 3][e % 3];
-  const M3<float> Fn = mul(G, Fo);
-  const MatDev m = mats[P.mat[p]];
-  if (m.model == kElastic) {
-    store_m3(P.F, cap, p, Fn);
-    return;
-  }
-  M3<float> U, V;
-  float sig[3], ns[3];
-  svd3(Fn, U, sig, V);
-  if (m.model == kVonMises) {
-    project_von_mises(sig, m.mu, m.yield_stress, ns);
-  } else if (m.model == kDruckerPrager) {
-    project_drucker_prager(sig, m.mu, m.lam, m.alpha, ns);
-  } else {  // snow (Stomakhin et al. 2013)
-    const float lo = 1.0f - m.crit_c, hi = 1.0f + m.crit_s;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) ns[a] = fminf(fmaxf(sig[a], lo), hi);
-    P.Jp[p] = P.Jp[p] * (sig[0] * sig[1] * sig[2]) / (ns[0] * ns[1] * ns[2]);
-  }
-  store_m3(P.F, cap, p, usv(U, ns, V));
-}
-
-__global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restrict__ mats, GridGeom g,
-                                             StepConsts sc, int n, float dt, const float4* __restrict__ vel) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  g2p_particle(P, mats, g, sc, dt, vel, p);
-}
-
 // ---------------------------------------------------------------------------------------------
 // max |v| for the CFL step (kernels.py:266-269). Speeds are >= 0, so their IEEE bit patterns order
 // like unsigned integers; a NaN or inf speed yields a non-finite maximum, which the host treats as 0.
@@ -319,8 +276,8 @@ __global__ void k_unpack(const float* __restrict__ src, int comps, int n, int64_
   for (int c = 0; c < comps; ++c) dst[c * cap + p0 + p] = src[(int64_t)p * comps + c];
 }
 
-// new particles: C = 0, F = I, Jp = 1, pid = insertion index
-__global__ void k_init_new(Particles P, int64_t p0, int n) {
+// new particles: C = 0, F = I, Jp = 1, tau = 0 (the stress of F = I), pid = insertion index
+__global__ void k_init_new(Particles P, float* __restrict__ tau, int64_t p0, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t p = p0 + i;
@@ -329,18 +286,28 @@ __global__ void k_init_new(Particles P, int64_t p0, int n) {
     P.C[e * P.cap + p] = 0.0f;
     P.F[e * P.cap + p] = (e % 4 == 0) ? 1.0f : 0.0f;
   }
+#pragma unroll
+  for (int e = 0; e < 6; ++e) tau[e * P.cap + p] = 0.0f;
   P.Jp[p] = 1.0f;
   P.pid[p] = (int32_t)p;
 }
 
+__global__ void k_iota(int32_t* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = i;
+}
+
 // bricked grid <-> reference C-order (D0, D1, D2) arrays
-__global__ void k_grid_to_ref(GridGeom g, const float4* __restrict__ src, float* __restrict__ v3,
-                              float* __restrict__ m) {
+// live (optional): per-brick flag; nodes of bricks without a fresh value read as zero
+__global__ void k_grid_to_ref(GridGeom g, const float4* __restrict__ src, const uint8_t* __restrict__ live,
+                              float* __restrict__ v3, float* __restrict__ m) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int total = g.dims[0] * g.dims[1] * g.dims[2];
   if (idx >= total) return;
   const int k = idx % g.dims[2], j = (idx / g.dims[2]) % g.dims[1], i = idx / (g.dims[2] * g.dims[1]);
-  const float4 a = src[node_addr(g, i, j, k)];
+  const int addr = node_addr(g, i, j, k);
+  float4 a = src[addr];
+  if (live && !live[addr >> 6]) a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   v3[3 * (int64_t)idx] = a.x;
   v3[3 * (int64_t)idx + 1] = a.y;
   v3[3 * (int64_t)idx + 2] = a.z;
@@ -365,12 +332,8 @@ __global__ void __launch_bounds__(256) k_scatter_volume(Particles P, GridGeom g,
   if (p >= n) return;
   float fx[3];
   int base[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float xp = (P.x[a * P.cap + p] - g.origin[a]) * g.inv_dx;
-    base[a] = (int)floorf(xp - 0.5f);
-    fx[a] = xp - (float)base[a];
-  }
+  const float xo[3] = {P.x[p], P.x[P.cap + p], P.x[2 * P.cap + p]};
+  stencil_base(g, xo, base, fx);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -1,0 +1,331 @@
+// Binned P2G and brick-sparse grid update on gfx950.
+//
+// P2G (kernels.py:92-140 semantics) runs one workgroup per occupied brick B (grid-stride over the
+// device-side list of occupied bricks, so the launch needs no host-side count). Particles were sorted
+// by brick, so brick B's particles are the contiguous slots [offsets[B], offsets[B+1]). Their stencils
+// lie in the 8x8x8-node window starting at node 4B - 1, which leaves one node of margin for particles
+// that drifted since the sort; the window (momentum xyz + mass, 8 KiB as four planes) is accumulated
+// with LDS float atomics and flushed to the bricked HBM grid with one global atomic per non-zero node
+// component. A particle whose stencil left the window (drift of more than one node) scatters straight
+// to HBM with global atomics. Every brick a workgroup may have written is flagged dirty.
+//
+// The grid update (kernels.py:155-180) runs one wave per brick (64 nodes = 64 lanes) and only on dirty
+// bricks: momentum -> velocity, gravity, walls, then contact with the mesh colliders; it clears the
+// accumulator and the dirty flag behind itself and records which bricks hold a fresh velocity.
+#pragma once
+
+#include "colliders.hpp"
+#include "kernels.hpp"
+
+namespace mpm {
+
+constexpr int kWin = 8;
+constexpr int kWinNodes = kWin * kWin * kWin;
+constexpr int kP2GThreads = 256;
+
+__device__ __forceinline__ void mark_brick(const GridGeom& g, uint32_t* dirty, int bx, int by, int bz) {
+  if (bx < 0 || by < 0 || bz < 0 || bx >= g.bdims[0] || by >= g.bdims[1] || bz >= g.bdims[2]) return;
+  dirty[(bx * g.bdims[1] + by) * g.bdims[2] + bz] = 1u;
+}
+
+// scatter of one particle: affine momentum and mass to its 27 stencil nodes, through LDS when the
+// stencil is inside the window, else through HBM atomics
+__device__ __forceinline__ void p2g_particle(const Particles& P, const float* __restrict__ tau,
+                                             const MatDev* __restrict__ mats, const GridGeom& g, float p_vol,
+                                             float dt, int64_t p, const int (&wo)[3], float (*s_acc)[kWinNodes],
+                                             float4* __restrict__ accum, uint32_t* __restrict__ dirty) {
+  const int64_t cap = P.cap;
+  const float pm = mats[P.mat[p]].p_mass;
+  const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
+  const float t00 = tau[p], t11 = tau[cap + p], t22 = tau[2 * cap + p];
+  const float t01 = tau[3 * cap + p], t02 = tau[4 * cap + p], t12 = tau[5 * cap + p];
+  float A[3][3];
+  A[0][0] = k * t00 + pm * P.C[p];
+  A[0][1] = k * t01 + pm * P.C[cap + p];
+  A[0][2] = k * t02 + pm * P.C[2 * cap + p];
+  A[1][0] = k * t01 + pm * P.C[3 * cap + p];
+  A[1][1] = k * t11 + pm * P.C[4 * cap + p];
+  A[1][2] = k * t12 + pm * P.C[5 * cap + p];
+  A[2][0] = k * t02 + pm * P.C[6 * cap + p];
+  A[2][1] = k * t12 + pm * P.C[7 * cap + p];
+  A[2][2] = k * t22 + pm * P.C[8 * cap + p];
+  const float xo[3] = {P.x[p], P.x[cap + p], P.x[2 * cap + p]};
+  const float mv[3] = {pm * P.v[p], pm * P.v[cap + p], pm * P.v[2 * cap + p]};
+  int base[3];
+  float fx[3];
+  stencil_base(g, xo, base, fx);
+  float wx[3][3], dp[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      wx[a][i] = bspline_w(fx[a], i);
+      dp[a][i] = ((float)i - fx[a]) * g.dx;
+    }
+  const int l0 = base[0] - wo[0], l1 = base[1] - wo[1], l2 = base[2] - wo[2];
+  const bool in_window = l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 <= kWin - 3 && l1 <= kWin - 3 && l2 <= kWin - 3;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wij = wx[0][i] * wx[1][j];
+      const float ax = mv[0] + A[0][0] * dp[0][i] + A[0][1] * dp[1][j];
+      const float ay = mv[1] + A[1][0] * dp[0][i] + A[1][1] * dp[1][j];
+      const float az = mv[2] + A[2][0] * dp[0][i] + A[2][1] * dp[1][j];
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const float w = wij * wx[2][kk];
+        const float vx = w * (ax + A[0][2] * dp[2][kk]);
+        const float vy = w * (ay + A[1][2] * dp[2][kk]);
+        const float vz = w * (az + A[2][2] * dp[2][kk]);
+        const float vm = w * pm;
+        if (in_window) {
+          const int li = ((l0 + i) * kWin + (l1 + j)) * kWin + (l2 + kk);
+          atomicAdd(&s_acc[0][li], vx);
+          atomicAdd(&s_acc[1][li], vy);
+          atomicAdd(&s_acc[2][li], vz);
+          atomicAdd(&s_acc[3][li], vm);
+        } else {
+          float4* node = accum + node_addr(g, base[0] + i, base[1] + j, base[2] + kk);
+          atomicAdd(&node->x, vx);
+          atomicAdd(&node->y, vy);
+          atomicAdd(&node->z, vz);
+          atomicAdd(&node->w, vm);
+        }
+      }
+    }
+  }
+  if (!in_window) {
+    // the stencil spans at most two bricks per axis
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+          mark_brick(g, dirty, (base[0] + 2 * i) >> 2, (base[1] + 2 * j) >> 2, (base[2] + 2 * kk) >> 2);
+  }
+}
+
+__global__ void __launch_bounds__(kP2GThreads) k_p2g_binned(Particles P, const float* __restrict__ tau,
+                                                            const MatDev* __restrict__ mats, GridGeom g, float p_vol,
+                                                            float dt, const uint32_t* __restrict__ offsets,
+                                                            const uint32_t* __restrict__ active,
+                                                            const uint32_t* __restrict__ n_active_ptr,
+                                                            float4* __restrict__ accum, uint32_t* __restrict__ dirty) {
+  __shared__ float s_acc[4][kWinNodes];
+  const uint32_t n_active = *n_active_ptr;
+  for (uint32_t w = blockIdx.x; w < n_active; w += gridDim.x) {
+    const uint32_t b = active[w];
+    const int bz = (int)(b % (uint32_t)g.bdims[2]);
+    const int by = (int)((b / (uint32_t)g.bdims[2]) % (uint32_t)g.bdims[1]);
+    const int bx = (int)(b / ((uint32_t)g.bdims[2] * (uint32_t)g.bdims[1]));
+    const int wo[3] = {4 * bx - 1, 4 * by - 1, 4 * bz - 1};
+    for (int i = threadIdx.x; i < kWinNodes; i += kP2GThreads) {
+      s_acc[0][i] = 0.0f;
+      s_acc[1][i] = 0.0f;
+      s_acc[2][i] = 0.0f;
+      s_acc[3][i] = 0.0f;
+    }
+    __syncthreads();
+    const uint32_t start = offsets[b], end = offsets[b + 1];
+    for (uint32_t s = start + threadIdx.x; s < end; s += kP2GThreads)
+      p2g_particle(P, tau, mats, g, p_vol, dt, (int64_t)s, wo, s_acc, accum, dirty);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kWinNodes; i += kP2GThreads) {
+      const int gi = wo[0] + (i >> 6), gj = wo[1] + ((i >> 3) & 7), gk = wo[2] + (i & 7);
+      if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
+      const float vx = s_acc[0][i], vy = s_acc[1][i], vz = s_acc[2][i], vm = s_acc[3][i];
+      if (vx == 0.0f && vy == 0.0f && vz == 0.0f && vm == 0.0f) continue;
+      float4* node = accum + node_addr(g, gi, gj, gk);
+      if (vx != 0.0f) atomicAdd(&node->x, vx);
+      if (vy != 0.0f) atomicAdd(&node->y, vy);
+      if (vz != 0.0f) atomicAdd(&node->z, vz);
+      if (vm != 0.0f) atomicAdd(&node->w, vm);
+    }
+    if (threadIdx.x < 27) {
+      const int t = threadIdx.x;
+      mark_brick(g, dirty, bx - 1 + t / 9, by - 1 + (t / 3) % 3, bz - 1 + t % 3);
+    }
+    __syncthreads();
+  }
+}
+
+// one wave per brick; non-dirty bricks are skipped (and flagged stale for the grid download)
+__global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, float dt, int n_bricks,
+                                                     uint32_t* __restrict__ dirty, uint8_t* __restrict__ live,
+                                                     float4* __restrict__ accum, float4* __restrict__ vel,
+                                                     ColliderSet cs, float s, float inv_frame_dt,
+                                                     unsigned int* __restrict__ vmax_bits) {
+  __shared__ ColliderFrame frames[kMaxColliders];
+  if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && vmax_bits) *vmax_bits = 0u;  // G2P of this substep refills it
+  const int brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (brick >= n_bricks) return;
+  const int lane = threadIdx.x & 63;
+  const bool d = dirty[brick] != 0u;
+  if (lane == 0) live[brick] = d ? 1 : 0;
+  if (!d) return;
+  const int addr = brick * 64 + lane;
+  int i, j, k;
+  node_ijk(g, addr, i, j, k);
+  const float4 a = accum[addr];
+  float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (i < g.dims[0] && j < g.dims[1] && k < g.dims[2]) {
+    out = update_node(g, sc, dt, i, j, k, a);
+    if (cs.count > 0 && out.w > 0.0f) collide_node_frames(cs, frames, g, i, j, k, out);
+  }
+  vel[addr] = out;
+  if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f || a.w != 0.0f) accum[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (lane == 0) dirty[brick] = 0u;
+}
+
+// dense variant for the reference-order P2G (which flags no bricks): every node of the grid
+__global__ void __launch_bounds__(256) k_grid_dense(GridGeom g, StepConsts sc, float dt, int n_nodes,
+                                                    float4* __restrict__ accum, float4* __restrict__ vel,
+                                                    uint8_t* __restrict__ live, ColliderSet cs, float s,
+                                                    float inv_frame_dt, unsigned int* __restrict__ vmax_bits) {
+  __shared__ ColliderFrame frames[kMaxColliders];
+  if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && vmax_bits) *vmax_bits = 0u;
+  const int addr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (addr >= n_nodes) return;
+  if ((addr & 63) == 0) live[addr >> 6] = 1;
+  const float4 a = accum[addr];
+  int i, j, k;
+  node_ijk(g, addr, i, j, k);
+  float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (i < g.dims[0] && j < g.dims[1] && k < g.dims[2]) {
+    out = update_node(g, sc, dt, i, j, k, a);
+    if (cs.count > 0 && out.w > 0.0f) collide_node_frames(cs, frames, g, i, j, k, out);
+  }
+  vel[addr] = out;
+  if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f || a.w != 0.0f) accum[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------------------------
+// G2P (kernels.py:183-263): gather v and C, advect with wall projection and finite-only clamp,
+// F <- (I + dt C) F, per-model return map. One SVD of the new F per particle also gives the Kirchhoff
+// stress that the next P2G scatters (stored as 6 symmetric entries), so P2G needs no SVD. The
+// largest |v| of the substep is reduced into *vmax_bits for the next CFL step.
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ void g2p_particle(Particles& P, const MatDev* __restrict__ mats, const GridGeom& g,
+                                             const StepConsts& sc, float dt, const float4* __restrict__ vel,
+                                             float* __restrict__ tau_out, int64_t p, const ColliderSet& cs,
+                                             const ColliderFrame* frames, float& speed) {
+  const int64_t cap = P.cap;
+  const float xo[3] = {P.x[p], P.x[cap + p], P.x[2 * cap + p]};
+  float fx[3];
+  int base[3];
+  stencil_base(g, xo, base, fx);
+  float wx[3][3], dp[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      wx[a][i] = bspline_w(fx[a], i);
+      dp[a][i] = ((float)i - fx[a]) * g.dx;
+    }
+  float nv[3] = {0.0f, 0.0f, 0.0f};
+  M3<float> nC;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) nC.a[e / 3][e % 3] = 0.0f;
+  const float c4 = 4.0f * g.inv_dx * g.inv_dx;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wij = wx[0][i] * wx[1][j];
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const float w = wij * wx[2][kk];
+        const float dpos[3] = {dp[0][i], dp[1][j], dp[2][kk]};
+        const float4 gv4 = vel[node_addr(g, base[0] + i, base[1] + j, base[2] + kk)];
+        const float gv[3] = {gv4.x, gv4.y, gv4.z};
+        const float wc = c4 * w;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          nv[a] += w * gv[a];
+#pragma unroll
+          for (int b = 0; b < 3; ++b) nC.a[a][b] += wc * gv[a] * dpos[b];
+        }
+      }
+    }
+  }
+  store_m3(P.C, cap, p, nC);
+  float nx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nx[a] = xo[a] + dt * nv[a];
+    if (sc.lo_on[a] != 0 && nx[a] < sc.wall_lo[a]) nx[a] = sc.wall_lo[a];
+    if (sc.hi_on[a] != 0 && nx[a] > sc.wall_hi[a]) nx[a] = sc.wall_hi[a];
+  }
+  // a non-finite position is stored as it is so the NaN check of the runner sees it (kernels.py:236-241)
+  if (isfinite(nx[0]) && isfinite(nx[1]) && isfinite(nx[2])) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) nx[a] = fminf(fmaxf(nx[a], sc.clamp_lo[a]), sc.clamp_hi[a]);
+  }
+  // project_particles (colliders.py:120-149) on the advected position and the new velocity
+  if (cs.count > 0) project_particle(cs, frames, nx, nv);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    P.x[a * cap + p] = nx[a];
+    P.v[a * cap + p] = nv[a];
+  }
+  speed = sqrtf(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+  M3<float> Fo;
+  load_m3(P.F, cap, p, Fo);
+  M3<float> G;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) G.a[e / 3][e % 3] = (e % 4 == 0 ? 1.0f : 0.0f) + dt * nC.a[e / 3][e % 3];
+  const M3<float> Fn = mul(G, Fo);
+  const MatDev m = mats[P.mat[p]];
+  M3<float> U, V;
+  float sig[3], t[6];
+  svd3(Fn, U, sig, V);
+  if (m.model == kElastic) {
+    store_m3(P.F, cap, p, Fn);
+    stress_from_svd(m, U, sig, 1.0f, t);
+    store_tau(tau_out, cap, p, t);
+    return;
+  }
+  float ns[3];
+  float jp = 1.0f;
+  if (m.model == kVonMises) {
+    project_von_mises(sig, m.mu, m.yield_stress, ns);
+  } else if (m.model == kDruckerPrager) {
+    project_drucker_prager(sig, m.mu, m.lam, m.alpha, ns);
+  } else {  // snow (Stomakhin et al. 2013)
+    const float lo = 1.0f - m.crit_c, hi = 1.0f + m.crit_s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ns[a] = fminf(fmaxf(sig[a], lo), hi);
+    jp = P.Jp[p] * (sig[0] * sig[1] * sig[2]) / (ns[0] * ns[1] * ns[2]);
+    P.Jp[p] = jp;
+  }
+  store_m3(P.F, cap, p, usv(U, ns, V));
+  stress_from_svd(m, U, ns, jp, t);
+  store_tau(tau_out, cap, p, t);
+}
+
+__global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restrict__ mats, GridGeom g,
+                                             StepConsts sc, int n, float dt, const float4* __restrict__ vel,
+                                             float* __restrict__ tau, unsigned int* __restrict__ vmax_bits,
+                                             ColliderSet cs, float s, float inv_frame_dt) {
+  __shared__ ColliderFrame frames[kMaxColliders];
+  if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  float speed = 0.0f;
+  if (p < n) g2p_particle(P, mats, g, sc, dt, vel, tau, p, cs, frames, speed);
+  // NaN compares false in fmaxf and would vanish from the maximum: map it to +inf, which the host
+  // also reads as "not finite"
+  if (!(speed == speed)) speed = __int_as_float(0x7f800000);
+  const float wm = wave_max(speed);
+  if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(wm));
+}
+
+}  // namespace mpm

@@ -250,7 +250,7 @@ class State:
 
     @classmethod
     def from_dict(cls, s: dict) -> State:
-        return cls(*(np.ascontiguousarray(s[k], t) for k, t in
+        return cls(*(np.array(s[k], t, copy=True, order="C") for k, t in
                      (("x", np.float32), ("v", np.float32), ("C", np.float32), ("F", np.float32),
                       ("Jp", np.float32), ("mat", np.int32), ("emitter", np.int32))))
 

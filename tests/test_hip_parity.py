@@ -59,20 +59,26 @@ def seeded_state(n_target: int, res: int, seed: int, strain: float = 0.05, mater
     }
 
 
-def pair(state, res=64, walls=None, gravity=(0.0, 0.0, -9.81), materials=MATERIALS):
+def pair(state, res=64, walls=None, gravity=(0.0, 0.0, -9.81), materials=MATERIALS, mode=0):
     from mpm_core.backends import create_solver
+    from mpm_plugin_amd import _native as N
 
     d = Domain((0, 0, 0), (1, 1, 1), res, walls=walls or {})
     hip = create_solver("hip", domain=d, materials=materials, gravity=gravity, device="cuda:0")
+    hip.set_option(N.OPT_P2G_MODE, mode)
     ora = O.OracleSolver(d, materials, gravity)
     hip.load_state(state)
     ora.load_state(state)
     return hip, ora
 
 
+MODES = [pytest.param(0, id="binned"), pytest.param(1, id="atomic")]
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("seed", [0, 1])
-def test_p2g_grid_matches_oracle(seed):
-    hip, ora = pair(seeded_state(20000, 64, seed))
+def test_p2g_grid_matches_oracle(seed, mode):
+    hip, ora = pair(seeded_state(20000, 64, seed), mode=mode)
     dt = 1.0e-4
     gv_h, gm_h = hip.debug_p2g(dt)
     gv_o, gm_o = ora.p2g(dt)
@@ -81,10 +87,11 @@ def test_p2g_grid_matches_oracle(seed):
     assert rel_err(gv_h, gv_o) <= 1e-5
 
 
-def test_grid_update_matches_oracle():
+@pytest.mark.parametrize("mode", MODES)
+def test_grid_update_matches_oracle(mode):
     state = seeded_state(20000, 64, 3)
     state["x"][:, 2] = np.float32(0.5 / 64 * 1.2)  # a layer on the floor exercises the wall branch
-    hip, ora = pair(state, gravity=(1.0, -2.0, -9.81))
+    hip, ora = pair(state, gravity=(1.0, -2.0, -9.81), mode=mode)
     dt = 1.0e-4
     hip.debug_p2g(dt)
     ora.p2g(dt)
@@ -111,9 +118,10 @@ def test_g2p_matches_oracle_from_same_grid(model):
     assert np.array_equal(a["mat"], b["mat"]) and np.array_equal(a["emitter"], b["emitter"])
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("seed", [0, 1])
-def test_full_substep_matches_oracle(seed):
-    hip, ora = pair(seeded_state(30000, 64, 20 + seed))
+def test_full_substep_matches_oracle(seed, mode):
+    hip, ora = pair(seeded_state(30000, 64, 20 + seed), mode=mode)
     dt = 1.0e-4
     hip.substep(dt)
     ora.substep(dt)
@@ -122,9 +130,13 @@ def test_full_substep_matches_oracle(seed):
         assert rel_err(a[k], b[k]) <= 1e-4, k
 
 
-def test_elastic_block_ten_substeps():
+@pytest.mark.parametrize("mode", MODES)
+def test_elastic_block_ten_substeps(mode):
+    from mpm_plugin_amd import _native as N
+
     w = workloads.elastic_block()
     hip = w.make_solver()
+    hip.set_option(N.OPT_P2G_MODE, mode)
     ora = O.OracleSolver(w.domain, w.materials, w.gravity)
     ora.add_particles(w.positions, w.velocities, w.material_ids, w.emitter_ids)
     for _ in range(10):
@@ -134,3 +146,21 @@ def test_elastic_block_ten_substeps():
     a, b = hip.save_state(), ora.save_state()
     for k in ("x", "v", "F"):
         assert rel_err(a[k], b[k]) <= 1e-4, k
+
+
+def test_sorted_substeps_match_oracle_over_many_steps():
+    """Sixteen substeps with re-binning every 4: the LDS windows, the drift margin and the reorder hold."""
+    from mpm_plugin_amd import _native as N
+
+    state = seeded_state(30000, 64, 40, strain=0.02)
+    state["v"] *= 4.0  # drift across brick boundaries between re-sorts
+    hip, ora = pair(state)
+    hip.set_option(N.OPT_SORT_INTERVAL, 4)
+    dt = 1.0e-4
+    for _ in range(16):
+        hip.substep(dt)
+        ora.substep(dt)
+    a, b = hip.save_state(), ora.save_state()
+    for k in ("x", "v", "F"):
+        assert rel_err(a[k], b[k]) <= 1e-4, k
+    assert np.array_equal(a["emitter"], b["emitter"]) and np.array_equal(a["mat"], b["mat"])
