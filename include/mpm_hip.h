@@ -131,8 +131,10 @@ int32_t mpm_debug_g2p(mpm_ctx* ctx, double dt, double s, double frame_dt);
 int32_t mpm_get_grid(mpm_ctx* ctx, int32_t which, float* grid_v, float* grid_m);
 /* overwrite the grid velocity and mass (which = 1 layout) before mpm_debug_g2p */
 int32_t mpm_set_grid(mpm_ctx* ctx, const float* grid_v, const float* grid_m);
-/* particle binning of the last substep: per-particle brick key, brick offsets, permutation */
-int32_t mpm_get_binning(mpm_ctx* ctx, int32_t* keys, int32_t* perm, int32_t* offsets, int64_t* n_bricks);
+/* particle binning of the last sort: per-particle cell key (brick * 64 + cell within the 4x4x4-node
+ * brick, in slot order before the sort), the stable permutation, and per-cell offsets (n_bins + 1);
+ * call with null arrays to get n_bins */
+int32_t mpm_get_binning(mpm_ctx* ctx, int32_t* keys, int32_t* perm, int32_t* offsets, int64_t* n_bins);
 
 /* run the brick binning (stable sort + reorder) now, as the next substep would (parity tests) */
 int32_t mpm_debug_sort(mpm_ctx* ctx);

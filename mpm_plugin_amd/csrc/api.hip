@@ -6,8 +6,9 @@
 //
 // Substep on the default (binned) path:
 //   [every sort_interval substeps, or after any state change]
-//     k_brick_keys -> radix passes (k_radix_hist, scan, k_radix_scatter) -> k_brick_bounds -> k_reorder
-//   k_p2g_binned   (LDS windows per occupied brick)
+//     k_cell_keys -> radix passes (k_radix_hist, scan, k_radix_scatter) -> k_cell_bounds
+//     -> k_active_bricks -> k_reorder
+//   k_p2g_cells    (one wave per occupied brick, one lane per cell, wave-private LDS window)
 //   k_grid_bricks  (dirty bricks: update, walls, colliders)
 //   k_g2p          (gather, advection, collider projection, return map, stress, max speed)
 #include <hip/hip_runtime.h>
@@ -20,7 +21,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mpm_hip.h"
+#include "mpm_hip.h"
 #include "binning.hpp"
 #include "substep.hpp"
 
@@ -143,14 +144,14 @@ struct mpm_ctx {
   // grid
   GridGeom geom{};
   StepConsts sc{};
-  int n_nodes = 0, n_bricks = 0, key_bits = 1;
+  int n_nodes = 0, n_bricks = 0, n_cells = 0, key_bits = 1;
   float4 *accum = nullptr, *vel = nullptr;
   uint32_t* dirty = nullptr;
   uint8_t* live = nullptr;
 
   // binning
   uint32_t *keys0 = nullptr, *keys1 = nullptr, *keys2 = nullptr, *vals1 = nullptr, *vals2 = nullptr;
-  uint32_t *offsets = nullptr, *active = nullptr, *n_active = nullptr;
+  uint32_t *offsets = nullptr, *active = nullptr, *n_active = nullptr;  // offsets: per cell
   Scratch hist, scan_sums;
   bool sorted = false;
   int64_t steps_since_sort = 0;
@@ -164,11 +165,15 @@ struct mpm_ctx {
   // colliders
   std::vector<ColliderHost> colliders;
   ColliderDev* d_cols = nullptr;
+  std::vector<ColliderDev> col_records;  // host copy of d_cols (kept alive for the async upload)
   TriGeom* d_tris = nullptr;
   TriNormals* d_normals = nullptr;
   uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
 
-  unsigned int* d_vmax = nullptr;
+  float* d_vmax = nullptr;        // reduced maximum speed
+  float* d_block_max = nullptr;   // per-workgroup maxima of the last G2P (or speed pass)
+  int64_t block_max_cap = 0;
+  int64_t block_max_count = 0;
   bool vmax_valid = false;
   Scratch scratch;
   Timing timing;
@@ -246,6 +251,18 @@ void collect_timing(mpm_ctx* c) {
 }
 
 void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
+
+// per-workgroup speed maxima, one entry per 256 particle slots
+float* block_max(mpm_ctx* c) {
+  const int64_t need = std::max<int64_t>(1, (c->cap + kBlock - 1) / kBlock);
+  if (need > c->block_max_cap) {
+    dev_free(c->d_block_max);
+    c->d_block_max = nullptr;
+    c->d_block_max = dev_alloc<float>(need);
+    c->block_max_cap = need;
+  }
+  return c->d_block_max;
+}
 
 // ---- particle storage ------------------------------------------------------------------------
 void free_particles(mpm_ctx* c) {
@@ -348,14 +365,14 @@ void sort_particles(mpm_ctx* c) {
   Timed t(c, MPM_K_BIN);
   HIP_CHECK(hipMemsetAsync(c->n_active, 0, sizeof(uint32_t), c->stream));
   if (n == 0) {
-    k_brick_bounds<<<1, kBlock, 0, c->stream>>>(c->keys1, 0, c->n_bricks, c->offsets, c->active, c->n_active);
-    check_launch("k_brick_bounds");
+    k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(c->keys1, 0, c->n_cells, c->offsets);
+    check_launch("k_cell_bounds");
     c->sorted = true;
     c->steps_since_sort = 0;
     return;
   }
-  k_brick_keys<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)n, c->keys0, c->vals2);
-  check_launch("k_brick_keys");
+  k_cell_keys<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)n, c->keys0, c->vals2);
+  check_launch("k_cell_keys");
   const int passes = (c->key_bits + 7) / 8;
   const int tiles = (int)((n + kSortTile - 1) / kSortTile);
   uint32_t* hist = static_cast<uint32_t*>(c->hist.get((size_t)256 * tiles * sizeof(uint32_t)));
@@ -372,9 +389,10 @@ void sort_particles(mpm_ctx* c) {
     kin = kout;
     vin = vout;
   }
-  k_brick_bounds<<<grid_for(n + 1), kBlock, 0, c->stream>>>(kin, (int)n, c->n_bricks, c->offsets, c->active,
-                                                            c->n_active);
-  check_launch("k_brick_bounds");
+  k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(kin, (int)n, c->n_cells, c->offsets);
+  check_launch("k_cell_bounds");
+  k_active_bricks<<<grid_for(c->n_bricks), kBlock, 0, c->stream>>>(c->offsets, c->n_bricks, c->active, c->n_active);
+  check_launch("k_active_bricks");
   k_reorder<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.view(), c->B.view(), c->A.tau(), c->B.tau(), vin, (int)n);
   check_launch("k_reorder");
   std::swap(c->A, c->B);
@@ -396,25 +414,24 @@ void run_p2g(mpm_ctx* c, float dt) {
   if (!c->sorted || c->steps_since_sort >= c->sort_interval) sort_particles(c);
   if (c->n == 0) return;
   Timed t(c, MPM_K_P2G);
-  int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 4096;
-  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, std::min<int64_t>(c->n_bricks, c->n)));
-  k_p2g_binned<<<(unsigned)blocks, kP2GThreads, 0, c->stream>>>(c->parts(), c->A.tau(), c->d_mats, c->geom,
-                                                                c->sc.p_vol, dt, c->offsets, c->active, c->n_active,
-                                                                c->accum, c->dirty);
-  check_launch("k_p2g_binned");
+  // four waves (four bricks at a time) per workgroup, grid-stride over the device-side brick list
+  int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 2048;
+  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, (std::min<int64_t>(c->n_bricks, c->n) + 3) / 4));
+  k_p2g_cells<<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->A.tau(), c->d_mats, c->geom, c->sc.p_vol, dt,
+                                                       c->offsets, c->active, c->n_active, c->accum, c->dirty);
+  check_launch("k_p2g_cells");
 }
 
 void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   Timed t(c, MPM_K_GRID);
   if (c->p2g_mode == 1) {
     k_grid_dense<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->sc, dt, c->n_nodes, c->accum, c->vel,
-                                                                  c->live, c->collider_set(), s, inv_frame_dt,
-                                                                  c->d_vmax);
+                                                                  c->live, c->collider_set(), s, inv_frame_dt);
     check_launch("k_grid_dense");
   } else {
     const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
     k_grid_bricks<<<blocks, 256, 0, c->stream>>>(c->geom, c->sc, dt, c->n_bricks, c->dirty, c->live, c->accum,
-                                                 c->vel, c->collider_set(), s, inv_frame_dt, c->d_vmax);
+                                                 c->vel, c->collider_set(), s, inv_frame_dt);
     check_launch("k_grid_bricks");
   }
 }
@@ -423,8 +440,9 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   if (c->n == 0) return;
   Timed t(c, MPM_K_G2P);
   k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel,
-                                                  c->A.tau(), c->d_vmax, c->collider_set(), s, inv_frame_dt);
+                                                  c->A.tau(), block_max(c), c->collider_set(), s, inv_frame_dt);
   check_launch("k_g2p");
+  c->block_max_count = grid_for(c->n);
   c->vmax_valid = true;
 }
 
@@ -436,19 +454,23 @@ void substep(mpm_ctx* c, double dt, double s, double frame_dt) {
   c->steps_since_sort += 1;
 }
 
+// max |v| over the particles (kernels.py:266-269): the last G2P left per-workgroup maxima; after any
+// state change a speed pass recomputes them
 float max_speed(mpm_ctx* c) {
-  if (!c->vmax_valid) {
-    HIP_CHECK(hipMemsetAsync(c->d_vmax, 0, sizeof(unsigned int), c->stream));
+  {
     Timed t(c, MPM_K_OTHER);
-    k_max_speed<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), (int)c->n, c->d_vmax);
-    check_launch("k_max_speed");
-    c->vmax_valid = true;
+    if (!c->vmax_valid) {
+      k_speed_partials<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), (int)c->n, block_max(c));
+      check_launch("k_speed_partials");
+      c->block_max_count = grid_for(c->n);
+      c->vmax_valid = true;
+    }
+    k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->d_block_max, (int)c->block_max_count, c->d_vmax);
+    check_launch("k_reduce_max");
   }
-  unsigned int bits = 0;
-  HIP_CHECK(hipMemcpyAsync(&bits, c->d_vmax, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+  float f = 0.0f;
+  HIP_CHECK(hipMemcpyAsync(&f, c->d_vmax, sizeof f, hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  float f;
-  std::memcpy(&f, &bits, sizeof f);
   return f;
 }
 
@@ -538,7 +560,28 @@ int64_t remove_escaped(mpm_ctx* c) {
 }
 
 // ---- colliders -------------------------------------------------------------------------------
+// per-collider records (pose, margin, offsets into the shared tables); small, rewritten on every pose
+// update with an async copy on the context's stream
+void upload_collider_records(mpm_ctx* c) {
+  if (c->colliders.empty()) return;
+  std::vector<ColliderDev> cols;
+  int tri = 0, cell = 0;
+  for (auto& h : c->colliders) {
+    ColliderDev d = h.dev;
+    d.tri_offset = tri;
+    d.cell_offset = cell;
+    tri += (int)h.tris.size();
+    cell += h.cell_start.empty() ? 0 : (int)h.cell_start.size() - 1;
+    cols.push_back(d);
+  }
+  c->col_records = cols;
+  HIP_CHECK(hipMemcpyAsync(c->d_cols, c->col_records.data(), cols.size() * sizeof(ColliderDev),
+                           hipMemcpyHostToDevice, c->stream));
+}
+
+// meshes, pseudonormals and candidate lists of every collider (after set_colliders or a grid rebuild)
 void upload_colliders(mpm_ctx* c) {
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   for (void* p : {(void*)c->d_cols, (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start,
                   (void*)c->d_cell_tris})
     dev_free(p);
@@ -548,35 +591,30 @@ void upload_colliders(mpm_ctx* c) {
   c->d_cell_start = nullptr;
   c->d_cell_tris = nullptr;
   if (c->colliders.empty()) return;
-  std::vector<ColliderDev> cols;
   std::vector<TriGeom> tris;
   std::vector<TriNormals> normals;
   std::vector<uint32_t> starts, cand;
   for (auto& h : c->colliders) {
-    ColliderDev d = h.dev;
-    d.tri_offset = (int)tris.size();
-    d.cell_offset = (int)starts.size();
     tris.insert(tris.end(), h.tris.begin(), h.tris.end());
     normals.insert(normals.end(), h.normals.begin(), h.normals.end());
     const uint32_t base = (uint32_t)cand.size();
     const size_t cells = h.cell_start.empty() ? 0 : h.cell_start.size() - 1;
     for (size_t i = 0; i < cells; ++i) starts.push_back(base + h.cell_start[i]);
     cand.insert(cand.end(), h.cell_tris.begin(), h.cell_tris.end());
-    cols.push_back(d);
   }
   starts.push_back((uint32_t)cand.size());
-  c->d_cols = dev_alloc<ColliderDev>(cols.size());
+  c->d_cols = dev_alloc<ColliderDev>(c->colliders.size());
   c->d_tris = dev_alloc<TriGeom>(tris.size());
   c->d_normals = dev_alloc<TriNormals>(normals.size());
   c->d_cell_start = dev_alloc<uint32_t>(starts.size());
   c->d_cell_tris = dev_alloc<uint32_t>(cand.size());
-  HIP_CHECK(hipMemcpy(c->d_cols, cols.data(), cols.size() * sizeof(ColliderDev), hipMemcpyHostToDevice));
   if (!tris.empty()) {
     HIP_CHECK(hipMemcpy(c->d_tris, tris.data(), tris.size() * sizeof(TriGeom), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->d_normals, normals.data(), normals.size() * sizeof(TriNormals), hipMemcpyHostToDevice));
   }
   HIP_CHECK(hipMemcpy(c->d_cell_start, starts.data(), starts.size() * 4, hipMemcpyHostToDevice));
   if (!cand.empty()) HIP_CHECK(hipMemcpy(c->d_cell_tris, cand.data(), cand.size() * 4, hipMemcpyHostToDevice));
+  upload_collider_records(c);
 }
 
 // candidate-triangle voxel grid of one collider over its AABB grown by `pad` (local units)
@@ -779,13 +817,14 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     if (bricks * 64 > ((int64_t)1 << 30)) fail(MPM_ERR_VALUE, "grid too large");
     c->n_bricks = (int)bricks;
     c->n_nodes = (int)(bricks * 64);
+    c->n_cells = (int)(bricks * 64);
     c->key_bits = 1;
-    while (((int64_t)1 << c->key_bits) < bricks) ++c->key_bits;
+    while (((int64_t)1 << c->key_bits) < c->n_cells) ++c->key_bits;
     c->accum = dev_alloc<float4>(c->n_nodes);
     c->vel = dev_alloc<float4>(c->n_nodes);
     c->dirty = dev_alloc<uint32_t>(bricks);
     c->live = dev_alloc<uint8_t>(bricks);
-    c->offsets = dev_alloc<uint32_t>(bricks + 1);
+    c->offsets = dev_alloc<uint32_t>((size_t)c->n_cells + 1);
     c->active = dev_alloc<uint32_t>(bricks);
     c->n_active = dev_alloc<uint32_t>(1);
     HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
@@ -793,7 +832,7 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     HIP_CHECK(hipMemsetAsync(c->dirty, 0, bricks * 4, c->stream));
     HIP_CHECK(hipMemsetAsync(c->live, 0, bricks, c->stream));
     HIP_CHECK(hipMemsetAsync(c->n_active, 0, 4, c->stream));
-    c->d_vmax = dev_alloc<unsigned int>(1);
+    c->d_vmax = dev_alloc<float>(1);
     HIP_CHECK(hipMemsetAsync(c->d_vmax, 0, 4, c->stream));
     reserve(c, 256);
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -814,7 +853,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
-                  (void*)c->active, (void*)c->n_active, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_cols,
+                  (void*)c->active, (void*)c->n_active, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
                   (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris})
     dev_free(p);
   c->scratch.release();
@@ -929,6 +968,7 @@ int32_t mpm_set_collider_poses(mpm_ctx* c, int32_t n, const mpm_collider_pose* p
     if (n != (int32_t)c->colliders.size())
       fail(MPM_ERR_VALUE, "expected " + std::to_string(c->colliders.size()) + " collider transforms");
     if (n > 0 && !poses) fail(MPM_ERR_VALUE, "null pose array");
+    bool rebuilt = false;
     for (int i = 0; i < n; ++i) {
       ColliderHost& h = c->colliders[i];
       ColliderDev& d = h.dev;
@@ -940,9 +980,15 @@ int32_t mpm_set_collider_poses(mpm_ctx* c, int32_t n, const mpm_collider_pose* p
       std::memcpy(d.S0, p.S0, sizeof d.S0);
       std::memcpy(d.S1, p.S1, sizeof d.S1);
       d.margin = p.margin;
-      if (d.margin > h.built_pad) build_candidates(c, h, d.margin * 1.25f);
+      if (d.margin > h.built_pad) {
+        build_candidates(c, h, d.margin * 1.25f);
+        rebuilt = true;
+      }
     }
-    if (n > 0) upload_colliders(c);
+    if (n == 0) return;
+    // the previous substeps may still read the records: order the copy after them on the stream
+    if (rebuilt) upload_colliders(c);
+    else upload_collider_records(c);
   });
 }
 
@@ -1059,10 +1105,10 @@ int32_t mpm_set_grid(mpm_ctx* c, const float* grid_v, const float* grid_m) {
   });
 }
 
-int32_t mpm_get_binning(mpm_ctx* c, int32_t* keys, int32_t* perm, int32_t* offsets, int64_t* n_bricks) {
-  if (!c || !n_bricks) return MPM_ERR_VALUE;
+int32_t mpm_get_binning(mpm_ctx* c, int32_t* keys, int32_t* perm, int32_t* offsets, int64_t* n_bins) {
+  if (!c || !n_bins) return MPM_ERR_VALUE;
   return run(c, [&] {
-    *n_bricks = c->n_bricks;
+    *n_bins = c->n_cells;
     if (!c->last_perm) {
       if (keys || perm || offsets) fail(MPM_ERR_VALUE, "no binning has run yet");
       return;
@@ -1070,7 +1116,7 @@ int32_t mpm_get_binning(mpm_ctx* c, int32_t* keys, int32_t* perm, int32_t* offse
     if (keys && c->n) HIP_CHECK(hipMemcpyAsync(keys, c->keys0, c->n * 4, hipMemcpyDeviceToHost, c->stream));
     if (perm && c->n) HIP_CHECK(hipMemcpyAsync(perm, c->last_perm, c->n * 4, hipMemcpyDeviceToHost, c->stream));
     if (offsets)
-      HIP_CHECK(hipMemcpyAsync(offsets, c->offsets, (c->n_bricks + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipMemcpyAsync(offsets, c->offsets, ((size_t)c->n_cells + 1) * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }

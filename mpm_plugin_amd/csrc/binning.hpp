@@ -1,10 +1,11 @@
-// Particle binning by 4x4x4-node brick on gfx950: stable LSD radix sort of (brick key, slot) pairs,
-// brick offsets and the list of occupied bricks.
+// Particle binning on gfx950: stable LSD radix sort of (cell key, slot) pairs, cell offsets and the list
+// of occupied bricks.
 //
-// A particle's key is the brick of its stencil base node floor(x / dx - 0.5) (the node the quadratic
-// stencil starts at, kernels.py:26-27), so all 27 stencil nodes of a particle lie in the brick's
-// 6x6x6-node neighbourhood. The sort is stable: particles of one brick keep their previous relative
-// order, and the permutation equals numpy.argsort(keys, kind="stable") bit for bit.
+// A particle's cell is its stencil base node floor(x / dx - 0.5) (the node its quadratic stencil starts
+// at, kernels.py:26-27). Cells are numbered brick-major: key = brick * 64 + (cell within the 4x4x4-node
+// brick), so one brick's particles are contiguous after the sort and, inside the brick, grouped by cell.
+// The sort is stable: particles of one cell keep their previous relative order, and the permutation equals
+// numpy.argsort(keys, kind="stable") bit for bit.
 //
 // Radix sort: 8-bit digits, least significant first. Each pass runs
 //   k_radix_hist     per-tile digit histogram, stored digit-major (hist[d * tiles + t])
@@ -24,16 +25,16 @@ constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// keys[p] = brick of particle slot p; vals[p] = p
-__global__ void __launch_bounds__(256) k_brick_keys(Particles P, GridGeom g, int n, uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals) {
+// keys[p] = cell key of particle slot p; vals[p] = p
+__global__ void __launch_bounds__(256) k_cell_keys(Particles P, GridGeom g, int n, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const float x[3] = {P.x[p], P.x[P.cap + p], P.x[2 * P.cap + p]};
   int base[3];
   float fx[3];
   stencil_base(g, x, base, fx);
-  keys[p] = (uint32_t)brick_of_base(g, base);
+  keys[p] = cell_key(g, base);
   vals[p] = (uint32_t)p;
 }
 
@@ -176,17 +177,27 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* __restrict_
     if (base + k < n) out[base + k] += add;
 }
 
-// offsets[b] = first sorted position with key >= b, for b in [0, n_bricks]; occupied bricks are
-// appended to `active` (their order there is irrelevant to every result)
-__global__ void __launch_bounds__(256) k_brick_bounds(const uint32_t* __restrict__ sorted_keys, int n, int n_bricks,
-                                                      uint32_t* __restrict__ offsets, uint32_t* __restrict__ active,
-                                                      uint32_t* __restrict__ n_active) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
-  const int prev = (i == 0) ? -1 : (int)sorted_keys[i - 1];
-  const int cur = (i == n) ? n_bricks : (int)sorted_keys[i];
-  for (int b = prev + 1; b <= cur; ++b) offsets[b] = (uint32_t)i;
-  if (i < n && cur != prev) active[atomicAdd(n_active, 1u)] = (uint32_t)cur;
+// offsets[c] = first sorted position with key >= c, for c in [0, n_cells] (binary search per cell)
+__global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict__ sorted_keys, int n, int n_cells,
+                                                     uint32_t* __restrict__ offsets) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > n_cells) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int)sorted_keys[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  offsets[c] = (uint32_t)lo;
+}
+
+// bricks holding at least one particle are appended to `active` (their order there is irrelevant to
+// every result)
+__global__ void __launch_bounds__(256) k_active_bricks(const uint32_t* __restrict__ offsets, int n_bricks,
+                                                       uint32_t* __restrict__ active, uint32_t* __restrict__ n_active) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_bricks) return;
+  if (offsets[64 * (b + 1)] > offsets[64 * b]) active[atomicAdd(n_active, 1u)] = (uint32_t)b;
 }
 
 // gather every per-slot particle array into sorted order: dst[i] = src[perm[i]]

@@ -94,6 +94,11 @@ __device__ __forceinline__ int brick_of_base(const GridGeom& g, const int (&base
   return ((base[0] >> 2) * g.bdims[1] + (base[1] >> 2)) * g.bdims[2] + (base[2] >> 2);
 }
 
+// brick-major cell key: brick * 64 + ((i & 3) * 16 + (j & 3) * 4 + (k & 3))
+__device__ __forceinline__ uint32_t cell_key(const GridGeom& g, const int (&base)[3]) {
+  return (uint32_t)brick_of_base(g, base) * 64u + (uint32_t)(((base[0] & 3) << 4) | ((base[1] & 3) << 2) | (base[2] & 3));
+}
+
 // Kirchhoff stress of one particle (kernels.py:115-124): Hencky for von Mises and Drucker-Prager,
 // fixed corotated for snow (with hardening h = min(exp(xi (1 - Jp)), h_max)) and elastic.
 __device__ __forceinline__ M3<float> particle_tau(const MatDev& m, const M3<float>& F, float Jp) {
@@ -230,30 +235,6 @@ __device__ __forceinline__ float4 update_node(const GridGeom& g, const StepConst
     if (sc.hi_on[ax] != 0 && node[ax] >= sc.wall_hi_node[ax] && vel[ax] > 0.0f) wall_contact(vel, ax, sc.wall_friction);
   }
   return make_float4(vel[0], vel[1], vel[2], m);
-}
-
-// ---------------------------------------------------------------------------------------------
-// max |v| for the CFL step (kernels.py:266-269). Speeds are >= 0, so their IEEE bit patterns order
-// like unsigned integers; a NaN or inf speed yields a non-finite maximum, which the host treats as 0.
-__global__ void __launch_bounds__(256) k_max_speed(Particles P, int n, unsigned int* __restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  float s = 0.0f;
-  if (p < n) {
-    const float a = P.v[p], b = P.v[P.cap + p], c = P.v[2 * P.cap + p];
-    s = sqrtf(a * a + b * b + c * c);
-    if (!(s == s)) s = __int_as_float(0x7fc00000);
-  }
-  unsigned int u = __float_as_uint(s);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) u = max(u, (unsigned int)__shfl_xor((int)u, off, 64));
-  __shared__ unsigned int wmax[4];
-  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = u;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int b = wmax[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = max(b, wmax[w]);
-    atomicMax(out, b);
-  }
 }
 
 // ---------------------------------------------------------------------------------------------

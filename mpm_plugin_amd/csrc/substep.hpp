@@ -21,72 +21,79 @@ namespace mpm {
 
 constexpr int kWin = 8;
 constexpr int kWinNodes = kWin * kWin * kWin;
-constexpr int kP2GThreads = 256;
 
 __device__ __forceinline__ void mark_brick(const GridGeom& g, uint32_t* dirty, int bx, int by, int bz) {
   if (bx < 0 || by < 0 || bz < 0 || bx >= g.bdims[0] || by >= g.bdims[1] || bz >= g.bdims[2]) return;
   dirty[(bx * g.bdims[1] + by) * g.bdims[2] + bz] = 1u;
 }
 
-// scatter of one particle: affine momentum and mass to its 27 stencil nodes, through LDS when the
-// stencil is inside the window, else through HBM atomics
-__device__ __forceinline__ void p2g_particle(const Particles& P, const float* __restrict__ tau,
-                                             const MatDev* __restrict__ mats, const GridGeom& g, float p_vol,
-                                             float dt, int64_t p, const int (&wo)[3], float (*s_acc)[kWinNodes],
-                                             float4* __restrict__ accum, uint32_t* __restrict__ dirty) {
+// The affine momentum of one particle (kernels.py:126-131): A = -dt p_vol 4 / dx^2 tau + m C, with the
+// stencil base, the per-axis weights and node offsets of its quadratic stencil.
+struct P2GParticle {
+  float A[3][3];
+  float mv[3];
+  float pm;
+  int base[3];
+  float wx[3][3], dp[3][3];
+};
+
+__device__ __forceinline__ void load_p2g_particle(const Particles& P, const float* __restrict__ tau,
+                                                  const MatDev* __restrict__ mats, const GridGeom& g, float k,
+                                                  int64_t p, P2GParticle& q) {
   const int64_t cap = P.cap;
   const float pm = mats[P.mat[p]].p_mass;
-  const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
   const float t00 = tau[p], t11 = tau[cap + p], t22 = tau[2 * cap + p];
   const float t01 = tau[3 * cap + p], t02 = tau[4 * cap + p], t12 = tau[5 * cap + p];
-  float A[3][3];
-  A[0][0] = k * t00 + pm * P.C[p];
-  A[0][1] = k * t01 + pm * P.C[cap + p];
-  A[0][2] = k * t02 + pm * P.C[2 * cap + p];
-  A[1][0] = k * t01 + pm * P.C[3 * cap + p];
-  A[1][1] = k * t11 + pm * P.C[4 * cap + p];
-  A[1][2] = k * t12 + pm * P.C[5 * cap + p];
-  A[2][0] = k * t02 + pm * P.C[6 * cap + p];
-  A[2][1] = k * t12 + pm * P.C[7 * cap + p];
-  A[2][2] = k * t22 + pm * P.C[8 * cap + p];
+  q.A[0][0] = k * t00 + pm * P.C[p];
+  q.A[0][1] = k * t01 + pm * P.C[cap + p];
+  q.A[0][2] = k * t02 + pm * P.C[2 * cap + p];
+  q.A[1][0] = k * t01 + pm * P.C[3 * cap + p];
+  q.A[1][1] = k * t11 + pm * P.C[4 * cap + p];
+  q.A[1][2] = k * t12 + pm * P.C[5 * cap + p];
+  q.A[2][0] = k * t02 + pm * P.C[6 * cap + p];
+  q.A[2][1] = k * t12 + pm * P.C[7 * cap + p];
+  q.A[2][2] = k * t22 + pm * P.C[8 * cap + p];
+  q.pm = pm;
   const float xo[3] = {P.x[p], P.x[cap + p], P.x[2 * cap + p]};
-  const float mv[3] = {pm * P.v[p], pm * P.v[cap + p], pm * P.v[2 * cap + p]};
-  int base[3];
+  q.mv[0] = pm * P.v[p];
+  q.mv[1] = pm * P.v[cap + p];
+  q.mv[2] = pm * P.v[2 * cap + p];
   float fx[3];
-  stencil_base(g, xo, base, fx);
-  float wx[3][3], dp[3][3];
+  stencil_base(g, xo, q.base, fx);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      wx[a][i] = bspline_w(fx[a], i);
-      dp[a][i] = ((float)i - fx[a]) * g.dx;
+      q.wx[a][i] = bspline_w(fx[a], i);
+      q.dp[a][i] = ((float)i - fx[a]) * g.dx;
     }
-  const int l0 = base[0] - wo[0], l1 = base[1] - wo[1], l2 = base[2] - wo[2];
+}
+
+// Scatter of a particle whose stencil is not its sorted cell's (it drifted since the last sort): through
+// atomics into the wave's window when the stencil fits the window, else straight to HBM.
+__device__ __forceinline__ void p2g_scatter_atomic(const P2GParticle& q, const GridGeom& g, const int (&wo)[3],
+                                                float (*win)[kWinNodes], float4* __restrict__ accum,
+                                                uint32_t* __restrict__ dirty) {
+  const int l0 = q.base[0] - wo[0], l1 = q.base[1] - wo[1], l2 = q.base[2] - wo[2];
   const bool in_window = l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 <= kWin - 3 && l1 <= kWin - 3 && l2 <= kWin - 3;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-#pragma unroll
+  for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      const float wij = wx[0][i] * wx[1][j];
-      const float ax = mv[0] + A[0][0] * dp[0][i] + A[0][1] * dp[1][j];
-      const float ay = mv[1] + A[1][0] * dp[0][i] + A[1][1] * dp[1][j];
-      const float az = mv[2] + A[2][0] * dp[0][i] + A[2][1] * dp[1][j];
-#pragma unroll
+      const float wij = q.wx[0][i] * q.wx[1][j];
       for (int kk = 0; kk < 3; ++kk) {
-        const float w = wij * wx[2][kk];
-        const float vx = w * (ax + A[0][2] * dp[2][kk]);
-        const float vy = w * (ay + A[1][2] * dp[2][kk]);
-        const float vz = w * (az + A[2][2] * dp[2][kk]);
-        const float vm = w * pm;
+        const float w = wij * q.wx[2][kk];
+        const float d0 = q.dp[0][i], d1 = q.dp[1][j], d2 = q.dp[2][kk];
+        const float vx = w * (q.mv[0] + (q.A[0][0] * d0 + q.A[0][1] * d1 + q.A[0][2] * d2));
+        const float vy = w * (q.mv[1] + (q.A[1][0] * d0 + q.A[1][1] * d1 + q.A[1][2] * d2));
+        const float vz = w * (q.mv[2] + (q.A[2][0] * d0 + q.A[2][1] * d1 + q.A[2][2] * d2));
+        const float vm = w * q.pm;
         if (in_window) {
           const int li = ((l0 + i) * kWin + (l1 + j)) * kWin + (l2 + kk);
-          atomicAdd(&s_acc[0][li], vx);
-          atomicAdd(&s_acc[1][li], vy);
-          atomicAdd(&s_acc[2][li], vz);
-          atomicAdd(&s_acc[3][li], vm);
+          atomicAdd(&win[0][li], vx);
+          atomicAdd(&win[1][li], vy);
+          atomicAdd(&win[2][li], vz);
+          atomicAdd(&win[3][li], vm);
         } else {
-          float4* node = accum + node_addr(g, base[0] + i, base[1] + j, base[2] + kk);
+          float4* node = accum + node_addr(g, q.base[0] + i, q.base[1] + j, q.base[2] + kk);
           atomicAdd(&node->x, vx);
           atomicAdd(&node->y, vy);
           atomicAdd(&node->z, vz);
@@ -94,48 +101,112 @@ __device__ __forceinline__ void p2g_particle(const Particles& P, const float* __
         }
       }
     }
-  }
   if (!in_window) {
     // the stencil spans at most two bricks per axis
-#pragma unroll
     for (int i = 0; i < 2; ++i)
-#pragma unroll
       for (int j = 0; j < 2; ++j)
-#pragma unroll
         for (int kk = 0; kk < 2; ++kk)
-          mark_brick(g, dirty, (base[0] + 2 * i) >> 2, (base[1] + 2 * j) >> 2, (base[2] + 2 * kk) >> 2);
+          mark_brick(g, dirty, (q.base[0] + 2 * i) >> 2, (q.base[1] + 2 * j) >> 2, (q.base[2] + 2 * kk) >> 2);
   }
 }
 
-__global__ void __launch_bounds__(kP2GThreads) k_p2g_binned(Particles P, const float* __restrict__ tau,
-                                                            const MatDev* __restrict__ mats, GridGeom g, float p_vol,
-                                                            float dt, const uint32_t* __restrict__ offsets,
-                                                            const uint32_t* __restrict__ active,
-                                                            const uint32_t* __restrict__ n_active_ptr,
-                                                            float4* __restrict__ accum, uint32_t* __restrict__ dirty) {
-  __shared__ float s_acc[4][kWinNodes];
+// orders this wave's LDS accesses across lanes (a wave's LDS operations complete in issue order)
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// P2G with one wave per occupied brick and one lane per cell of the brick. Particles are sorted by cell,
+// so lane l walks the particles of cell l and sums their 27 stencil contributions in registers. The
+// 64 lanes of a wave own 64 different cells, so when they then add their sums into the wave's private
+// LDS window every store of one instruction hits a different node: plain read-add-write, no LDS atomics.
+// The window is flushed to the bricked HBM grid with global atomics (neighbouring windows overlap).
+__global__ void __launch_bounds__(256) k_p2g_cells(Particles P, const float* __restrict__ tau,
+                                                   const MatDev* __restrict__ mats, GridGeom g, float p_vol, float dt,
+                                                   const uint32_t* __restrict__ cell_off,
+                                                   const uint32_t* __restrict__ active,
+                                                   const uint32_t* __restrict__ n_active_ptr,
+                                                   float4* __restrict__ accum, uint32_t* __restrict__ dirty) {
+  constexpr int kWaves = 4;
+  __shared__ float s_win[kWaves][4][kWinNodes];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float (*win)[kWinNodes] = s_win[wave];
   const uint32_t n_active = *n_active_ptr;
-  for (uint32_t w = blockIdx.x; w < n_active; w += gridDim.x) {
+  const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
+  const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
+  for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves) {
     const uint32_t b = active[w];
     const int bz = (int)(b % (uint32_t)g.bdims[2]);
     const int by = (int)((b / (uint32_t)g.bdims[2]) % (uint32_t)g.bdims[1]);
     const int bx = (int)(b / ((uint32_t)g.bdims[2] * (uint32_t)g.bdims[1]));
     const int wo[3] = {4 * bx - 1, 4 * by - 1, 4 * bz - 1};
-    for (int i = threadIdx.x; i < kWinNodes; i += kP2GThreads) {
-      s_acc[0][i] = 0.0f;
-      s_acc[1][i] = 0.0f;
-      s_acc[2][i] = 0.0f;
-      s_acc[3][i] = 0.0f;
+    for (int i = lane; i < kWinNodes; i += 64) {
+      win[0][i] = 0.0f;
+      win[1][i] = 0.0f;
+      win[2][i] = 0.0f;
+      win[3][i] = 0.0f;
     }
-    __syncthreads();
-    const uint32_t start = offsets[b], end = offsets[b + 1];
-    for (uint32_t s = start + threadIdx.x; s < end; s += kP2GThreads)
-      p2g_particle(P, tau, mats, g, p_vol, dt, (int64_t)s, wo, s_acc, accum, dirty);
-    __syncthreads();
-    for (int i = threadIdx.x; i < kWinNodes; i += kP2GThreads) {
+    wave_lds_fence();
+    const int cb[3] = {4 * bx + cx, 4 * by + cy, 4 * bz + cz};
+    const uint32_t p0 = cell_off[64 * b + lane], p1 = cell_off[64 * b + lane + 1];
+    float acc[3][3][3][4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[i][j][kk][c] = 0.0f;
+    for (uint32_t p = p0; p < p1; ++p) {
+      P2GParticle q;
+      load_p2g_particle(P, tau, mats, g, k, (int64_t)p, q);
+      if (q.base[0] == cb[0] && q.base[1] == cb[1] && q.base[2] == cb[2]) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const float wij = q.wx[0][i] * q.wx[1][j];
+            const float ax = q.mv[0] + q.A[0][0] * q.dp[0][i] + q.A[0][1] * q.dp[1][j];
+            const float ay = q.mv[1] + q.A[1][0] * q.dp[0][i] + q.A[1][1] * q.dp[1][j];
+            const float az = q.mv[2] + q.A[2][0] * q.dp[0][i] + q.A[2][1] * q.dp[1][j];
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+              const float wgt = wij * q.wx[2][kk];
+              acc[i][j][kk][0] += wgt * (ax + q.A[0][2] * q.dp[2][kk]);
+              acc[i][j][kk][1] += wgt * (ay + q.A[1][2] * q.dp[2][kk]);
+              acc[i][j][kk][2] += wgt * (az + q.A[2][2] * q.dp[2][kk]);
+              acc[i][j][kk][3] += wgt * q.pm;
+            }
+          }
+        }
+      } else {
+        p2g_scatter_atomic(q, g, wo, win, accum, dirty);
+      }
+    }
+    wave_lds_fence();
+    // Node (cx + 1 + i, ...) of one lane is node (cx' + 1 + i', ...) of a neighbouring lane at another
+    // offset, so the 27 read-add-writes must stay in program order across lanes: the fence after each
+    // offset keeps the compiler from batching the reads of later offsets ahead of earlier writes (a
+    // wave's LDS operations then complete in issue order). Lanes without particles add nothing.
+    const bool any = p1 > p0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+          const int li = ((cx + 1 + i) * kWin + (cy + 1 + j)) * kWin + (cz + 1 + kk);
+          if (any) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) win[c][li] += acc[i][j][kk][c];
+          }
+          wave_lds_fence();
+        }
+    for (int i = lane; i < kWinNodes; i += 64) {
       const int gi = wo[0] + (i >> 6), gj = wo[1] + ((i >> 3) & 7), gk = wo[2] + (i & 7);
       if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
-      const float vx = s_acc[0][i], vy = s_acc[1][i], vz = s_acc[2][i], vm = s_acc[3][i];
+      const float vx = win[0][i], vy = win[1][i], vz = win[2][i], vm = win[3][i];
       if (vx == 0.0f && vy == 0.0f && vz == 0.0f && vm == 0.0f) continue;
       float4* node = accum + node_addr(g, gi, gj, gk);
       if (vx != 0.0f) atomicAdd(&node->x, vx);
@@ -143,11 +214,8 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g_binned(Particles P, const f
       if (vz != 0.0f) atomicAdd(&node->z, vz);
       if (vm != 0.0f) atomicAdd(&node->w, vm);
     }
-    if (threadIdx.x < 27) {
-      const int t = threadIdx.x;
-      mark_brick(g, dirty, bx - 1 + t / 9, by - 1 + (t / 3) % 3, bz - 1 + t % 3);
-    }
-    __syncthreads();
+    if (lane < 27) mark_brick(g, dirty, bx - 1 + lane / 9, by - 1 + (lane / 3) % 3, bz - 1 + lane % 3);
+    wave_lds_fence();
   }
 }
 
@@ -155,11 +223,9 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g_binned(Particles P, const f
 __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, float dt, int n_bricks,
                                                      uint32_t* __restrict__ dirty, uint8_t* __restrict__ live,
                                                      float4* __restrict__ accum, float4* __restrict__ vel,
-                                                     ColliderSet cs, float s, float inv_frame_dt,
-                                                     unsigned int* __restrict__ vmax_bits) {
+                                                     ColliderSet cs, float s, float inv_frame_dt) {
   __shared__ ColliderFrame frames[kMaxColliders];
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && vmax_bits) *vmax_bits = 0u;  // G2P of this substep refills it
   const int brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (brick >= n_bricks) return;
   const int lane = threadIdx.x & 63;
@@ -184,10 +250,9 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
 __global__ void __launch_bounds__(256) k_grid_dense(GridGeom g, StepConsts sc, float dt, int n_nodes,
                                                     float4* __restrict__ accum, float4* __restrict__ vel,
                                                     uint8_t* __restrict__ live, ColliderSet cs, float s,
-                                                    float inv_frame_dt, unsigned int* __restrict__ vmax_bits) {
+                                                    float inv_frame_dt) {
   __shared__ ColliderFrame frames[kMaxColliders];
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && vmax_bits) *vmax_bits = 0u;
   const int addr = blockIdx.x * blockDim.x + threadIdx.x;
   if (addr >= n_nodes) return;
   if ((addr & 63) == 0) live[addr >> 6] = 1;
@@ -207,7 +272,7 @@ __global__ void __launch_bounds__(256) k_grid_dense(GridGeom g, StepConsts sc, f
 // G2P (kernels.py:183-263): gather v and C, advect with wall projection and finite-only clamp,
 // F <- (I + dt C) F, per-model return map. One SVD of the new F per particle also gives the Kirchhoff
 // stress that the next P2G scatters (stored as 6 symmetric entries), so P2G needs no SVD. The
-// largest |v| of the substep is reduced into *vmax_bits for the next CFL step.
+// largest |v| of each workgroup goes to block_max for the next CFL step.
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
@@ -312,20 +377,57 @@ __device__ __forceinline__ void g2p_particle(Particles& P, const MatDev* __restr
   store_tau(tau_out, cap, p, t);
 }
 
+// largest speed of the workgroup into block_max[blockIdx.x]; NaN (which fmaxf would drop) counts as +inf,
+// which the host also reads as "not finite"
+__device__ __forceinline__ void block_max_speed(float speed, float* __restrict__ block_max) {
+  __shared__ float s_max[4];
+  if (!(speed == speed)) speed = __int_as_float(0x7f800000);
+  const float wm = wave_max(speed);
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = s_max[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = fmaxf(b, s_max[w]);
+    block_max[blockIdx.x] = b;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restrict__ mats, GridGeom g,
                                              StepConsts sc, int n, float dt, const float4* __restrict__ vel,
-                                             float* __restrict__ tau, unsigned int* __restrict__ vmax_bits,
+                                             float* __restrict__ tau, float* __restrict__ block_max,
                                              ColliderSet cs, float s, float inv_frame_dt) {
   __shared__ ColliderFrame frames[kMaxColliders];
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float speed = 0.0f;
   if (p < n) g2p_particle(P, mats, g, sc, dt, vel, tau, p, cs, frames, speed);
-  // NaN compares false in fmaxf and would vanish from the maximum: map it to +inf, which the host
-  // also reads as "not finite"
-  if (!(speed == speed)) speed = __int_as_float(0x7f800000);
-  const float wm = wave_max(speed);
-  if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(wm));
+  block_max_speed(speed, block_max);
+}
+
+// |v| of every particle reduced per workgroup (kernels.py:266-269), when no G2P has produced it
+__global__ void __launch_bounds__(256) k_speed_partials(Particles P, int n, float* __restrict__ block_max) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  float speed = 0.0f;
+  if (p < n) {
+    const float a = P.v[p], b = P.v[P.cap + p], c = P.v[2 * P.cap + p];
+    speed = sqrtf(a * a + b * b + c * c);
+  }
+  block_max_speed(speed, block_max);
+}
+
+// one workgroup: the maximum of the per-workgroup maxima
+__global__ void __launch_bounds__(256) k_reduce_max(const float* __restrict__ block_max, int count,
+                                                    float* __restrict__ out) {
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    const float v = block_max[i];
+    m = (v == v) ? fmaxf(m, v) : __int_as_float(0x7f800000);
+  }
+  __shared__ float s_max[4];
+  const float wm = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
 }
 
 }  // namespace mpm

@@ -1,7 +1,8 @@
-"""Brick binning on the GPU: keys, permutation and brick offsets are bit-exact against a CPU stable sort.
+"""Cell binning on the GPU: keys, permutation and cell offsets are bit-exact against a CPU stable sort.
 
-The key of a particle is the 4x4x4-node brick of its stencil base node floor(x / dx - 0.5)
-(src/mpm_core/backends/warp/kernels.py:25-27), computed here in float32 exactly as on the device.
+The key of a particle is its stencil base node floor(x / dx - 0.5) (src/mpm_core/backends/warp/kernels.py:25-27)
+numbered brick-major (brick * 64 + cell within the 4x4x4-node brick), computed here in float32 exactly as on
+the device.
 """
 
 import numpy as np
@@ -17,7 +18,9 @@ def cpu_keys(x: np.ndarray, dims, origin, inv_dx) -> np.ndarray:
     base = np.clip(base, 0, np.asarray(dims) - 3)
     bd = (np.asarray(dims) + 3) // 4
     b = base >> 2
-    return ((b[:, 0] * bd[1] + b[:, 1]) * bd[2] + b[:, 2]).astype(np.int32)
+    brick = (b[:, 0] * bd[1] + b[:, 1]) * bd[2] + b[:, 2]
+    cell = (base[:, 0] & 3) * 16 + (base[:, 1] & 3) * 4 + (base[:, 2] & 3)
+    return (brick * 64 + cell).astype(np.int32)
 
 
 @pytest.mark.parametrize("res,n,seed", [(32, 3000, 0), (64, 40000, 1), (128, 200000, 2)])
