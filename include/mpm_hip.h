@@ -37,6 +37,7 @@ extern "C" {
 #define MPM_MODEL_SNOW 3
 
 typedef struct mpm_ctx mpm_ctx;
+typedef struct mpm_batch mpm_batch;
 
 /* Domain constants, derived on the host in FP64 and rounded to FP32 as in
  * backends/warp/solver.py:37-54. */
@@ -163,6 +164,29 @@ int32_t mpm_get_kernel_time(mpm_ctx* ctx, int32_t kernel, double* total_ms, int6
 int32_t mpm_reset_timing(mpm_ctx* ctx);
 /* wait for all work on the context's stream */
 int32_t mpm_synchronize(mpm_ctx* ctx);
+
+/* ---- batches of scenes ---------------------------------------------------------------------
+ * No counterpart in the reference, whose runner bakes one scene per process (bake.py:116-208): a batch
+ * advances many contexts (scenes) by one frame each through one launch sequence per substep, captured
+ * as HIP graphs. Each scene keeps its own state and still answers every mpm_* call between frames;
+ * while it belongs to a batch its operations run on the batch's stream. A context belongs to at most
+ * one batch; mpm_destroy drops it from its batch. */
+int32_t mpm_batch_create(int32_t device, mpm_batch** out);
+int32_t mpm_batch_destroy(mpm_batch* batch);
+/* replace the batch's scenes (all on the batch's device; order fixes the index of frame_dt/out below) */
+int32_t mpm_batch_set_scenes(mpm_batch* batch, mpm_ctx* const* scenes, int32_t n);
+int32_t mpm_batch_scene_count(const mpm_batch* batch);
+/* one frame of every scene i of length frame_dt[i] (<= 0: scene i does not advance), each with its own
+ * CFL substep count and escaped-particle removal, exactly as mpm_advance; out[i] as mpm_advance */
+int32_t mpm_batch_advance(mpm_batch* batch, const double* frame_dt, mpm_advance_stats* out);
+#define MPM_BATCH_OPT_P2G_BLOCKS 0        /* workgroups of the persistent P2G launch (0: automatic) */
+#define MPM_BATCH_OPT_GRAPH 1             /* 1: replay captured graphs (default); 0: launch directly */
+#define MPM_BATCH_OPT_SORT_INTERVAL 2     /* substeps between re-sorts for every scene (0: each scene's own) */
+#define MPM_BATCH_OPT_SUBSTEPS_LAUNCHED 3 /* read only: substep launches since creation */
+#define MPM_BATCH_OPT_G2P_BLOCKS 4        /* workgroups of the persistent G2P launch (0: automatic) */
+#define MPM_BATCH_OPT_GRID_BLOCKS 5       /* workgroups of the persistent grid-update launch (0: automatic) */
+int32_t mpm_batch_set_option(mpm_batch* batch, int32_t option, int64_t value);
+int64_t mpm_batch_get_option(const mpm_batch* batch, int32_t option);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,8 @@ OK, ERR_VALUE, ERR_NOT_IMPLEMENTED, ERR_MEMORY, ERR_RUNTIME = 0, -1, -2, -3, -4
 K_P2G, K_GRID, K_G2P, K_BIN, K_COLLIDERS, K_OTHER = range(6)
 OPT_P2G_MODE, OPT_SORT_INTERVAL, OPT_P2G_BLOCKS = range(3)
 P2G_BINNED, P2G_ATOMIC = 0, 1
+(BATCH_OPT_P2G_BLOCKS, BATCH_OPT_GRAPH, BATCH_OPT_SORT_INTERVAL, BATCH_OPT_SUBSTEPS_LAUNCHED, BATCH_OPT_G2P_BLOCKS,
+ BATCH_OPT_GRID_BLOCKS) = range(6)
 KERNEL_NAMES = ("p2g", "grid", "g2p", "bin", "colliders", "other")
 
 
@@ -82,6 +84,13 @@ def _declare(L: C.CDLL) -> None:
         "mpm_get_kernel_time": ([ctx, i32, C.POINTER(C.c_double), i64p], i32),
         "mpm_reset_timing": ([ctx], i32),
         "mpm_synchronize": ([ctx], i32),
+        "mpm_batch_create": ([i32, C.POINTER(ctx)], i32),
+        "mpm_batch_destroy": ([ctx], i32),
+        "mpm_batch_set_scenes": ([ctx, C.POINTER(ctx), i32], i32),
+        "mpm_batch_scene_count": ([ctx], i32),
+        "mpm_batch_advance": ([ctx, C.POINTER(f64), C.POINTER(AdvanceStats)], i32),
+        "mpm_batch_set_option": ([ctx, i32, i64], i32),
+        "mpm_batch_get_option": ([ctx, i32], i64),
     }
     for name, (args, res) in table.items():
         fn = getattr(L, name)

@@ -22,8 +22,7 @@
 #include <vector>
 
 #include "mpm_hip.h"
-#include "binning.hpp"
-#include "substep.hpp"
+#include "batch.hpp"
 
 using namespace mpm;
 
@@ -130,7 +129,10 @@ struct ColliderHost {
 
 struct mpm_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every operation of the context runs on
+  hipStream_t own_stream = nullptr;  // the context's own stream (stream is the batch's while in a batch)
+  mpm_batch* batch = nullptr;
+  uint64_t version = 0;              // bumped by every change of the particle state
   mpm_domain_desc dom{};
   std::vector<mpm_material_desc> mats;
   MatDev* d_mats = nullptr;
@@ -178,6 +180,7 @@ struct mpm_ctx {
   Scratch scratch;
   Timing timing;
   double frame_dt = 1.0;
+  bool capturing = false;  // inside a graph capture: no events, no allocations, no host waits
 
   Particles parts() const { return A.view(); }
   ColliderSet collider_set() const {
@@ -192,6 +195,47 @@ struct mpm_ctx {
     cs.band = (float)dom.dx_f64;
     return cs;
   }
+};
+
+// A set of contexts advanced together (batch.hpp). While a context belongs to a batch, its operations
+// run on the batch's stream, so host-side calls on it (poses, state reads) are ordered with the batch's
+// frames without further synchronisation.
+struct mpm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<mpm_ctx*> scenes;
+  // per scene, host side: the speed maximum that the next frame's CFL step uses and the context version
+  // it belongs to
+  std::vector<float> speed;
+  std::vector<uint64_t> version;
+  std::vector<char> speed_ok;
+
+  // device tables and their pinned host mirrors
+  int64_t table_cap = 0;
+  SceneDev* d_scenes = nullptr;
+  SceneStep* d_steps = nullptr;
+  ColliderFrame* d_frames = nullptr;
+  uint32_t* d_pre = nullptr;  // work-unit prefixes (kPrefixKinds x (scenes + 1))
+  float* d_speed = nullptr;
+  SceneDev* h_scenes = nullptr;
+  SceneStep* h_steps = nullptr;
+  float* h_speed = nullptr;
+
+  // sizes the captured graphs depend on: the radix histogram (256 x all radix tiles at full capacity)
+  // and the largest radix pass count
+  int64_t hist_tiles = -1;
+  int max_passes = 1;
+  Scratch hist, scan_sums;
+
+  hipGraphExec_t g_sort = nullptr, g_plain = nullptr;
+
+  // options: persistent grid sizes (0: automatic), graphs, sort cadence
+  int64_t p2g_blocks = 0;
+  int64_t g2p_blocks = 0;
+  int64_t grid_blocks = 0;
+  int64_t use_graph = 1;
+  int64_t sort_interval = 0;   // 0: each context's own
+  int64_t substeps_launched = 0;
 };
 
 namespace {
@@ -222,7 +266,7 @@ struct Timed {
   int kernel;
   hipEvent_t a = nullptr;
   Timed(mpm_ctx* ctx, int k) : c(ctx), kernel(k) {
-    if (c->timing.on) {
+    if (c->timing.on && !c->capturing) {
       a = pool_event(c->timing);
       HIP_CHECK(hipEventRecord(a, c->stream));
     }
@@ -340,21 +384,22 @@ void check_materials(mpm_ctx* c, const int32_t* mat, int64_t n) {
 }
 
 void state_changed(mpm_ctx* c) {
+  c->version += 1;
   c->sorted = false;
   c->vmax_valid = false;
 }
 
 // ---- device-wide exclusive scan --------------------------------------------------------------
-void scan_u32(mpm_ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
+void scan_u32(Scratch& scan_sums, hipStream_t stream, const uint32_t* in, uint32_t* out, int64_t n) {
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   if (tiles > kScanTile) fail(MPM_ERR_VALUE, "scan too large");
-  uint32_t* sums = static_cast<uint32_t*>(c->scan_sums.get((tiles + 1) * sizeof(uint32_t)));
-  k_scan_tiles<<<(unsigned)tiles, kScanThreads, 0, c->stream>>>(in, (int)n, out, sums);
+  uint32_t* sums = static_cast<uint32_t*>(scan_sums.get((tiles + 1) * sizeof(uint32_t)));
+  k_scan_tiles<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, (int)n, out, sums);
   check_launch("k_scan_tiles");
   if (tiles > 1) {
-    k_scan_top<<<1, kScanThreads, 0, c->stream>>>(sums, (int)tiles);
+    k_scan_top<<<1, kScanThreads, 0, stream>>>(sums, (int)tiles);
     check_launch("k_scan_top");
-    k_scan_add<<<(unsigned)tiles, kScanThreads, 0, c->stream>>>(out, (int)n, sums);
+    k_scan_add<<<(unsigned)tiles, kScanThreads, 0, stream>>>(out, (int)n, sums);
     check_launch("k_scan_add");
   }
 }
@@ -363,7 +408,8 @@ void scan_u32(mpm_ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
 void sort_particles(mpm_ctx* c) {
   const int64_t n = c->n;
   Timed t(c, MPM_K_BIN);
-  HIP_CHECK(hipMemsetAsync(c->n_active, 0, sizeof(uint32_t), c->stream));
+  k_zero_u32<<<1, 64, 0, c->stream>>>(c->n_active, 1);
+  check_launch("k_zero_u32");
   if (n == 0) {
     k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(c->keys1, 0, c->n_cells, c->offsets);
     check_launch("k_cell_bounds");
@@ -383,7 +429,7 @@ void sort_particles(mpm_ctx* c) {
     uint32_t* vout = (pass % 2 == 0) ? c->vals1 : c->vals2;
     k_radix_hist<<<tiles, kSortThreads, 0, c->stream>>>(kin, (int)n, 8 * pass, tiles, hist);
     check_launch("k_radix_hist");
-    scan_u32(c, hist, hist, (int64_t)256 * tiles);
+    scan_u32(c->scan_sums, c->stream, hist, hist, (int64_t)256 * tiles);
     k_radix_scatter<<<tiles, kSortThreads, 0, c->stream>>>(kin, vin, (int)n, 8 * pass, tiles, hist, kout, vout);
     check_launch("k_radix_scatter");
     kin = kout;
@@ -411,7 +457,7 @@ void run_p2g(mpm_ctx* c, float dt) {
     check_launch("k_p2g_atomic");
     return;
   }
-  if (!c->sorted || c->steps_since_sort >= c->sort_interval) sort_particles(c);
+  if ((!c->sorted || c->steps_since_sort >= c->sort_interval)) sort_particles(c);
   if (c->n == 0) return;
   Timed t(c, MPM_K_P2G);
   // four waves (four bricks at a time) per workgroup, grid-stride over the device-side brick list
@@ -452,21 +498,27 @@ void substep(mpm_ctx* c, double dt, double s, double frame_dt) {
   run_grid(c, dtf, sf, inv_fdt);
   run_g2p(c, dtf, sf, inv_fdt);
   c->steps_since_sort += 1;
+  c->version += 1;
 }
 
 // max |v| over the particles (kernels.py:266-269): the last G2P left per-workgroup maxima; after any
 // state change a speed pass recomputes them
+// the speed reduction without a host wait: writes max |v| to `out` (device memory) on the context's stream
+void reduce_speed_async(mpm_ctx* c, float* out) {
+  if (!c->vmax_valid) {
+    k_speed_partials<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), (int)c->n, block_max(c));
+    check_launch("k_speed_partials");
+    c->block_max_count = grid_for(c->n);
+    c->vmax_valid = true;
+  }
+  k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->d_block_max, (int)c->block_max_count, out);
+  check_launch("k_reduce_max");
+}
+
 float max_speed(mpm_ctx* c) {
   {
     Timed t(c, MPM_K_OTHER);
-    if (!c->vmax_valid) {
-      k_speed_partials<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), (int)c->n, block_max(c));
-      check_launch("k_speed_partials");
-      c->block_max_count = grid_for(c->n);
-      c->vmax_valid = true;
-    }
-    k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->d_block_max, (int)c->block_max_count, c->d_vmax);
-    check_launch("k_reduce_max");
+    reduce_speed_async(c, c->d_vmax);
   }
   float f = 0.0f;
   HIP_CHECK(hipMemcpyAsync(&f, c->d_vmax, sizeof f, hipMemcpyDeviceToHost, c->stream));
@@ -729,10 +781,351 @@ void mesh_tables(const float* verts, int nv, const int32_t* tris, int nt, Collid
   }
 }
 
-template <class Fn> int32_t run(mpm_ctx* c, Fn&& fn) {
+// ---- batches -----------------------------------------------------------------------------------
+void batch_drop_graphs(mpm_batch* b) {
+  if (b->g_sort) (void)hipGraphExecDestroy(b->g_sort);
+  if (b->g_plain) (void)hipGraphExecDestroy(b->g_plain);
+  b->g_sort = b->g_plain = nullptr;
+}
+
+void batch_free_tables(mpm_batch* b) {
+  for (void* p : {(void*)b->d_scenes, (void*)b->d_steps, (void*)b->d_frames, (void*)b->d_pre, (void*)b->d_speed})
+    dev_free(p);
+  for (void* p : {(void*)b->h_scenes, (void*)b->h_steps, (void*)b->h_speed})
+    if (p) (void)hipHostFree(p);
+  b->d_scenes = nullptr;
+  b->d_steps = nullptr;
+  b->d_frames = nullptr;
+  b->d_pre = nullptr;
+  b->d_speed = nullptr;
+  b->h_scenes = nullptr;
+  b->h_steps = nullptr;
+  b->h_speed = nullptr;
+  b->table_cap = 0;
+}
+
+template <class T> T* host_alloc(size_t count) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    fail(MPM_ERR_MEMORY, "pinned host allocation failed");
+  }
+  return static_cast<T*>(p);
+}
+
+void batch_reserve_tables(mpm_batch* b, int64_t n) {
+  if (n <= b->table_cap) return;
+  batch_drop_graphs(b);
+  batch_free_tables(b);
+  const int64_t cap = std::max<int64_t>(n, 16);
+  b->d_scenes = dev_alloc<SceneDev>(cap);
+  b->d_steps = dev_alloc<SceneStep>(cap);
+  b->d_frames = dev_alloc<ColliderFrame>(cap * kMaxColliders);
+  b->d_pre = dev_alloc<uint32_t>((size_t)kPrefixKinds * (cap + 1));
+  b->d_speed = dev_alloc<float>(cap);
+  b->h_scenes = host_alloc<SceneDev>(cap);
+  b->h_steps = host_alloc<SceneStep>(cap);
+  b->h_speed = host_alloc<float>(cap);
+  b->table_cap = cap;
+}
+
+void detach_scene(mpm_ctx* c) {
+  if (!c->batch) return;
+  (void)hipStreamSynchronize(c->stream);
+  c->stream = c->own_stream;
+  c->batch = nullptr;
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+int radix_passes(const mpm_ctx* c) { return (c->key_bits + 7) / 8; }
+
+// Buffers the captured graphs reference, sized for every scene at full capacity: the radix histogram and
+// its scan. The graphs are recaptured when these grow (or the scene set or an option changes).
+void batch_layout(mpm_batch* b) {
+  int64_t tiles = 0;
+  int passes = 1;
+  for (mpm_ctx* c : b->scenes) {
+    tiles += ceil_div(c->cap, kSortTile);
+    passes = std::max(passes, radix_passes(c));
+  }
+  tiles = std::max<int64_t>(tiles, 1);
+  if (256 * tiles > (int64_t)kScanTile * kScanTile || 256 * tiles > (int64_t)UINT32_MAX)
+    fail(MPM_ERR_VALUE, "batch too large for the radix histogram scan");
+  if (tiles <= b->hist_tiles && passes == b->max_passes) return;
+  batch_drop_graphs(b);
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  // sized before any capture: nothing allocates inside one
+  b->hist.get(256 * tiles * sizeof(uint32_t));
+  b->scan_sums.get((ceil_div(256 * tiles, kScanTile) + 1) * sizeof(uint32_t));
+  b->hist_tiles = tiles;
+  b->max_passes = passes;
+}
+
+unsigned batch_blocks(int64_t option, int64_t automatic) { return (unsigned)(option > 0 ? option : automatic); }
+
+// one substep of every running scene, as kernel launches on the batch stream (captured or direct)
+void batch_enqueue(mpm_batch* b, bool sorting) {
+  const int n = (int)b->scenes.size();
+  hipStream_t s = b->stream;
+  // persistent grids: a few workgroups per CU, striding over the running scenes' work
+  const unsigned per_particle = batch_blocks(b->g2p_blocks, 2048);
+  k_batch_plan<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, b->d_frames, n, sorting ? 1 : 0, b->d_pre);
+  check_launch("k_batch_plan");
+  if (sorting) {
+    k_batch_cell_keys<<<2048, kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+    check_launch("k_batch_cell_keys");
+    uint32_t* hist = static_cast<uint32_t*>(b->hist.ptr);
+    for (int pass = 0; pass < b->max_passes; ++pass) {
+      k_batch_radix_hist<<<1024, kSortThreads, 0, s>>>(b->d_scenes, b->d_pre, n, pass, hist);
+      check_launch("k_batch_radix_hist");
+      scan_u32(b->scan_sums, s, hist, hist, 256 * b->hist_tiles);
+      k_batch_radix_scatter<<<1024, kSortThreads, 0, s>>>(b->d_scenes, b->d_pre, n, pass, hist);
+      check_launch("k_batch_radix_scatter");
+    }
+    k_batch_cell_bounds<<<4096, 256, 0, s>>>(b->d_scenes, b->d_pre, n);
+    check_launch("k_batch_cell_bounds");
+    k_batch_reorder<<<2048, kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+    check_launch("k_batch_reorder");
+  }
+  k_batch_prefix<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, n, b->d_pre);
+  check_launch("k_batch_prefix");
+  k_batch_p2g<<<batch_blocks(b->p2g_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+  check_launch("k_batch_p2g");
+  k_batch_grid<<<batch_blocks(b->grid_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n,
+                                                                  b->d_frames);
+  check_launch("k_batch_grid");
+  k_batch_g2p<<<per_particle, kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n, b->d_frames);
+  check_launch("k_batch_g2p");
+  k_batch_advance<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(b->d_steps, n);
+  check_launch("k_batch_advance");
+}
+
+hipGraphExec_t batch_capture(mpm_batch* b, bool sorting) {
+  hipGraph_t graph = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
   try {
-    if (c) {
-      DeviceGuard g(c->device);
+    batch_enqueue(b, sorting);
+  } catch (...) {
+    (void)hipStreamEndCapture(b->stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(b->stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIP_CHECK(e);
+  return exec;
+}
+
+SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
+  SceneDev d{};
+  d.set[0] = c->A.view();
+  d.set[1] = c->B.view();
+  d.tau[0] = c->A.tau();
+  d.tau[1] = c->B.tau();
+  d.mats = c->d_mats;
+  d.g = c->geom;
+  d.sc = c->sc;
+  d.cs = c->collider_set();
+  d.accum = c->accum;
+  d.vel = c->vel;
+  d.dirty = c->dirty;
+  d.live = c->live;
+  d.keys0 = c->keys0;
+  d.keys1 = c->keys1;
+  d.keys2 = c->keys2;
+  d.vals1 = c->vals1;
+  d.vals2 = c->vals2;
+  d.offsets = c->offsets;
+  d.active = c->active;
+  d.n_active = c->n_active;
+  d.n = (int32_t)c->n;
+  d.n_bricks = c->n_bricks;
+  d.n_cells = c->n_cells;
+  d.passes = radix_passes(c);
+  d.sort_interval = (int32_t)std::min<int64_t>(b->sort_interval > 0 ? b->sort_interval : c->sort_interval, 1 << 30);
+  return d;
+}
+
+// one frame of every scene (solver.py:213-220 per scene): CFL step counts from each scene's speed
+// maximum, then substep launches until every scene has run its count
+void batch_frame(mpm_batch* b, const double* frame_dt, mpm_advance_stats* out) {
+  const int64_t n = (int64_t)b->scenes.size();
+  if (n == 0) return;
+  batch_reserve_tables(b, n);
+  for (mpm_ctx* c : b->scenes)
+    if (c->p2g_mode != 0) fail(MPM_ERR_VALUE, "batched scenes use the binned P2G (option P2G_MODE 0)");
+  // speed maxima of scenes whose state changed since the batch last saw them
+  bool any = false;
+  std::vector<char> fresh(n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    mpm_ctx* c = b->scenes[i];
+    if (c->n == 0 || !(frame_dt[i] > 0.0)) continue;
+    if (b->speed_ok[i] && b->version[i] == c->version) continue;
+    reduce_speed_async(c, b->d_speed + i);
+    fresh[i] = 1;
+    any = true;
+  }
+  if (any) {
+    HIP_CHECK(hipMemcpyAsync(b->h_speed, b->d_speed, n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    for (int64_t i = 0; i < n; ++i)
+      if (fresh[i]) {
+        b->speed[i] = b->h_speed[i];
+        b->speed_ok[i] = 1;
+        b->version[i] = b->scenes[i]->version;
+      }
+  }
+  batch_layout(b);
+  // step counts (the arithmetic of mpm_advance) and the host mirror of the sort cadence
+  std::vector<int64_t> steps(n, 0);
+  std::vector<double> dts(n, 0.0);
+  std::vector<int32_t> since(n), sorted(n), interval(n);
+  int64_t max_steps = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    mpm_ctx* c = b->scenes[i];
+    const double fdt = frame_dt[i];
+    if (c->n > 0 && fdt > 0.0) {
+      double vmax = (double)b->speed[i];
+      if (!std::isfinite(vmax)) vmax = 0.0;
+      double wave = 0.0;
+      for (size_t m = 0; m < c->mats.size(); ++m)
+        if (c->present[m]) wave = std::max(wave, c->mats[m].wave_speed);
+      const double cfl_dt = std::min(0.4 * c->dom.dx_f64 / (wave + vmax), fdt);
+      steps[i] = (int64_t)std::ceil(fdt / cfl_dt - 1e-9);
+      dts[i] = fdt / (double)steps[i];
+      if (steps[i] > (int64_t)1 << 30) fail(MPM_ERR_VALUE, "too many substeps in one frame");
+      c->frame_dt = fdt;
+    }
+    max_steps = std::max(max_steps, steps[i]);
+    const SceneDev rec = scene_record(b, c);
+    b->h_scenes[i] = rec;
+    SceneStep& st = b->h_steps[i];
+    st = SceneStep{};
+    st.dt = (float)dts[i];
+    st.inv_frame_dt = fdt > 0.0 ? (float)(1.0 / fdt) : 0.0f;
+    st.k = 0;
+    st.steps = (int32_t)steps[i];
+    st.since_sort = (int32_t)std::min<int64_t>(c->steps_since_sort, 1 << 30);
+    st.sorted = c->sorted ? 1 : 0;
+    st.cur = 0;
+    since[i] = st.since_sort;
+    sorted[i] = st.sorted;
+    interval[i] = rec.sort_interval;
+  }
+  HIP_CHECK(hipMemcpyAsync(b->d_scenes, b->h_scenes, n * sizeof(SceneDev), hipMemcpyHostToDevice, b->stream));
+  HIP_CHECK(hipMemcpyAsync(b->d_steps, b->h_steps, n * sizeof(SceneStep), hipMemcpyHostToDevice, b->stream));
+  if (max_steps > 0 && b->use_graph && !b->g_sort) {
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->g_sort = batch_capture(b, true);
+    b->g_plain = batch_capture(b, false);
+  }
+  for (int64_t k = 0; k < max_steps; ++k) {
+    bool sorting = false;
+    for (int64_t i = 0; i < n; ++i) {
+      if (k >= steps[i]) continue;
+      if (!sorted[i] || since[i] >= interval[i]) {
+        sorting = true;
+        sorted[i] = 1;
+        since[i] = 0;
+      }
+      since[i] += 1;
+    }
+    if (b->use_graph) HIP_CHECK(hipGraphLaunch(sorting ? b->g_sort : b->g_plain, b->stream));
+    else batch_enqueue(b, sorting);
+  }
+  b->substeps_launched += max_steps;
+  HIP_CHECK(hipMemcpyAsync(b->h_steps, b->d_steps, n * sizeof(SceneStep), hipMemcpyDeviceToHost, b->stream));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  for (int64_t i = 0; i < n; ++i) {
+    mpm_ctx* c = b->scenes[i];
+    const SceneStep& st = b->h_steps[i];
+    if (st.k != (int32_t)steps[i] || st.sorted != sorted[i] || st.since_sort != since[i])
+      fail(MPM_ERR_RUNTIME, "batch bookkeeping diverged from the device");
+    if (steps[i] == 0) {
+      out[i] = mpm_advance_stats{0, frame_dt[i], 0};
+      continue;
+    }
+    if (st.cur) std::swap(c->A, c->B);
+    c->sorted = st.sorted != 0;
+    c->steps_since_sort = st.since_sort;
+    c->last_perm = const_cast<uint32_t*>((radix_passes(c) - 1) % 2 == 0 ? c->vals1 : c->vals2);
+    c->vmax_valid = false;  // the batch keeps the speed maximum, the context's partial maxima are stale
+    c->version += 1;
+    uint32_t bits = st.speed_bits;
+    float v;
+    std::memcpy(&v, &bits, 4);
+    b->speed[i] = v;
+    b->speed_ok[i] = 1;
+    int64_t removed = remove_escaped(c);
+    if (removed) refresh_present(c);
+    b->version[i] = c->version;
+    out[i] = mpm_advance_stats{steps[i], dts[i], removed};
+  }
+}
+
+void batch_set_scenes(mpm_batch* b, mpm_ctx* const* scenes, int32_t n) {
+  for (int32_t i = 0; i < n; ++i) {
+    mpm_ctx* c = scenes[i];
+    if (!c) fail(MPM_ERR_VALUE, "null scene");
+    if (c->device != b->device) fail(MPM_ERR_VALUE, "scene lives on another device than the batch");
+    if (c->batch && c->batch != b) fail(MPM_ERR_VALUE, "scene already belongs to another batch");
+    for (int32_t j = 0; j < i; ++j)
+      if (scenes[j] == c) fail(MPM_ERR_VALUE, "scene listed twice");
+  }
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  std::vector<float> speed(n, 0.0f);
+  std::vector<uint64_t> version(n, 0);
+  std::vector<char> ok(n, 0);
+  for (int32_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < b->scenes.size(); ++j)
+      if (b->scenes[j] == scenes[i]) {
+        speed[i] = b->speed[j];
+        version[i] = b->version[j];
+        ok[i] = b->speed_ok[j];
+      }
+  for (mpm_ctx* c : b->scenes) {
+    bool stays = false;
+    for (int32_t i = 0; i < n; ++i) stays = stays || scenes[i] == c;
+    if (!stays) detach_scene(c);
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    mpm_ctx* c = scenes[i];
+    if (c->batch == b) continue;
+    HIP_CHECK(hipStreamSynchronize(c->own_stream));
+    c->stream = b->stream;
+    c->batch = b;
+  }
+  b->scenes.assign(scenes, scenes + n);
+  b->speed = std::move(speed);
+  b->version = std::move(version);
+  b->speed_ok = std::move(ok);
+  batch_drop_graphs(b);
+  b->hist_tiles = -1;
+}
+
+// drops one scene from its batch (never throws: used by mpm_destroy)
+void batch_remove_scene(mpm_batch* b, mpm_ctx* c) {
+  (void)hipStreamSynchronize(b->stream);
+  for (size_t j = 0; j < b->scenes.size(); ++j)
+    if (b->scenes[j] == c) {
+      b->scenes.erase(b->scenes.begin() + j);
+      b->speed.erase(b->speed.begin() + j);
+      b->version.erase(b->version.begin() + j);
+      b->speed_ok.erase(b->speed_ok.begin() + j);
+      break;
+    }
+  batch_drop_graphs(b);
+  b->hist_tiles = -1;
+  detach_scene(c);
+}
+
+template <class Fn> int32_t run_on(int device, Fn&& fn) {
+  try {
+    if (device >= 0) {
+      DeviceGuard g(device);
       fn();
     } else {
       fn();
@@ -748,6 +1141,8 @@ template <class Fn> int32_t run(mpm_ctx* c, Fn&& fn) {
     return MPM_ERR_RUNTIME;
   }
 }
+
+template <class Fn> int32_t run(mpm_ctx* c, Fn&& fn) { return run_on(c ? c->device : -1, std::forward<Fn>(fn)); }
 
 }  // namespace
 
@@ -779,7 +1174,8 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
         fail(MPM_ERR_NOT_IMPLEMENTED, "material model is not supported by the HIP backend");
     for (int a = 0; a < 3; ++a)
       if (domain->dims[a] < 3) fail(MPM_ERR_VALUE, "grid dimensions must be at least 3 nodes");
-    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
     c->dom = *domain;
     c->mats.assign(materials, materials + n_materials);
     c->present.assign(std::max(1, n_materials), 0);
@@ -850,6 +1246,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   int prev = 0;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(c->device);  // best effort: destroy never throws
+  if (c->batch) batch_remove_scene(c->batch, c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
@@ -864,7 +1261,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->timing.pool) (void)hipEventDestroy(e);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   (void)hipSetDevice(prev);
   delete c;
   return MPM_OK;
@@ -1199,6 +1596,107 @@ int32_t mpm_reset_timing(mpm_ctx* c) {
 int32_t mpm_synchronize(mpm_ctx* c) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] { HIP_CHECK(hipStreamSynchronize(c->stream)); });
+}
+
+// ---- batches ------------------------------------------------------------------------------------
+int32_t mpm_batch_create(int32_t device, mpm_batch** out) {
+  if (!out) return MPM_ERR_VALUE;
+  *out = nullptr;
+  mpm_batch* b = new (std::nothrow) mpm_batch();
+  if (!b) {
+    g_error = "host out of memory";
+    return MPM_ERR_MEMORY;
+  }
+  b->device = device;
+  const int32_t rc = run_on(-1, [&] {
+    int count = 0;
+    HIP_CHECK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) fail(MPM_ERR_VALUE, "device ordinal out of range");
+    DeviceGuard g(device);
+    HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  });
+  if (rc != MPM_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return MPM_OK;
+}
+
+int32_t mpm_batch_destroy(mpm_batch* b) {
+  if (!b) return MPM_OK;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (mpm_ctx* c : b->scenes) detach_scene(c);
+  batch_drop_graphs(b);
+  batch_free_tables(b);
+  b->hist.release();
+  b->scan_sums.release();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  (void)hipSetDevice(prev);
+  delete b;
+  return MPM_OK;
+}
+
+int32_t mpm_batch_set_scenes(mpm_batch* b, mpm_ctx* const* scenes, int32_t n) {
+  if (!b || n < 0 || (n > 0 && !scenes)) return MPM_ERR_VALUE;
+  return run_on(b->device, [&] { batch_set_scenes(b, scenes, n); });
+}
+
+int32_t mpm_batch_scene_count(const mpm_batch* b) { return b ? (int32_t)b->scenes.size() : 0; }
+
+int32_t mpm_batch_advance(mpm_batch* b, const double* frame_dt, mpm_advance_stats* out) {
+  if (!b || (!b->scenes.empty() && (!frame_dt || !out))) return MPM_ERR_VALUE;
+  return run_on(b->device, [&] { batch_frame(b, frame_dt, out); });
+}
+
+int32_t mpm_batch_set_option(mpm_batch* b, int32_t option, int64_t value) {
+  if (!b) return MPM_ERR_VALUE;
+  return run_on(b->device, [&] {
+    switch (option) {
+      case MPM_BATCH_OPT_P2G_BLOCKS:
+      case MPM_BATCH_OPT_G2P_BLOCKS:
+      case MPM_BATCH_OPT_GRID_BLOCKS:
+        if (value < 0 || value > (1 << 20)) fail(MPM_ERR_VALUE, "workgroup counts must be in [0, 2^20]");
+        (option == MPM_BATCH_OPT_P2G_BLOCKS ? b->p2g_blocks
+                                            : option == MPM_BATCH_OPT_G2P_BLOCKS ? b->g2p_blocks : b->grid_blocks) = value;
+        break;
+      case MPM_BATCH_OPT_GRAPH:
+        if (value != 0 && value != 1) fail(MPM_ERR_VALUE, "graph option is 0 or 1");
+        b->use_graph = value;
+        break;
+      case MPM_BATCH_OPT_SORT_INTERVAL:
+        if (value < 0) fail(MPM_ERR_VALUE, "sort interval must be >= 0");
+        b->sort_interval = value;
+        break;
+      default:
+        fail(MPM_ERR_VALUE, "unknown batch option");
+    }
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    batch_drop_graphs(b);
+  });
+}
+
+int64_t mpm_batch_get_option(const mpm_batch* b, int32_t option) {
+  if (!b) return -1;
+  switch (option) {
+    case MPM_BATCH_OPT_P2G_BLOCKS:
+      return batch_blocks(b->p2g_blocks, 2048);
+    case MPM_BATCH_OPT_G2P_BLOCKS:
+      return batch_blocks(b->g2p_blocks, 2048);
+    case MPM_BATCH_OPT_GRID_BLOCKS:
+      return batch_blocks(b->grid_blocks, 2048);
+    case MPM_BATCH_OPT_GRAPH:
+      return b->use_graph;
+    case MPM_BATCH_OPT_SORT_INTERVAL:
+      return b->sort_interval;
+    case MPM_BATCH_OPT_SUBSTEPS_LAUNCHED:
+      return b->substeps_launched;
+    default:
+      return -1;
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,466 @@
+// Batched substep: many independent scenes advance through one launch sequence.
+//
+// Every scene keeps its own particle sets, grid, bins and colliders (those of its context). A table of
+// SceneDev records in device memory describes them, and a table of SceneStep records holds what changes
+// from substep to substep (substep index, dt, the live particle set, sort bookkeeping, the speed maximum).
+// The kernels read both tables, so one captured graph per substep kind serves every substep of every
+// frame, whatever each scene's CFL step count: a scene that has run its `steps` substeps of the frame
+// takes no further part until the next frame.
+//
+// Work is flattened across scenes on the device. At the start of a substep k_batch_plan computes, over
+// the scenes that run (or sort) in it, prefix sums of their work units: 256-slot particle tiles, radix
+// tiles, 256-cell blocks, groups of 64 bricks. Every kernel then runs a fixed, persistent grid that
+// strides over the concatenated units of the running scenes only and finds each unit's scene in the
+// prefix (searching forward from the previous one). A substep in which only a few scenes still run so
+// costs only their work, and no launch size depends on the scenes' state, which is what lets the
+// sequence be captured once. P2G strides the same way over the concatenation of the running scenes'
+// occupied-brick lists (prefix computed after the sort), so large and small scenes share the machine at
+// brick granularity.
+//
+// One substep (semantics of solver.py:176-211 per scene):
+//   k_batch_plan            per scene: runs?, frame fraction s, sort now?, work prefixes, collider frames
+//   [k_batch_cell_keys, radix passes, k_batch_cell_bounds, k_batch_reorder]   only in the sorting graph
+//   k_batch_prefix          switches scenes that sorted to their reordered set, occupied-brick prefix
+//   k_batch_p2g, k_batch_grid, k_batch_g2p, k_batch_advance
+#pragma once
+
+#include "binning.hpp"
+#include "substep.hpp"
+
+namespace mpm {
+
+constexpr int kTile = 256;        // particle slots per unit of a per-particle kernel
+constexpr int kGroupBricks = 64;  // bricks whose dirty flags one wave of the grid update examines
+constexpr int kPlanThreads = 1024;
+
+struct SceneDev {
+  Particles set[2];  // set[cur] is live, the other is the reorder target
+  float* tau[2];
+  const MatDev* mats;
+  GridGeom g;
+  StepConsts sc;
+  ColliderSet cs;
+  float4* accum;
+  float4* vel;
+  uint32_t* dirty;
+  uint8_t* live;
+  uint32_t *keys0, *keys1, *keys2, *vals1, *vals2;
+  uint32_t* offsets;   // per cell
+  uint32_t* active;    // occupied bricks
+  uint32_t* n_active;  // their count
+  int32_t n;           // particles
+  int32_t n_bricks, n_cells, passes;
+  int32_t sort_interval;
+};
+
+struct SceneStep {
+  float dt, inv_frame_dt;
+  int32_t k, steps;
+  int32_t run;          // this substep advances the scene
+  int32_t sort_now;     // this substep sorts the scene's particles first
+  int32_t since_sort;   // substeps since the last sort
+  int32_t sorted;
+  int32_t cur;          // live particle set
+  uint32_t speed_bits;  // max |v| of the scene's last G2P, as float bits (NaN recorded as +inf)
+};
+
+// work-unit prefixes, each n_scenes + 1 entries, in one array: pre + q * (n_scenes + 1)
+enum PrefixKind {
+  kPreTiles = 0,       // particle tiles of running scenes (G2P)
+  kPreGroups = 1,      // brick groups of running scenes (grid update)
+  kPreSortTiles = 2,   // particle tiles of sorting scenes (keys, reorder)
+  kPreRadixTiles = 3,  // radix tiles of sorting scenes (histogram segment = 256 * this prefix)
+  kPreCellBlocks = 4,  // 256-cell blocks of sorting scenes (cell offsets)
+  kPreBricks = 5,      // occupied bricks of running scenes (P2G)
+  kPrefixKinds = 6
+};
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uniform_u(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// the scene that owns unit t of a prefix (pre[i] <= t holds for the current guess i; units are visited
+// in increasing order, so the search only moves forward)
+__device__ __forceinline__ int scene_of(const uint32_t* __restrict__ pre, int n_scenes, uint32_t t, int i) {
+  if (pre[i + 1] > t) return i;
+  int lo = i + 1, hi = n_scenes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t ceil_div_u(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// radix pass `pass` reads keys0/vals2 (pass 0), keys1/vals1 (odd) or keys2/vals2 (even) and writes the
+// other pair; the sort's result is the output of pass passes-1
+__device__ __forceinline__ const uint32_t* sorted_keys(const SceneDev& S) {
+  return ((S.passes - 1) % 2 == 0) ? S.keys1 : S.keys2;
+}
+__device__ __forceinline__ const uint32_t* sorted_vals(const SceneDev& S) {
+  return ((S.passes - 1) % 2 == 0) ? S.vals1 : S.vals2;
+}
+
+// Single workgroup. Per scene: does it run this substep (k < steps), its frame fraction, does it sort
+// (never when allow_sort == 0, the graph without the binning kernels); the work prefixes of the
+// running and sorting scenes; then the collider frames of every running scene at its fraction s.
+__global__ void __launch_bounds__(kPlanThreads) k_batch_plan(const SceneDev* __restrict__ sd,
+                                                             SceneStep* __restrict__ st,
+                                                             ColliderFrame* __restrict__ frames, int n_scenes,
+                                                             int allow_sort, uint32_t* __restrict__ pre) {
+  __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
+  const int stride = n_scenes + 1;
+  uint32_t carry[5] = {0, 0, 0, 0, 0};
+  for (int base = 0; base < n_scenes; base += kPlanThreads) {
+    const int i = base + threadIdx.x;
+    uint32_t count[5] = {0, 0, 0, 0, 0};
+    if (i < n_scenes) {
+      const SceneDev& S = sd[i];
+      SceneStep& T = st[i];
+      const bool run = T.k < T.steps;
+      const bool sort_now = allow_sort && run && (T.sorted == 0 || T.since_sort >= S.sort_interval);
+      T.run = run ? 1 : 0;
+      T.sort_now = sort_now ? 1 : 0;
+      if (run) {
+        T.speed_bits = 0u;
+        count[kPreTiles] = ceil_div_u((uint32_t)S.n, kTile);
+        count[kPreGroups] = ceil_div_u((uint32_t)S.n_bricks, kGroupBricks);
+      }
+      if (sort_now) {
+        *S.n_active = 0u;
+        count[kPreSortTiles] = ceil_div_u((uint32_t)S.n, kTile);
+        count[kPreRadixTiles] = ceil_div_u((uint32_t)S.n, kSortTile);
+        count[kPreCellBlocks] = ceil_div_u((uint32_t)S.n_cells + 1u, 256u);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      uint32_t total;
+      const uint32_t ex = block_exclusive_scan(count[q], s_warp, total);
+      if (i < n_scenes) pre[q * stride + i] = carry[q] + ex;
+      carry[q] += total;
+    }
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) pre[q * stride + n_scenes] = carry[q];
+  }
+  for (int t = threadIdx.x; t < n_scenes * kMaxColliders; t += kPlanThreads) {
+    const int i = t / kMaxColliders, j = t % kMaxColliders;
+    const int32_t k = st[i].k, steps = st[i].steps;
+    if (k < steps && j < sd[i].cs.count) {
+      // substep k of `steps` ends at frame fraction (k + 1) / steps (solver.py:217-218)
+      const float s = (float)((double)(k + 1) / (double)steps);
+      collider_frame(sd[i].cs.cols[j], s, st[i].inv_frame_dt, frames[t]);
+    }
+  }
+}
+
+// ---- binning of the scenes that sort this substep (binning.hpp, per scene) ---------------------
+__global__ void __launch_bounds__(kTile) k_batch_cell_keys(const SceneDev* __restrict__ sd,
+                                                           const SceneStep* __restrict__ st,
+                                                           const uint32_t* __restrict__ pre, int n_scenes) {
+  const uint32_t* P0 = pre + kPreSortTiles * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  int i = 0;
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    i = uniform(scene_of(P0, n_scenes, u, i));
+    const SceneDev& S = sd[i];
+    const int p = (int)(u - P0[i]) * kTile + threadIdx.x;
+    if (p >= S.n) continue;
+    const Particles P = S.set[st[i].cur];
+    const float x[3] = {P.x[p], P.x[P.cap + p], P.x[2 * P.cap + p]};
+    int base[3];
+    float fx[3];
+    stencil_base(S.g, x, base, fx);
+    S.keys0[p] = cell_key(S.g, base);
+    S.vals2[p] = (uint32_t)p;
+  }
+}
+
+__device__ __forceinline__ void radix_io(const SceneDev& S, int pass, const uint32_t*& kin, const uint32_t*& vin,
+                                         uint32_t*& kout, uint32_t*& vout) {
+  kin = pass == 0 ? S.keys0 : (pass % 2 ? S.keys1 : S.keys2);
+  vin = pass == 0 ? S.vals2 : (pass % 2 ? S.vals1 : S.vals2);
+  kout = pass % 2 == 0 ? S.keys1 : S.keys2;
+  vout = pass % 2 == 0 ? S.vals1 : S.vals2;
+}
+
+// digit histogram of one radix tile; a sorting scene's histogram is the segment that starts at
+// 256 * (its radix-tile prefix), digit-major
+__global__ void __launch_bounds__(kSortThreads) k_batch_radix_hist(const SceneDev* __restrict__ sd,
+                                                                   const uint32_t* __restrict__ pre, int n_scenes,
+                                                                   int pass, uint32_t* __restrict__ hist) {
+  const uint32_t* P0 = pre + kPreRadixTiles * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  __shared__ uint32_t h[256];
+  int i = 0;
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    i = uniform(scene_of(P0, n_scenes, u, i));
+    const SceneDev& S = sd[i];
+    if (pass >= S.passes) continue;
+    const int tiles = (S.n + kSortTile - 1) / kSortTile;
+    const int t = (int)(u - P0[i]);
+    const uint32_t *kin, *vin;
+    uint32_t *kout, *vout;
+    radix_io(S, pass, kin, vin, kout, vout);
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int shift = 8 * pass;
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+      const int p = t * kSortTile + j * kSortThreads + threadIdx.x;
+      if (p < S.n) atomicAdd(&h[(kin[p] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[256u * P0[i] + threadIdx.x * tiles + t] = h[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// stable scatter of one radix tile. `offsets` is the exclusive scan of the whole histogram buffer, so the
+// scene's own exclusive scan is offsets[e] - offsets[first entry of its segment] (modulo 2^32, exact).
+__global__ void __launch_bounds__(kSortThreads) k_batch_radix_scatter(const SceneDev* __restrict__ sd,
+                                                                      const uint32_t* __restrict__ pre, int n_scenes,
+                                                                      int pass, const uint32_t* __restrict__ offsets) {
+  constexpr int kWaves = kSortThreads / 64;
+  const uint32_t* P0 = pre + kPreRadixTiles * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wave_count[kWaves][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int shift = 8 * pass;
+  int i = 0;
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    i = uniform(scene_of(P0, n_scenes, u, i));
+    const SceneDev& S = sd[i];
+    if (pass >= S.passes) continue;
+    const int tiles = (S.n + kSortTile - 1) / kSortTile;
+    const int t = (int)(u - P0[i]);
+    const uint32_t *kin, *vin;
+    uint32_t *kout, *vout;
+    radix_io(S, pass, kin, vin, kout, vout);
+    const uint32_t seg = 256u * P0[i];
+    running[threadIdx.x] = offsets[seg + threadIdx.x * tiles + t] - offsets[seg];
+    for (int j = 0; j < kSortItems; ++j) {
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) wave_count[w][threadIdx.x] = 0;
+      __syncthreads();
+      const int p = t * kSortTile + j * kSortThreads + threadIdx.x;
+      const bool valid = p < S.n;
+      const uint32_t key = valid ? kin[p] : 0u;
+      const uint32_t val = valid ? vin[p] : 0u;
+      const uint32_t d = (key >> shift) & 255u;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const uint64_t m = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? m : ~m;
+      }
+      const uint32_t rank_in_wave = (uint32_t)__popcll(peers & lt_mask);
+      if (valid && rank_in_wave == 0) wave_count[wave][d] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      if (valid) {
+        uint32_t pos = running[d] + rank_in_wave;
+        for (int w = 0; w < wave; ++w) pos += wave_count[w][d];
+        kout[pos] = key;
+        vout[pos] = val;
+      }
+      __syncthreads();
+      uint32_t add = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) add += wave_count[w][threadIdx.x];
+      running[threadIdx.x] += add;
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ keys, int n, uint32_t c) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (keys[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  return (uint32_t)lo;
+}
+
+// offsets[c] for c in [0, n_cells] (k_cell_bounds); the thread of a brick's first cell also appends the
+// brick to the occupied list when its 64 cells hold a particle (k_active_bricks)
+__global__ void __launch_bounds__(256) k_batch_cell_bounds(const SceneDev* __restrict__ sd,
+                                                           const uint32_t* __restrict__ pre, int n_scenes) {
+  const uint32_t* P0 = pre + kPreCellBlocks * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  int i = 0;
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    i = uniform(scene_of(P0, n_scenes, u, i));
+    const SceneDev& S = sd[i];
+    const int c = (int)(u - P0[i]) * 256 + threadIdx.x;
+    if (c > S.n_cells) continue;
+    const uint32_t* keys = sorted_keys(S);
+    const uint32_t lo = lower_bound_u32(keys, S.n, (uint32_t)c);
+    S.offsets[c] = lo;
+    if ((c & 63) == 0 && c < S.n_cells) {
+      const uint32_t hi = lower_bound_u32(keys, S.n, (uint32_t)(c + 64));
+      if (hi > lo) S.active[atomicAdd(S.n_active, 1u)] = (uint32_t)(c >> 6);
+    }
+  }
+}
+
+// gather into the other particle set in sorted order (k_reorder)
+__global__ void __launch_bounds__(kTile) k_batch_reorder(const SceneDev* __restrict__ sd,
+                                                         const SceneStep* __restrict__ st,
+                                                         const uint32_t* __restrict__ pre, int n_scenes) {
+  const uint32_t* P0 = pre + kPreSortTiles * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  int i = 0;
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    i = uniform(scene_of(P0, n_scenes, u, i));
+    const SceneDev& S = sd[i];
+    const int p = (int)(u - P0[i]) * kTile + threadIdx.x;
+    if (p >= S.n) continue;
+    const int cur = st[i].cur;
+    const Particles src = S.set[cur], dst = S.set[1 - cur];
+    const float* tau_src = S.tau[cur];
+    float* tau_dst = S.tau[1 - cur];
+    const int64_t s = sorted_vals(S)[p];
+    const int64_t cs = src.cap, cd = dst.cap;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      dst.x[a * cd + p] = src.x[a * cs + s];
+      dst.v[a * cd + p] = src.v[a * cs + s];
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      dst.C[e * cd + p] = src.C[e * cs + s];
+      dst.F[e * cd + p] = src.F[e * cs + s];
+    }
+#pragma unroll
+    for (int e = 0; e < 6; ++e) tau_dst[e * cd + p] = tau_src[e * cs + s];
+    dst.Jp[p] = src.Jp[s];
+    dst.mat[p] = src.mat[s];
+    dst.pid[p] = src.pid[s];
+  }
+}
+
+// Single workgroup: scenes that sorted switch to the reordered set; prefix of the running scenes'
+// occupied bricks.
+__global__ void __launch_bounds__(kPlanThreads) k_batch_prefix(const SceneDev* __restrict__ sd,
+                                                               SceneStep* __restrict__ st, int n_scenes,
+                                                               uint32_t* __restrict__ pre) {
+  __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
+  uint32_t* P0 = pre + kPreBricks * (n_scenes + 1);
+  uint32_t carry = 0;
+  for (int base = 0; base < n_scenes; base += kPlanThreads) {
+    const int i = base + threadIdx.x;
+    uint32_t count = 0;
+    if (i < n_scenes) {
+      SceneStep& T = st[i];
+      if (T.sort_now) {
+        T.cur = 1 - T.cur;
+        T.sorted = 1;
+        T.since_sort = 0;
+      }
+      if (T.run) count = *sd[i].n_active;
+    }
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(count, s_warp, total);
+    if (i < n_scenes) P0[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) P0[n_scenes] = carry;
+}
+
+// P2G over every running scene's occupied bricks, one wave per brick (p2g_brick)
+__global__ void __launch_bounds__(256) k_batch_p2g(const SceneDev* __restrict__ sd, const SceneStep* __restrict__ st,
+                                                   const uint32_t* __restrict__ pre, int n_scenes) {
+  constexpr int kWaves = 4;
+  __shared__ float s_win[kWaves][4][kWinNodes];
+  const uint32_t* P0 = pre + kPreBricks * (n_scenes + 1);
+  const int wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const uint32_t total = P0[n_scenes];
+  int i = 0;
+  for (uint32_t w = blockIdx.x * kWaves + wave; w < total; w += gridDim.x * kWaves) {
+    i = uniform(scene_of(P0, n_scenes, w, i));
+    const SceneDev& S = sd[i];
+    const int cur = st[i].cur;
+    const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
+    p2g_brick(S.set[cur], S.tau[cur], S.mats, S.g, k, S.offsets, S.active[w - P0[i]], s_win[wave], S.accum,
+              S.dirty, lane);
+  }
+}
+
+// grid update: one wave per group of 64 bricks; each lane reads one brick's dirty flag, then the wave
+// updates the dirty bricks one after another (grid_brick)
+__global__ void __launch_bounds__(256) k_batch_grid(const SceneDev* __restrict__ sd, const SceneStep* __restrict__ st,
+                                                    const uint32_t* __restrict__ pre, int n_scenes,
+                                                    const ColliderFrame* __restrict__ frames) {
+  const uint32_t* P0 = pre + kPreGroups * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  const int wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  int i = 0;
+  for (uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wave; gw < total; gw += gridDim.x * (blockDim.x >> 6)) {
+    i = uniform(scene_of(P0, n_scenes, gw, i));
+    const SceneDev& S = sd[i];
+    const int first = (int)(gw - P0[i]) * kGroupBricks;
+    const int brick = first + lane;
+    bool d = false;
+    if (brick < S.n_bricks) {
+      d = S.dirty[brick] != 0u;
+      S.live[brick] = d ? 1 : 0;
+      if (d) S.dirty[brick] = 0u;
+    }
+    uint64_t mask = __ballot(d);
+    const float dt = st[i].dt;
+    const ColliderFrame* f = frames + i * kMaxColliders;
+    while (mask) {
+      const int b = first + __builtin_ctzll(mask);
+      mask &= mask - 1;
+      grid_brick(S.g, S.sc, dt, b, S.accum, S.vel, S.cs, f, lane);
+    }
+  }
+}
+
+// G2P of every running scene (g2p_particle); each tile's speed maximum goes to the scene's record
+__global__ void __launch_bounds__(kTile) k_batch_g2p(const SceneDev* __restrict__ sd, SceneStep* __restrict__ st,
+                                                     const uint32_t* __restrict__ pre, int n_scenes,
+                                                     const ColliderFrame* __restrict__ frames) {
+  const uint32_t* P0 = pre + kPreTiles * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  __shared__ float s_max[kTile / 64];
+  int i = 0;
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    i = uniform(scene_of(P0, n_scenes, u, i));
+    const SceneDev& S = sd[i];
+    const int p = (int)(u - P0[i]) * kTile + threadIdx.x;
+    const int cur = st[i].cur;
+    Particles P = S.set[cur];
+    float speed = 0.0f;
+    if (p < S.n)
+      g2p_particle(P, S.mats, S.g, S.sc, st[i].dt, S.vel, S.tau[cur], p, S.cs, frames + i * kMaxColliders, speed);
+    if (!(speed == speed)) speed = __int_as_float(0x7f800000);
+    const float wm = wave_max(speed);
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float b = s_max[0];
+#pragma unroll
+      for (int w = 1; w < kTile / 64; ++w) b = fmaxf(b, s_max[w]);
+      // non-negative floats order like their bit patterns
+      atomicMax(&st[i].speed_bits, __float_as_uint(b));
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_batch_advance(SceneStep* __restrict__ st, int n_scenes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_scenes || !st[i].run) return;
+  st[i].since_sort += 1;
+  st[i].k += 1;
+}
+
+}  // namespace mpm

@@ -51,6 +51,11 @@ struct StepConsts {
   float wall_friction;
 };
 
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 __device__ __forceinline__ int node_addr(const GridGeom& g, int i, int j, int k) {
   return (((i >> 2) * g.bdims[1] + (j >> 2)) * g.bdims[2] + (k >> 2)) * 64 + ((i & 3) << 4) + ((j & 3) << 2) + (k & 3);
 }

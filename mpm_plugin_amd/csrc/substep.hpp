@@ -116,11 +116,100 @@ __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// P2G with one wave per occupied brick and one lane per cell of the brick. Particles are sorted by cell,
-// so lane l walks the particles of cell l and sums their 27 stencil contributions in registers. The
-// 64 lanes of a wave own 64 different cells, so when they then add their sums into the wave's private
-// LDS window every store of one instruction hits a different node: plain read-add-write, no LDS atomics.
-// The window is flushed to the bricked HBM grid with global atomics (neighbouring windows overlap).
+// P2G of occupied brick b by one wave, one lane per cell of the brick. Particles are sorted by cell, so
+// lane l walks the particles of cell l and sums their 27 stencil contributions in registers. The 64 lanes
+// of a wave own 64 different cells, so when they then add their sums into the wave's private LDS window
+// every store of one instruction hits a different node: plain read-add-write, no LDS atomics. The window
+// is flushed to the bricked HBM grid with global atomics (neighbouring windows overlap).
+__device__ __forceinline__ void p2g_brick(const Particles& P, const float* __restrict__ tau,
+                                          const MatDev* __restrict__ mats, const GridGeom& g, float k,
+                                          const uint32_t* __restrict__ cell_off, uint32_t b, float (*win)[kWinNodes],
+                                          float4* __restrict__ accum, uint32_t* __restrict__ dirty, int lane) {
+  const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
+  const int bz = (int)(b % (uint32_t)g.bdims[2]);
+  const int by = (int)((b / (uint32_t)g.bdims[2]) % (uint32_t)g.bdims[1]);
+  const int bx = (int)(b / ((uint32_t)g.bdims[2] * (uint32_t)g.bdims[1]));
+  const int wo[3] = {4 * bx - 1, 4 * by - 1, 4 * bz - 1};
+  for (int i = lane; i < kWinNodes; i += 64) {
+    win[0][i] = 0.0f;
+    win[1][i] = 0.0f;
+    win[2][i] = 0.0f;
+    win[3][i] = 0.0f;
+  }
+  wave_lds_fence();
+  const int cb[3] = {4 * bx + cx, 4 * by + cy, 4 * bz + cz};
+  const uint32_t p0 = cell_off[64 * b + lane], p1 = cell_off[64 * b + lane + 1];
+  float acc[3][3][3][4];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[i][j][kk][c] = 0.0f;
+  for (uint32_t p = p0; p < p1; ++p) {
+    P2GParticle q;
+    load_p2g_particle(P, tau, mats, g, k, (int64_t)p, q);
+    if (q.base[0] == cb[0] && q.base[1] == cb[1] && q.base[2] == cb[2]) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const float wij = q.wx[0][i] * q.wx[1][j];
+          const float ax = q.mv[0] + q.A[0][0] * q.dp[0][i] + q.A[0][1] * q.dp[1][j];
+          const float ay = q.mv[1] + q.A[1][0] * q.dp[0][i] + q.A[1][1] * q.dp[1][j];
+          const float az = q.mv[2] + q.A[2][0] * q.dp[0][i] + q.A[2][1] * q.dp[1][j];
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) {
+            const float wgt = wij * q.wx[2][kk];
+            acc[i][j][kk][0] += wgt * (ax + q.A[0][2] * q.dp[2][kk]);
+            acc[i][j][kk][1] += wgt * (ay + q.A[1][2] * q.dp[2][kk]);
+            acc[i][j][kk][2] += wgt * (az + q.A[2][2] * q.dp[2][kk]);
+            acc[i][j][kk][3] += wgt * q.pm;
+          }
+        }
+      }
+    } else {
+      p2g_scatter_atomic(q, g, wo, win, accum, dirty);
+    }
+  }
+  wave_lds_fence();
+  // Node (cx + 1 + i, ...) of one lane is node (cx' + 1 + i', ...) of a neighbouring lane at another
+  // offset, so the 27 read-add-writes must stay in program order across lanes: the fence after each
+  // offset keeps the compiler from batching the reads of later offsets ahead of earlier writes (a
+  // wave's LDS operations then complete in issue order). Lanes without particles add nothing.
+  const bool any = p1 > p0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const int li = ((cx + 1 + i) * kWin + (cy + 1 + j)) * kWin + (cz + 1 + kk);
+        if (any) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) win[c][li] += acc[i][j][kk][c];
+        }
+        wave_lds_fence();
+      }
+  for (int i = lane; i < kWinNodes; i += 64) {
+    const int gi = wo[0] + (i >> 6), gj = wo[1] + ((i >> 3) & 7), gk = wo[2] + (i & 7);
+    if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
+    const float vx = win[0][i], vy = win[1][i], vz = win[2][i], vm = win[3][i];
+    if (vx == 0.0f && vy == 0.0f && vz == 0.0f && vm == 0.0f) continue;
+    float4* node = accum + node_addr(g, gi, gj, gk);
+    if (vx != 0.0f) atomicAdd(&node->x, vx);
+    if (vy != 0.0f) atomicAdd(&node->y, vy);
+    if (vz != 0.0f) atomicAdd(&node->z, vz);
+    if (vm != 0.0f) atomicAdd(&node->w, vm);
+  }
+  if (lane < 27) mark_brick(g, dirty, bx - 1 + lane / 9, by - 1 + (lane / 3) % 3, bz - 1 + lane % 3);
+  wave_lds_fence();
+}
+
+// P2G over the device-side list of occupied bricks: four waves (four bricks at a time) per workgroup,
+// grid-stride, so the launch needs no host-side count
 __global__ void __launch_bounds__(256) k_p2g_cells(Particles P, const float* __restrict__ tau,
                                                    const MatDev* __restrict__ mats, GridGeom g, float p_vol, float dt,
                                                    const uint32_t* __restrict__ cell_off,
@@ -130,93 +219,28 @@ __global__ void __launch_bounds__(256) k_p2g_cells(Particles P, const float* __r
   constexpr int kWaves = 4;
   __shared__ float s_win[kWaves][4][kWinNodes];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float (*win)[kWinNodes] = s_win[wave];
   const uint32_t n_active = *n_active_ptr;
   const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
-  const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
-  for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves) {
-    const uint32_t b = active[w];
-    const int bz = (int)(b % (uint32_t)g.bdims[2]);
-    const int by = (int)((b / (uint32_t)g.bdims[2]) % (uint32_t)g.bdims[1]);
-    const int bx = (int)(b / ((uint32_t)g.bdims[2] * (uint32_t)g.bdims[1]));
-    const int wo[3] = {4 * bx - 1, 4 * by - 1, 4 * bz - 1};
-    for (int i = lane; i < kWinNodes; i += 64) {
-      win[0][i] = 0.0f;
-      win[1][i] = 0.0f;
-      win[2][i] = 0.0f;
-      win[3][i] = 0.0f;
-    }
-    wave_lds_fence();
-    const int cb[3] = {4 * bx + cx, 4 * by + cy, 4 * bz + cz};
-    const uint32_t p0 = cell_off[64 * b + lane], p1 = cell_off[64 * b + lane + 1];
-    float acc[3][3][3][4];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) acc[i][j][kk][c] = 0.0f;
-    for (uint32_t p = p0; p < p1; ++p) {
-      P2GParticle q;
-      load_p2g_particle(P, tau, mats, g, k, (int64_t)p, q);
-      if (q.base[0] == cb[0] && q.base[1] == cb[1] && q.base[2] == cb[2]) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const float wij = q.wx[0][i] * q.wx[1][j];
-            const float ax = q.mv[0] + q.A[0][0] * q.dp[0][i] + q.A[0][1] * q.dp[1][j];
-            const float ay = q.mv[1] + q.A[1][0] * q.dp[0][i] + q.A[1][1] * q.dp[1][j];
-            const float az = q.mv[2] + q.A[2][0] * q.dp[0][i] + q.A[2][1] * q.dp[1][j];
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk) {
-              const float wgt = wij * q.wx[2][kk];
-              acc[i][j][kk][0] += wgt * (ax + q.A[0][2] * q.dp[2][kk]);
-              acc[i][j][kk][1] += wgt * (ay + q.A[1][2] * q.dp[2][kk]);
-              acc[i][j][kk][2] += wgt * (az + q.A[2][2] * q.dp[2][kk]);
-              acc[i][j][kk][3] += wgt * q.pm;
-            }
-          }
-        }
-      } else {
-        p2g_scatter_atomic(q, g, wo, win, accum, dirty);
-      }
-    }
-    wave_lds_fence();
-    // Node (cx + 1 + i, ...) of one lane is node (cx' + 1 + i', ...) of a neighbouring lane at another
-    // offset, so the 27 read-add-writes must stay in program order across lanes: the fence after each
-    // offset keeps the compiler from batching the reads of later offsets ahead of earlier writes (a
-    // wave's LDS operations then complete in issue order). Lanes without particles add nothing.
-    const bool any = p1 > p0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk) {
-          const int li = ((cx + 1 + i) * kWin + (cy + 1 + j)) * kWin + (cz + 1 + kk);
-          if (any) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) win[c][li] += acc[i][j][kk][c];
-          }
-          wave_lds_fence();
-        }
-    for (int i = lane; i < kWinNodes; i += 64) {
-      const int gi = wo[0] + (i >> 6), gj = wo[1] + ((i >> 3) & 7), gk = wo[2] + (i & 7);
-      if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
-      const float vx = win[0][i], vy = win[1][i], vz = win[2][i], vm = win[3][i];
-      if (vx == 0.0f && vy == 0.0f && vz == 0.0f && vm == 0.0f) continue;
-      float4* node = accum + node_addr(g, gi, gj, gk);
-      if (vx != 0.0f) atomicAdd(&node->x, vx);
-      if (vy != 0.0f) atomicAdd(&node->y, vy);
-      if (vz != 0.0f) atomicAdd(&node->z, vz);
-      if (vm != 0.0f) atomicAdd(&node->w, vm);
-    }
-    if (lane < 27) mark_brick(g, dirty, bx - 1 + lane / 9, by - 1 + (lane / 3) % 3, bz - 1 + lane % 3);
-    wave_lds_fence();
+  for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves)
+    p2g_brick(P, tau, mats, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane);
+}
+
+// grid update of one dirty brick by one wave (lane = node): momentum -> velocity, gravity, walls,
+// collider contact; clears the accumulator behind itself
+__device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& sc, float dt, int brick,
+                                           float4* __restrict__ accum, float4* __restrict__ vel,
+                                           const ColliderSet& cs, const ColliderFrame* frames, int lane) {
+  const int addr = brick * 64 + lane;
+  int i, j, k;
+  node_ijk(g, addr, i, j, k);
+  const float4 a = accum[addr];
+  float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (i < g.dims[0] && j < g.dims[1] && k < g.dims[2]) {
+    out = update_node(g, sc, dt, i, j, k, a);
+    if (cs.count > 0 && out.w > 0.0f) collide_node_frames(cs, frames, g, i, j, k, out);
   }
+  vel[addr] = out;
+  if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f || a.w != 0.0f) accum[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // one wave per brick; non-dirty bricks are skipped (and flagged stale for the grid download)
@@ -232,17 +256,7 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
   const bool d = dirty[brick] != 0u;
   if (lane == 0) live[brick] = d ? 1 : 0;
   if (!d) return;
-  const int addr = brick * 64 + lane;
-  int i, j, k;
-  node_ijk(g, addr, i, j, k);
-  const float4 a = accum[addr];
-  float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (i < g.dims[0] && j < g.dims[1] && k < g.dims[2]) {
-    out = update_node(g, sc, dt, i, j, k, a);
-    if (cs.count > 0 && out.w > 0.0f) collide_node_frames(cs, frames, g, i, j, k, out);
-  }
-  vel[addr] = out;
-  if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f || a.w != 0.0f) accum[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  grid_brick(g, sc, dt, brick, accum, vel, cs, frames, lane);
   if (lane == 0) dirty[brick] = 0u;
 }
 

@@ -11,6 +11,13 @@
   (-3, 0, 0) m/s: about 1.0 M particles at 8 per cell; one closed icosphere collider (2562 vertices,
   r 0.1, friction 0.5) moving from (0.5, 0.5, 0.88) to (0.5, 0.5, 0.70) over 0.1 s; dt = 5e-5 for 2000
   substeps (one substep per advance: the CFL step is about 6.6e-5).
+* ``scene_mix``: many independent scenes for the batched engine. Grid edge uniform in
+  {32, 48, 64, 96, 128} over the unit cube; particle count log-uniform in [5k, 400k], at most
+  0.35 * 8 * edge^3 so the body fits at 8 particles per cell; one body (sphere or box) of one material
+  drawn uniformly from the four models, its parameters jittered by a uniform factor in [0.8, 1.2] around
+  the values above (elastic: the elastic block's); random velocity up to 3 m/s; 0-2 icosphere colliders
+  (162 vertices) on straight paths; frames of 1/24 s. Bodies are jittered lattices (spacing dx/2, jitter
+  +-0.25 of the spacing), generated directly rather than through mesh sampling for speed.
 
 Particles are sampled with mpm_core.geometry.sample_mesh_volume, as the runner emits them.
 """
@@ -102,6 +109,8 @@ class Workload:
     colliders: list[ColliderGeometry] = field(default_factory=list)
     collider_path: list[tuple[np.ndarray, np.ndarray]] = field(default_factory=list)  # (start, end) per collider
     duration: float = 0.0
+    collider_scale: list[float] = field(default_factory=list)  # uniform scale per collider (default 0.1)
+    frame_dt: float = 1.0 / 24.0
 
     @property
     def n(self) -> int:
@@ -110,11 +119,12 @@ class Workload:
     def collider_transforms(self, t0: float, t1: float) -> tuple[np.ndarray, np.ndarray]:
         """(C,4,4) start/end world matrices at times t0 and t1 of a linear translation path."""
         starts, ends = [], []
-        for (a, b), geom in zip(self.collider_path, self.colliders):
+        for c, ((a, b), geom) in enumerate(zip(self.collider_path, self.colliders)):
             del geom
+            scale = self.collider_scale[c] if c < len(self.collider_scale) else 0.1
             for t, out in ((t0, starts), (t1, ends)):
                 u = min(max(t / self.duration, 0.0), 1.0) if self.duration > 0 else 0.0
-                out.append(_translate(*(a + u * (b - a))) @ _scale(0.1))
+                out.append(_translate(*(a + u * (b - a))) @ _scale(scale))
         return np.stack(starts), np.stack(ends)
 
     def make_solver(self, backend: str = "hip", device: str | None = None):
@@ -162,3 +172,74 @@ def multi_material(seed: int = 1, with_collider: bool = True) -> Workload:
         w.colliders = [ColliderGeometry(MeshData(verts, tris), closed=True, friction=0.5)]
         w.collider_path = [(np.array([0.5, 0.5, 0.88]), np.array([0.5, 0.5, 0.70]))]
     return w
+
+
+MIX_EDGES = (32, 48, 64, 96, 128)
+MIX_MODELS = ("elastic", "von_mises", "drucker_prager", "snow")
+
+
+def _jittered_material(model: str, rng: np.random.Generator) -> MaterialParams:
+    def j(v: float) -> float:
+        return float(v * rng.uniform(0.8, 1.2))
+
+    if model == "elastic":
+        return MaterialParams("elastic", "elastic", j(1.0e4), min(j(0.3), 0.45), j(1000.0))
+    if model == "von_mises":
+        return MaterialParams("plastic", "von_mises", j(5.0e3), min(j(0.45), 0.49), j(1000.0), yield_stress=j(200.0))
+    if model == "drucker_prager":
+        return MaterialParams("sand", "drucker_prager", j(3.5e4), min(j(0.3), 0.45), j(1600.0),
+                              friction_angle_deg=j(30.0))
+    return MaterialParams("snow", "snow", j(1.4e5), min(j(0.2), 0.45), j(400.0), hardening=j(10.0),
+                          critical_compression=j(2.5e-2), critical_stretch=j(7.5e-3), max_hardening=j(5.0))
+
+
+def lattice_body(shape: str, center, size: float, spacing: float, rng: np.random.Generator) -> np.ndarray:
+    """Jittered lattice points (jitter +-0.25 spacing) inside a sphere (size = radius) or box (size = side)."""
+    half = size if shape == "sphere" else size / 2
+    axes = [np.arange(c - half + spacing / 2, c + half, spacing) for c in center]
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    if shape == "sphere":
+        g = g[np.sum((g - np.asarray(center)) ** 2, axis=1) <= half * half]
+    g = g + rng.uniform(-0.25 * spacing, 0.25 * spacing, g.shape)
+    return g.astype(np.float32)
+
+
+def mix_scene(index: int, seed: int = 7, n_range=(5_000, 400_000), edges=MIX_EDGES, model: str | None = None,
+              n_colliders: int | None = None, walls: dict | None = None) -> Workload:
+    """Scene `index` of the batched mix (deterministic in (seed, index)); `model`, `n_colliders` and
+    `walls` pin what is otherwise drawn (all walls closed by default)."""
+    rng = np.random.default_rng([seed, index])
+    edge = int(edges[rng.integers(len(edges))])
+    n_target = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+    n_target = min(n_target, 0.35 * 8 * edge ** 3)
+    drawn = MIX_MODELS[rng.integers(len(MIX_MODELS))]
+    model = model or drawn
+    mat = _jittered_material(model, rng)
+    domain = Domain((0, 0, 0), (1, 1, 1), edge, walls=dict(walls or {}))
+    dx = domain.dx
+    volume = n_target / (8.0 / dx ** 3)
+    shape = "sphere" if rng.random() < 0.5 else "box"
+    size = (3.0 * volume / (4.0 * np.pi)) ** (1.0 / 3.0) if shape == "sphere" else volume ** (1.0 / 3.0)
+    half = size if shape == "sphere" else size / 2
+    lo, hi = half + 2 * dx, 1.0 - half - 2 * dx
+    center = [float(rng.uniform(lo, hi)) if hi > lo else 0.5 for _ in range(3)]
+    x = lattice_body(shape, center, size, dx / 2, rng)
+    direction = rng.normal(size=3)
+    vel = (direction / np.linalg.norm(direction) * rng.uniform(0.0, 3.0)).astype(np.float32)
+    n = len(x)
+    w = Workload(f"mix_{index}_{model}_{edge}", domain, [mat], (0.0, -9.81, 0.0), x, np.tile(vel, (n, 1)),
+                 np.zeros(n, np.int32), np.zeros(n, np.int32), dt=0.0, substeps=0, duration=1.0)
+    n_col = int(rng.integers(3))
+    n_col = n_col if n_colliders is None else n_colliders
+    if n_col:
+        verts, tris = icosphere(2)
+        for _ in range(n_col):
+            w.colliders.append(ColliderGeometry(MeshData(verts, tris), closed=True, friction=float(rng.uniform(0, 1))))
+            w.collider_path.append((rng.uniform(0.15, 0.85, 3), rng.uniform(0.15, 0.85, 3)))
+            w.collider_scale.append(float(rng.uniform(0.06, 0.15)))
+    return w
+
+
+def scene_mix(n_scenes: int = 256, seed: int = 7, **kw) -> list[Workload]:
+    """The batched mix: ``n_scenes`` scenes of :func:`mix_scene` (about 25 M particles for 256)."""
+    return [mix_scene(i, seed, **kw) for i in range(n_scenes)]
