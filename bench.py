@@ -8,6 +8,11 @@ projection. Inputs are resident in HBM before the timed region; the timed region
 device synchronisation. Per-kernel device time comes from HIP events recorded by the library on its
 own stream around every launch.
 
+A second measurement (skipped with --no-batch) runs the batched engine on the 256-scene mix
+(mpm_plugin_amd.workloads.scene_mix: grid edges 32-128, 5k-400k particles per scene, four material models,
+0-2 colliders each): the same frames advanced once one scene at a time (``Solver.advance`` per scene, in
+turn) and once through one ``SceneBatch``; both counts are sums of particles x substeps actually run.
+
 The printed JSON line carries:
   roofline      for the kernel with the largest device time: algorithmic bytes per launch (bytes per
                 particle x particles + bytes per active grid node x active nodes) over its mean launch
@@ -51,6 +56,9 @@ def parse_args(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-collider", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched scene-mix measurement")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-frames", type=int, default=2, help="timed frames of the scene mix")
     return ap.parse_args(argv)
 
 
@@ -97,6 +105,64 @@ def cpu_baseline(w, seconds: float) -> dict:
     return {"value": w.n * steps / elapsed, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} substeps of the same {w.n}-particle workload (FP32 C++ restatement of the "
                       f"reference Warp kernels, OpenMP, {threads} threads on {cpu}), {elapsed:.1f} s"}
+
+
+def run_frames(pairs, frames: int, first_frame: int, advance_all) -> tuple[float, int]:
+    """Advance every scene `frames` frames; returns (seconds, particle-substeps)."""
+    work = 0
+    t0 = time.perf_counter()
+    for f in range(first_frame, first_frame + frames):
+        for w, s in pairs:
+            if w.colliders:
+                s.set_collider_transforms(*w.collider_transforms(f * w.frame_dt, (f + 1) * w.frame_dt))
+        counts = [s.particle_count for _, s in pairs]
+        stats = advance_all(pairs)
+        work += sum(n * st.substeps for n, st in zip(counts, stats))
+        log(f"  frame {f}: substeps per scene {min(st.substeps for st in stats)}-"
+            f"{max(st.substeps for st in stats)}, {time.perf_counter() - t0:.2f} s")
+    return time.perf_counter() - t0, work
+
+
+def batch_bench(n_scenes: int, frames: int) -> dict:
+    """The scene mix advanced one scene at a time and through one SceneBatch, same frames."""
+    from mpm_core.backends.hip.batch import SceneBatch
+    from mpm_plugin_amd.workloads import scene_mix
+
+    mix = scene_mix(n_scenes)
+    total = sum(w.n for w in mix)
+    log(f"scene mix: {len(mix)} scenes, {total} particles")
+    result = {"workload": f"scene_mix_{n_scenes}", "scenes": len(mix), "particles": int(total),
+              "frames": frames, "frame_dt": mix[0].frame_dt}
+
+    def plain(pairs):
+        return [s.advance(w.frame_dt) for w, s in pairs]
+
+    pairs = [(w, w.make_solver(device="cuda:0")) for w in mix]
+    run_frames(pairs, 1, 0, plain)  # warm-up frame
+    secs, work = run_frames(pairs, frames, 1, plain)
+    result["plain"] = {"value": work / secs, "seconds": round(secs, 3), "particle_substeps": int(work)}
+    log(f"plain: {work / secs:.3e} particle-substeps/s ({secs:.2f} s)")
+    for _, s in pairs:
+        s.close()
+    del pairs
+
+    pairs = [(w, w.make_solver(device="cuda:0")) for w in mix]
+    batch = SceneBatch([s for _, s in pairs])
+
+    def batched(ps):
+        return batch.advance([w.frame_dt for w, _ in ps])
+
+    run_frames(pairs, 1, 0, batched)
+    launched = batch.substeps_launched
+    secs, work = run_frames(pairs, frames, 1, batched)
+    result["batched"] = {"value": work / secs, "seconds": round(secs, 3), "particle_substeps": int(work),
+                         "substep_launches": int(batch.substeps_launched - launched)}
+    log(f"batched: {work / secs:.3e} particle-substeps/s ({secs:.2f} s)")
+    result["speedup"] = result["batched"]["value"] / result["plain"]["value"]
+    batch.close()
+    for _, s in pairs:
+        s.close()
+    return result
 
 
 def main(argv=None) -> int:
@@ -160,6 +226,11 @@ def main(argv=None) -> int:
         baseline = cpu_baseline(w, args.cpu_seconds)
         log(f"cpu baseline {baseline['value']:.3e} particle-substeps/s")
 
+    batched = None
+    if not args.no_batch:
+        solver.close()
+        batched = batch_bench(args.batch_scenes, args.batch_frames)
+
     value = n * args.steps / elapsed
     line = {
         "metric": "particle-substeps/s at 1M pts/128^3",
@@ -180,6 +251,7 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "cpu_baseline": baseline,
         "kernels": kernels,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     return 0
