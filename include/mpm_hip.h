@@ -179,6 +179,16 @@ int32_t mpm_batch_scene_count(const mpm_batch* batch);
 /* one frame of every scene i of length frame_dt[i] (<= 0: scene i does not advance), each with its own
  * CFL substep count and escaped-particle removal, exactly as mpm_advance; out[i] as mpm_advance */
 int32_t mpm_batch_advance(mpm_batch* batch, const double* frame_dt, mpm_advance_stats* out);
+/* Frames that start and end independently (continuous batching): mpm_batch_begin starts a frame of
+ * length frame_dt[i] for every scene i with frame_dt[i] > 0 (the scene must not be in a frame); then each
+ * mpm_batch_run replays substeps of every scene in a frame: max_launches > 0 replays that many, 0 until
+ * every frame in flight has ended, < 0 until the first one has. Scenes whose frame ended in the call get
+ * finished[i] = 1 and out[i]. While a scene is in a frame, the mpm_* calls on it fail with MPM_ERR_VALUE. */
+int32_t mpm_batch_begin(mpm_batch* batch, const double* frame_dt);
+int32_t mpm_batch_run(mpm_batch* batch, int64_t max_launches, int64_t* launched, int32_t* finished,
+                      mpm_advance_stats* out);
+/* substeps done / in the frame in flight, per scene (equal when the scene is not in a frame) */
+int32_t mpm_batch_progress(const mpm_batch* batch, int32_t* substeps_done, int32_t* substeps_total);
 #define MPM_BATCH_OPT_P2G_BLOCKS 0        /* workgroups of the persistent P2G launch (0: automatic) */
 #define MPM_BATCH_OPT_GRAPH 1             /* 1: replay captured graphs (default); 0: launch directly */
 #define MPM_BATCH_OPT_SORT_INTERVAL 2     /* substeps between re-sorts for every scene (0: each scene's own) */

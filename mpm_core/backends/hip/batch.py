@@ -6,6 +6,10 @@ state reads, emission, checkpoints) and ``SceneBatch.advance`` does, per scene, 
 does (src/mpm_core/backends/warp/solver.py:213-240: CFL step count from the scene's own speed maximum,
 its substeps, escaped-particle removal), only for all scenes at once: each substep of the batch is one
 replay of a captured HIP graph in which every still-running scene takes part.
+
+``begin`` and ``run`` expose the same engine with frames that start and end independently (continuous
+batching): a scene whose frame ended can be serviced (frame written, emission, poses) and start its next
+frame while the others are still in theirs.
 """
 
 from __future__ import annotations
@@ -67,12 +71,42 @@ class SceneBatch:
     def advance(self, frame_dt: float | Sequence[float]) -> list[AdvanceStats]:
         """One frame of every member (frame_dt per member, or one value for all; <= 0 skips a member)."""
         n = len(self._scenes)
-        if any(s._ctx is None for s in self._scenes) or self._lib.mpm_batch_scene_count(self._b) != n:
-            raise ValueError("a batch member was closed: call set_scenes with the open ones")
+        self._check_members()
         fdt = np.broadcast_to(np.asarray(frame_dt, np.float64), (n,)).copy()
         stats = (N.AdvanceStats * max(1, n))()
         N.check(self._lib.mpm_batch_advance(self._b, fdt.ctypes.data_as(C.POINTER(C.c_double)), stats))
         return [AdvanceStats(int(stats[i].substeps), float(stats[i].dt), int(stats[i].removed)) for i in range(n)]
+
+    def begin(self, frame_dt: float | Sequence[float]) -> None:
+        """Start a frame of every member with frame_dt > 0 (it must not be in a frame)."""
+        n = len(self._scenes)
+        self._check_members()
+        fdt = np.broadcast_to(np.asarray(frame_dt, np.float64), (n,)).copy()
+        N.check(self._lib.mpm_batch_begin(self._b, fdt.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def run(self, max_launches: int = -1) -> tuple[int, dict[int, AdvanceStats]]:
+        """Replay substeps (max_launches > 0: that many; 0: until every frame in flight ended; < 0: until the
+        first one ended). Returns (substep launches, {member index: stats of its frame that ended})."""
+        n = len(self._scenes)
+        self._check_members()
+        launched = C.c_int64()
+        finished = np.zeros(max(1, n), np.int32)
+        stats = (N.AdvanceStats * max(1, n))()
+        N.check(self._lib.mpm_batch_run(self._b, int(max_launches), C.byref(launched), N.iptr(finished), stats))
+        done = {i: AdvanceStats(int(stats[i].substeps), float(stats[i].dt), int(stats[i].removed))
+                for i in range(n) if finished[i]}
+        return int(launched.value), done
+
+    def progress(self) -> tuple[np.ndarray, np.ndarray]:
+        """(substeps done, substeps of the frame in flight) per member."""
+        n = len(self._scenes)
+        done, total = np.zeros(max(1, n), np.int32), np.zeros(max(1, n), np.int32)
+        N.check(self._lib.mpm_batch_progress(self._b, N.iptr(done), N.iptr(total)))
+        return done[:n], total[:n]
+
+    def _check_members(self) -> None:
+        if any(s._ctx is None for s in self._scenes) or self._lib.mpm_batch_scene_count(self._b) != len(self._scenes):
+            raise ValueError("a batch member was closed: call set_scenes with the open ones")
 
     def set_option(self, option: int, value: int) -> None:
         N.check(self._lib.mpm_batch_set_option(self._b, int(option), int(value)))

@@ -89,6 +89,9 @@ def _declare(L: C.CDLL) -> None:
         "mpm_batch_set_scenes": ([ctx, C.POINTER(ctx), i32], i32),
         "mpm_batch_scene_count": ([ctx], i32),
         "mpm_batch_advance": ([ctx, C.POINTER(f64), C.POINTER(AdvanceStats)], i32),
+        "mpm_batch_begin": ([ctx, C.POINTER(f64)], i32),
+        "mpm_batch_run": ([ctx, i64, i64p, ip, C.POINTER(AdvanceStats)], i32),
+        "mpm_batch_progress": ([ctx, ip, ip], i32),
         "mpm_batch_set_option": ([ctx, i32, i64], i32),
         "mpm_batch_get_option": ([ctx, i32], i64),
     }

@@ -132,6 +132,7 @@ struct mpm_ctx {
   hipStream_t stream = nullptr;      // the stream every operation of the context runs on
   hipStream_t own_stream = nullptr;  // the context's own stream (stream is the batch's while in a batch)
   mpm_batch* batch = nullptr;
+  bool busy = false;                 // in the middle of a batched frame: its state belongs to the batch
   uint64_t version = 0;              // bumped by every change of the particle state
   mpm_domain_desc dom{};
   std::vector<mpm_material_desc> mats;
@@ -204,13 +205,18 @@ struct mpm_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<mpm_ctx*> scenes;
-  // per scene, host side: the speed maximum that the next frame's CFL step uses and the context version
-  // it belongs to
-  std::vector<float> speed;
-  std::vector<uint64_t> version;
+  // per scene, host side. `step` is an exact copy of the device step table between runs (every run
+  // uploads it, replays substeps, and reads it back); a frame starts by rewriting a scene's entries.
+  std::vector<SceneDev> rec;
+  std::vector<SceneStep> step;
+  std::vector<float> speed;        // speed maximum the next frame's CFL step uses
+  std::vector<uint64_t> version;   // context version `speed` belongs to
   std::vector<char> speed_ok;
+  std::vector<double> frame_dt;    // of the frame in flight
+  std::vector<double> dt;          // its substep length
+  std::vector<char> empty_frame;   // a frame of zero substeps was started (reported by the next run)
 
-  // device tables and their pinned host mirrors
+  // device tables and pinned staging for their transfers
   int64_t table_cap = 0;
   SceneDev* d_scenes = nullptr;
   SceneStep* d_steps = nullptr;
@@ -234,7 +240,7 @@ struct mpm_batch {
   int64_t g2p_blocks = 0;
   int64_t grid_blocks = 0;
   int64_t use_graph = 1;
-  int64_t sort_interval = 0;   // 0: each context's own
+  int64_t sort_interval = 0;  // 0: each context's own
   int64_t substeps_launched = 0;
 };
 
@@ -782,6 +788,10 @@ void mesh_tables(const float* verts, int nv, const int32_t* tris, int nt, Collid
 }
 
 // ---- batches -----------------------------------------------------------------------------------
+void require_idle(const mpm_ctx* c) {
+  if (c->busy) fail(MPM_ERR_VALUE, "the scene is in the middle of a batched frame: run its batch first");
+}
+
 void batch_drop_graphs(mpm_batch* b) {
   if (b->g_sort) (void)hipGraphExecDestroy(b->g_sort);
   if (b->g_plain) (void)hipGraphExecDestroy(b->g_plain);
@@ -813,8 +823,10 @@ template <class T> T* host_alloc(size_t count) {
   return static_cast<T*>(p);
 }
 
+// device tables for n scenes (contents are uploaded from the host copies at every run)
 void batch_reserve_tables(mpm_batch* b, int64_t n) {
   if (n <= b->table_cap) return;
+  HIP_CHECK(hipStreamSynchronize(b->stream));
   batch_drop_graphs(b);
   batch_free_tables(b);
   const int64_t cap = std::max<int64_t>(n, 16);
@@ -827,13 +839,6 @@ void batch_reserve_tables(mpm_batch* b, int64_t n) {
   b->h_steps = host_alloc<SceneStep>(cap);
   b->h_speed = host_alloc<float>(cap);
   b->table_cap = cap;
-}
-
-void detach_scene(mpm_ctx* c) {
-  if (!c->batch) return;
-  (void)hipStreamSynchronize(c->stream);
-  c->stream = c->own_stream;
-  c->batch = nullptr;
 }
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -853,8 +858,8 @@ void batch_layout(mpm_batch* b) {
   if (256 * tiles > (int64_t)kScanTile * kScanTile || 256 * tiles > (int64_t)UINT32_MAX)
     fail(MPM_ERR_VALUE, "batch too large for the radix histogram scan");
   if (tiles <= b->hist_tiles && passes == b->max_passes) return;
-  batch_drop_graphs(b);
   HIP_CHECK(hipStreamSynchronize(b->stream));
+  batch_drop_graphs(b);
   // sized before any capture: nothing allocates inside one
   b->hist.get(256 * tiles * sizeof(uint32_t));
   b->scan_sums.get((ceil_div(256 * tiles, kScanTile) + 1) * sizeof(uint32_t));
@@ -869,7 +874,6 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   const int n = (int)b->scenes.size();
   hipStream_t s = b->stream;
   // persistent grids: a few workgroups per CU, striding over the running scenes' work
-  const unsigned per_particle = batch_blocks(b->g2p_blocks, 2048);
   k_batch_plan<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, b->d_frames, n, sorting ? 1 : 0, b->d_pre);
   check_launch("k_batch_plan");
   if (sorting) {
@@ -895,7 +899,8 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   k_batch_grid<<<batch_blocks(b->grid_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n,
                                                                   b->d_frames);
   check_launch("k_batch_grid");
-  k_batch_g2p<<<per_particle, kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n, b->d_frames);
+  k_batch_g2p<<<batch_blocks(b->g2p_blocks, 2048), kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n,
+                                                                  b->d_frames);
   check_launch("k_batch_g2p");
   k_batch_advance<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(b->d_steps, n);
   check_launch("k_batch_advance");
@@ -949,123 +954,211 @@ SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
   return d;
 }
 
-// one frame of every scene (solver.py:213-220 per scene): CFL step counts from each scene's speed
-// maximum, then substep launches until every scene has run its count
-void batch_frame(mpm_batch* b, const double* frame_dt, mpm_advance_stats* out) {
+bool in_frame(const SceneStep& s) { return s.k < s.steps; }
+
+// Start a frame of every scene i with frame_dt[i] > 0 (it must not be in a frame): its CFL substep count
+// from its speed maximum (the arithmetic of mpm_advance, solver.py:213-220) and fresh table entries. The
+// substeps run in the following runs.
+void batch_begin(mpm_batch* b, const double* frame_dt) {
   const int64_t n = (int64_t)b->scenes.size();
-  if (n == 0) return;
-  batch_reserve_tables(b, n);
-  for (mpm_ctx* c : b->scenes)
+  std::vector<int64_t> starting;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!(frame_dt[i] > 0.0)) continue;
+    mpm_ctx* c = b->scenes[i];
+    if (in_frame(b->step[i]) || b->empty_frame[i]) fail(MPM_ERR_VALUE, "a scene of the batch is still in a frame");
     if (c->p2g_mode != 0) fail(MPM_ERR_VALUE, "batched scenes use the binned P2G (option P2G_MODE 0)");
+    starting.push_back(i);
+  }
+  if (starting.empty()) return;
   // speed maxima of scenes whose state changed since the batch last saw them
   bool any = false;
-  std::vector<char> fresh(n, 0);
-  for (int64_t i = 0; i < n; ++i) {
+  for (int64_t i : starting) {
     mpm_ctx* c = b->scenes[i];
-    if (c->n == 0 || !(frame_dt[i] > 0.0)) continue;
-    if (b->speed_ok[i] && b->version[i] == c->version) continue;
+    if (c->n == 0 || (b->speed_ok[i] && b->version[i] == c->version)) continue;
+    batch_reserve_tables(b, n);
     reduce_speed_async(c, b->d_speed + i);
-    fresh[i] = 1;
     any = true;
   }
   if (any) {
     HIP_CHECK(hipMemcpyAsync(b->h_speed, b->d_speed, n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
-    for (int64_t i = 0; i < n; ++i)
-      if (fresh[i]) {
-        b->speed[i] = b->h_speed[i];
-        b->speed_ok[i] = 1;
-        b->version[i] = b->scenes[i]->version;
-      }
+    for (int64_t i : starting) {
+      mpm_ctx* c = b->scenes[i];
+      if (c->n == 0 || (b->speed_ok[i] && b->version[i] == c->version)) continue;
+      b->speed[i] = b->h_speed[i];
+      b->speed_ok[i] = 1;
+      b->version[i] = c->version;
+    }
   }
-  batch_layout(b);
-  // step counts (the arithmetic of mpm_advance) and the host mirror of the sort cadence
-  std::vector<int64_t> steps(n, 0);
-  std::vector<double> dts(n, 0.0);
-  std::vector<int32_t> since(n), sorted(n), interval(n);
-  int64_t max_steps = 0;
-  for (int64_t i = 0; i < n; ++i) {
+  for (int64_t i : starting) {
     mpm_ctx* c = b->scenes[i];
     const double fdt = frame_dt[i];
-    if (c->n > 0 && fdt > 0.0) {
-      double vmax = (double)b->speed[i];
-      if (!std::isfinite(vmax)) vmax = 0.0;
-      double wave = 0.0;
-      for (size_t m = 0; m < c->mats.size(); ++m)
-        if (c->present[m]) wave = std::max(wave, c->mats[m].wave_speed);
-      const double cfl_dt = std::min(0.4 * c->dom.dx_f64 / (wave + vmax), fdt);
-      steps[i] = (int64_t)std::ceil(fdt / cfl_dt - 1e-9);
-      dts[i] = fdt / (double)steps[i];
-      if (steps[i] > (int64_t)1 << 30) fail(MPM_ERR_VALUE, "too many substeps in one frame");
-      c->frame_dt = fdt;
+    b->frame_dt[i] = fdt;
+    if (c->n == 0) {  // nothing to advance (mpm_advance returns at once)
+      b->dt[i] = fdt;
+      b->empty_frame[i] = 1;
+      continue;
     }
-    max_steps = std::max(max_steps, steps[i]);
-    const SceneDev rec = scene_record(b, c);
-    b->h_scenes[i] = rec;
-    SceneStep& st = b->h_steps[i];
+    double vmax = (double)b->speed[i];
+    if (!std::isfinite(vmax)) vmax = 0.0;  // already blown up: keep stepping so the NaN reaches x
+    double wave = 0.0;
+    for (size_t m = 0; m < c->mats.size(); ++m)
+      if (c->present[m]) wave = std::max(wave, c->mats[m].wave_speed);
+    const double cfl_dt = std::min(0.4 * c->dom.dx_f64 / (wave + vmax), fdt);
+    const int64_t steps = (int64_t)std::ceil(fdt / cfl_dt - 1e-9);
+    if (steps > (int64_t)1 << 30) fail(MPM_ERR_VALUE, "too many substeps in one frame");
+    b->dt[i] = fdt / (double)steps;
+    c->frame_dt = fdt;
+    b->rec[i] = scene_record(b, c);
+    SceneStep& st = b->step[i];
     st = SceneStep{};
-    st.dt = (float)dts[i];
-    st.inv_frame_dt = fdt > 0.0 ? (float)(1.0 / fdt) : 0.0f;
-    st.k = 0;
-    st.steps = (int32_t)steps[i];
+    st.dt = (float)b->dt[i];
+    st.inv_frame_dt = (float)(1.0 / fdt);
+    st.steps = (int32_t)steps;
     st.since_sort = (int32_t)std::min<int64_t>(c->steps_since_sort, 1 << 30);
     st.sorted = c->sorted ? 1 : 0;
-    st.cur = 0;
-    since[i] = st.since_sort;
-    sorted[i] = st.sorted;
-    interval[i] = rec.sort_interval;
+    c->busy = true;
   }
+}
+
+// hand a scene back to its context after its frame's last substep (or when it leaves the batch)
+void batch_release_scene(mpm_batch* b, int64_t i) {
+  mpm_ctx* c = b->scenes[i];
+  SceneStep& st = b->step[i];
+  if (!c->busy) return;
+  if (st.cur) std::swap(c->A, c->B);
+  st.cur = 0;
+  c->sorted = st.sorted != 0;
+  c->steps_since_sort = st.since_sort;
+  c->last_perm = const_cast<uint32_t*>((radix_passes(c) - 1) % 2 == 0 ? c->vals1 : c->vals2);
+  c->vmax_valid = false;  // the batch keeps the speed maximum; the context's partial maxima are stale
+  c->version += 1;
+  c->busy = false;
+  b->rec[i] = scene_record(b, c);
+}
+
+void batch_finish_scene(mpm_batch* b, int64_t i, mpm_advance_stats* out) {
+  mpm_ctx* c = b->scenes[i];
+  const SceneStep st = b->step[i];
+  batch_release_scene(b, i);
+  uint32_t bits = st.speed_bits;
+  float v;
+  std::memcpy(&v, &bits, 4);
+  b->speed[i] = v;
+  b->speed_ok[i] = 1;
+  const int64_t removed = remove_escaped(c);
+  if (removed) {
+    refresh_present(c);
+    b->rec[i] = scene_record(b, c);
+  }
+  b->version[i] = c->version;
+  *out = mpm_advance_stats{st.steps, b->dt[i], removed};
+}
+
+// Replay substeps: max_launches > 0 launches that many (at most until every scene's frame is done),
+// 0 runs until every frame in flight is done, < 0 until the first frame in flight is done. Scenes whose
+// frame ended get finished[i] = 1 and out[i] (as mpm_advance reports it).
+void batch_run(mpm_batch* b, int64_t max_launches, int64_t* launched, int32_t* finished, mpm_advance_stats* out) {
+  const int64_t n = (int64_t)b->scenes.size();
+  *launched = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    finished[i] = 0;
+    if (b->empty_frame[i]) {
+      b->empty_frame[i] = 0;
+      finished[i] = 1;
+      out[i] = mpm_advance_stats{0, b->frame_dt[i], 0};
+    }
+  }
+  int64_t most = 0, least = INT64_MAX;
+  for (int64_t i = 0; i < n; ++i)
+    if (in_frame(b->step[i])) {
+      const int64_t rem = b->step[i].steps - b->step[i].k;
+      most = std::max(most, rem);
+      least = std::min(least, rem);
+    }
+  if (most == 0) return;
+  const int64_t L = max_launches > 0 ? std::min(max_launches, most) : (max_launches == 0 ? most : least);
+  batch_reserve_tables(b, n);
+  batch_layout(b);
+  std::copy(b->rec.begin(), b->rec.end(), b->h_scenes);
+  std::copy(b->step.begin(), b->step.end(), b->h_steps);
   HIP_CHECK(hipMemcpyAsync(b->d_scenes, b->h_scenes, n * sizeof(SceneDev), hipMemcpyHostToDevice, b->stream));
   HIP_CHECK(hipMemcpyAsync(b->d_steps, b->h_steps, n * sizeof(SceneStep), hipMemcpyHostToDevice, b->stream));
-  if (max_steps > 0 && b->use_graph && !b->g_sort) {
+  if (b->use_graph && !b->g_sort) {
     HIP_CHECK(hipStreamSynchronize(b->stream));
     b->g_sort = batch_capture(b, true);
     b->g_plain = batch_capture(b, false);
   }
-  for (int64_t k = 0; k < max_steps; ++k) {
+  // the host replays the device's bookkeeping (k_batch_plan, k_batch_prefix, k_batch_advance) to pick the
+  // graph of each substep, and checks it against the device afterwards
+  std::vector<SceneStep> sim = b->step;
+  std::vector<char> was_running(n);
+  for (int64_t i = 0; i < n; ++i) was_running[i] = in_frame(sim[i]);
+  for (int64_t l = 0; l < L; ++l) {
     bool sorting = false;
     for (int64_t i = 0; i < n; ++i) {
-      if (k >= steps[i]) continue;
-      if (!sorted[i] || since[i] >= interval[i]) {
+      SceneStep& t = sim[i];
+      if (!in_frame(t)) continue;
+      if (!t.sorted || t.since_sort >= b->rec[i].sort_interval) {
         sorting = true;
-        sorted[i] = 1;
-        since[i] = 0;
+        t.cur = 1 - t.cur;
+        t.sorted = 1;
+        t.since_sort = 0;
       }
-      since[i] += 1;
+      t.since_sort += 1;
+      t.k += 1;
     }
     if (b->use_graph) HIP_CHECK(hipGraphLaunch(sorting ? b->g_sort : b->g_plain, b->stream));
     else batch_enqueue(b, sorting);
   }
-  b->substeps_launched += max_steps;
+  b->substeps_launched += L;
+  *launched = L;
   HIP_CHECK(hipMemcpyAsync(b->h_steps, b->d_steps, n * sizeof(SceneStep), hipMemcpyDeviceToHost, b->stream));
   HIP_CHECK(hipStreamSynchronize(b->stream));
   for (int64_t i = 0; i < n; ++i) {
-    mpm_ctx* c = b->scenes[i];
-    const SceneStep& st = b->h_steps[i];
-    if (st.k != (int32_t)steps[i] || st.sorted != sorted[i] || st.since_sort != since[i])
+    const SceneStep& d = b->h_steps[i];
+    const SceneStep& h = sim[i];
+    if (d.k != h.k || d.sorted != h.sorted || d.since_sort != h.since_sort || d.cur != h.cur)
       fail(MPM_ERR_RUNTIME, "batch bookkeeping diverged from the device");
-    if (steps[i] == 0) {
-      out[i] = mpm_advance_stats{0, frame_dt[i], 0};
-      continue;
-    }
-    if (st.cur) std::swap(c->A, c->B);
-    c->sorted = st.sorted != 0;
-    c->steps_since_sort = st.since_sort;
-    c->last_perm = const_cast<uint32_t*>((radix_passes(c) - 1) % 2 == 0 ? c->vals1 : c->vals2);
-    c->vmax_valid = false;  // the batch keeps the speed maximum, the context's partial maxima are stale
-    c->version += 1;
-    uint32_t bits = st.speed_bits;
-    float v;
-    std::memcpy(&v, &bits, 4);
-    b->speed[i] = v;
-    b->speed_ok[i] = 1;
-    int64_t removed = remove_escaped(c);
-    if (removed) refresh_present(c);
-    b->version[i] = c->version;
-    out[i] = mpm_advance_stats{steps[i], dts[i], removed};
+    b->step[i] = d;
   }
+  for (int64_t i = 0; i < n; ++i)
+    if (was_running[i] && !in_frame(b->step[i])) {
+      batch_finish_scene(b, i, &out[i]);
+      finished[i] = 1;
+    }
 }
 
+// one frame of every scene with frame_dt > 0, all run to completion (scenes already in a frame finish
+// theirs too)
+void batch_frame(mpm_batch* b, const double* frame_dt, mpm_advance_stats* out) {
+  const int64_t n = (int64_t)b->scenes.size();
+  if (n == 0) return;
+  batch_begin(b, frame_dt);
+  std::vector<int32_t> fin(n);
+  std::vector<mpm_advance_stats> st(n);
+  for (int64_t i = 0; i < n; ++i) out[i] = mpm_advance_stats{0, frame_dt[i], 0};
+  int64_t launched = 0;
+  batch_run(b, 0, &launched, fin.data(), st.data());
+  for (int64_t i = 0; i < n; ++i)
+    if (fin[i]) out[i] = st[i];
+}
+
+void attach_scene(mpm_batch* b, mpm_ctx* c) {
+  HIP_CHECK(hipStreamSynchronize(c->own_stream));
+  c->stream = b->stream;
+  c->batch = b;
+}
+
+void detach_scene(mpm_ctx* c) {
+  if (!c->batch) return;
+  (void)hipStreamSynchronize(c->stream);
+  c->stream = c->own_stream;
+  c->batch = nullptr;
+}
+
+// New membership; scenes that stay keep their place in a frame in flight; scenes that leave in the middle
+// of a frame keep the state they reached (a partial frame).
 void batch_set_scenes(mpm_batch* b, mpm_ctx* const* scenes, int32_t n) {
   for (int32_t i = 0; i < n; ++i) {
     mpm_ctx* c = scenes[i];
@@ -1076,32 +1169,53 @@ void batch_set_scenes(mpm_batch* b, mpm_ctx* const* scenes, int32_t n) {
       if (scenes[j] == c) fail(MPM_ERR_VALUE, "scene listed twice");
   }
   HIP_CHECK(hipStreamSynchronize(b->stream));
+  const size_t old_n = b->scenes.size();
+  std::vector<int64_t> from(n, -1);
+  std::vector<char> stays(old_n, 0);
+  for (int32_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < old_n; ++j)
+      if (b->scenes[j] == scenes[i]) {
+        from[i] = (int64_t)j;
+        stays[j] = 1;
+      }
+  for (size_t j = 0; j < old_n; ++j)
+    if (!stays[j]) {
+      batch_release_scene(b, (int64_t)j);
+      detach_scene(b->scenes[j]);
+    }
+  std::vector<SceneDev> rec(n);
+  std::vector<SceneStep> step(n);
   std::vector<float> speed(n, 0.0f);
   std::vector<uint64_t> version(n, 0);
-  std::vector<char> ok(n, 0);
-  for (int32_t i = 0; i < n; ++i)
-    for (size_t j = 0; j < b->scenes.size(); ++j)
-      if (b->scenes[j] == scenes[i]) {
-        speed[i] = b->speed[j];
-        version[i] = b->version[j];
-        ok[i] = b->speed_ok[j];
-      }
-  for (mpm_ctx* c : b->scenes) {
-    bool stays = false;
-    for (int32_t i = 0; i < n; ++i) stays = stays || scenes[i] == c;
-    if (!stays) detach_scene(c);
-  }
+  std::vector<char> ok(n, 0), empty(n, 0);
+  std::vector<double> fdt(n, 0.0), dt(n, 0.0);
   for (int32_t i = 0; i < n; ++i) {
     mpm_ctx* c = scenes[i];
-    if (c->batch == b) continue;
-    HIP_CHECK(hipStreamSynchronize(c->own_stream));
-    c->stream = b->stream;
-    c->batch = b;
+    if (from[i] >= 0) {
+      const int64_t j = from[i];
+      rec[i] = b->rec[j];
+      step[i] = b->step[j];
+      speed[i] = b->speed[j];
+      version[i] = b->version[j];
+      ok[i] = b->speed_ok[j];
+      empty[i] = b->empty_frame[j];
+      fdt[i] = b->frame_dt[j];
+      dt[i] = b->dt[j];
+    } else {
+      attach_scene(b, c);
+      rec[i] = scene_record(b, c);
+      step[i] = SceneStep{};
+    }
   }
   b->scenes.assign(scenes, scenes + n);
+  b->rec = std::move(rec);
+  b->step = std::move(step);
   b->speed = std::move(speed);
   b->version = std::move(version);
   b->speed_ok = std::move(ok);
+  b->empty_frame = std::move(empty);
+  b->frame_dt = std::move(fdt);
+  b->dt = std::move(dt);
   batch_drop_graphs(b);
   b->hist_tiles = -1;
 }
@@ -1111,10 +1225,16 @@ void batch_remove_scene(mpm_batch* b, mpm_ctx* c) {
   (void)hipStreamSynchronize(b->stream);
   for (size_t j = 0; j < b->scenes.size(); ++j)
     if (b->scenes[j] == c) {
+      c->busy = false;  // the context goes away; its partial frame does not matter
       b->scenes.erase(b->scenes.begin() + j);
+      b->rec.erase(b->rec.begin() + j);
+      b->step.erase(b->step.begin() + j);
       b->speed.erase(b->speed.begin() + j);
       b->version.erase(b->version.begin() + j);
       b->speed_ok.erase(b->speed_ok.begin() + j);
+      b->empty_frame.erase(b->empty_frame.begin() + j);
+      b->frame_dt.erase(b->frame_dt.begin() + j);
+      b->dt.erase(b->dt.begin() + j);
       break;
     }
   batch_drop_graphs(b);
@@ -1271,6 +1391,7 @@ int32_t mpm_add_particles(mpm_ctx* c, int64_t n, const float* x, const float* v,
                           const int32_t* emitter) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     if (n < 0) fail(MPM_ERR_VALUE, "negative particle count");
     if (n == 0) return;
     if (!x || !v || !mat || !emitter) fail(MPM_ERR_VALUE, "null particle array");
@@ -1294,7 +1415,8 @@ int32_t mpm_add_particles(mpm_ctx* c, int64_t n, const float* x, const float* v,
 int32_t mpm_load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const float* Cm, const float* F,
                        const float* Jp, const int32_t* mat, const int32_t* emitter) {
   if (!c) return MPM_ERR_VALUE;
-  return run(c, [&] { load_state(c, n, x, v, Cm, F, Jp, mat, emitter); });
+  return run(c, [&] {
+    require_idle(c); load_state(c, n, x, v, Cm, F, Jp, mat, emitter); });
 }
 
 int64_t mpm_particle_count(const mpm_ctx* c) { return c ? c->n : 0; }
@@ -1303,6 +1425,7 @@ int32_t mpm_get_state(mpm_ctx* c, float* x, float* v, float* Cm, float* F, float
                       int32_t* emitter) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     if (x) download_planes(c, c->A.x(), x, 3);
     if (v) download_planes(c, c->A.v(), v, 3);
     if (Cm) download_planes(c, c->A.C(), Cm, 9);
@@ -1320,6 +1443,7 @@ int32_t mpm_set_colliders(mpm_ctx* c, int32_t n, const int32_t* n_verts, const i
                           const int32_t* tris, const int32_t* closed, const float* friction) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     if (n < 0 || n > kMaxColliders) fail(MPM_ERR_VALUE, "the HIP backend takes at most 8 colliders per domain");
     if (n > 0 && (!n_verts || !n_tris || !closed || !friction)) fail(MPM_ERR_VALUE, "null collider array");
     std::vector<ColliderHost> hs(n);
@@ -1362,6 +1486,7 @@ int32_t mpm_set_colliders(mpm_ctx* c, int32_t n, const int32_t* n_verts, const i
 int32_t mpm_set_collider_poses(mpm_ctx* c, int32_t n, const mpm_collider_pose* poses) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     if (n != (int32_t)c->colliders.size())
       fail(MPM_ERR_VALUE, "expected " + std::to_string(c->colliders.size()) + " collider transforms");
     if (n > 0 && !poses) fail(MPM_ERR_VALUE, "null pose array");
@@ -1392,6 +1517,7 @@ int32_t mpm_set_collider_poses(mpm_ctx* c, int32_t n, const mpm_collider_pose* p
 int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
   if (!c || !out) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     if (c->n == 0) {
       *out = mpm_advance_stats{0, frame_dt, 0};
       return;
@@ -1416,6 +1542,7 @@ int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
 int32_t mpm_substep(mpm_ctx* c, double dt, double s, double frame_dt) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     c->frame_dt = frame_dt;
     substep(c, dt, s, frame_dt);
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1425,6 +1552,7 @@ int32_t mpm_substep(mpm_ctx* c, double dt, double s, double frame_dt) {
 int32_t mpm_is_occupied(mpm_ctx* c, int64_t m, const float* pts, uint8_t* out) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     if (m <= 0) return;
     if (!pts || !out) fail(MPM_ERR_VALUE, "null query array");
     const size_t vol_bytes = (size_t)c->n_nodes * sizeof(float);
@@ -1449,6 +1577,7 @@ int32_t mpm_is_occupied(mpm_ctx* c, int64_t m, const float* pts, uint8_t* out) {
 int32_t mpm_debug_p2g(mpm_ctx* c, double dt) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
     c->sorted = false;  // binned path: bin from the current positions, as a first substep does
     run_p2g(c, (float)dt);
@@ -1459,6 +1588,7 @@ int32_t mpm_debug_p2g(mpm_ctx* c, double dt) {
 int32_t mpm_debug_grid_update(mpm_ctx* c, double dt, double s, double frame_dt) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     c->frame_dt = frame_dt;
     run_grid(c, (float)dt, (float)s, (float)(1.0 / frame_dt));
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1468,6 +1598,7 @@ int32_t mpm_debug_grid_update(mpm_ctx* c, double dt, double s, double frame_dt) 
 int32_t mpm_debug_g2p(mpm_ctx* c, double dt, double s, double frame_dt) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     c->frame_dt = frame_dt;
     run_g2p(c, (float)dt, (float)s, (float)(1.0 / frame_dt));
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1477,6 +1608,7 @@ int32_t mpm_debug_g2p(mpm_ctx* c, double dt, double s, double frame_dt) {
 int32_t mpm_get_grid(mpm_ctx* c, int32_t which, float* grid_v, float* grid_m) {
   if (!c || !grid_v || !grid_m || (which != 0 && which != 1)) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     const int64_t total = (int64_t)c->geom.dims[0] * c->geom.dims[1] * c->geom.dims[2];
     float* stage = static_cast<float*>(c->scratch.get(total * 4 * sizeof(float)));
     k_grid_to_ref<<<grid_for(total), kBlock, 0, c->stream>>>(c->geom, which == 0 ? c->accum : c->vel,
@@ -1491,6 +1623,7 @@ int32_t mpm_get_grid(mpm_ctx* c, int32_t which, float* grid_v, float* grid_m) {
 int32_t mpm_set_grid(mpm_ctx* c, const float* grid_v, const float* grid_m) {
   if (!c || !grid_v || !grid_m) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     const int64_t total = (int64_t)c->geom.dims[0] * c->geom.dims[1] * c->geom.dims[2];
     float* stage = static_cast<float*>(c->scratch.get(total * 4 * sizeof(float)));
     HIP_CHECK(hipMemcpyAsync(stage, grid_v, total * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -1505,6 +1638,7 @@ int32_t mpm_set_grid(mpm_ctx* c, const float* grid_v, const float* grid_m) {
 int32_t mpm_get_binning(mpm_ctx* c, int32_t* keys, int32_t* perm, int32_t* offsets, int64_t* n_bins) {
   if (!c || !n_bins) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     *n_bins = c->n_cells;
     if (!c->last_perm) {
       if (keys || perm || offsets) fail(MPM_ERR_VALUE, "no binning has run yet");
@@ -1521,6 +1655,7 @@ int32_t mpm_get_binning(mpm_ctx* c, int32_t* keys, int32_t* perm, int32_t* offse
 int32_t mpm_debug_sort(mpm_ctx* c) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     sort_particles(c);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -1529,6 +1664,7 @@ int32_t mpm_debug_sort(mpm_ctx* c) {
 int32_t mpm_set_option(mpm_ctx* c, int32_t option, int64_t value) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
+    require_idle(c);
     switch (option) {
       case MPM_OPT_P2G_MODE:
         if (value != 0 && value != 1) fail(MPM_ERR_VALUE, "p2g mode must be 0 or 1");
@@ -1650,6 +1786,26 @@ int32_t mpm_batch_scene_count(const mpm_batch* b) { return b ? (int32_t)b->scene
 int32_t mpm_batch_advance(mpm_batch* b, const double* frame_dt, mpm_advance_stats* out) {
   if (!b || (!b->scenes.empty() && (!frame_dt || !out))) return MPM_ERR_VALUE;
   return run_on(b->device, [&] { batch_frame(b, frame_dt, out); });
+}
+
+int32_t mpm_batch_begin(mpm_batch* b, const double* frame_dt) {
+  if (!b || (!b->scenes.empty() && !frame_dt)) return MPM_ERR_VALUE;
+  return run_on(b->device, [&] { batch_begin(b, frame_dt); });
+}
+
+int32_t mpm_batch_run(mpm_batch* b, int64_t max_launches, int64_t* launched, int32_t* finished,
+                      mpm_advance_stats* out) {
+  if (!b || !launched || (!b->scenes.empty() && (!finished || !out))) return MPM_ERR_VALUE;
+  return run_on(b->device, [&] { batch_run(b, max_launches, launched, finished, out); });
+}
+
+int32_t mpm_batch_progress(const mpm_batch* b, int32_t* substeps_done, int32_t* substeps_total) {
+  if (!b || (!b->scenes.empty() && (!substeps_done || !substeps_total))) return MPM_ERR_VALUE;
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    substeps_done[i] = b->step[i].k;
+    substeps_total[i] = b->step[i].steps;
+  }
+  return MPM_OK;
 }
 
 int32_t mpm_batch_set_option(mpm_batch* b, int32_t option, int64_t value) {

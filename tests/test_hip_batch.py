@@ -187,3 +187,49 @@ def test_open_walls_remove_like_advance():
         removed_batch += batch.advance(w.frame_dt)[0].removed
     assert removed_plain > 0
     assert abs(removed_plain - removed_batch) <= max(2, 0.01 * removed_plain)
+
+
+def test_frames_start_and_end_independently():
+    """Continuous batching: every scene starts its next frame as soon as its last one ended, while the
+    others are in the middle of theirs; each scene's frames match Solver.advance."""
+    specs = [dict(model="elastic", n_colliders=1), dict(model="snow", n_colliders=0),
+             dict(model="von_mises", n_colliders=2)]
+    alone = [make(i, **kw) for i, kw in enumerate(specs)]
+    mixed = [make(i, **kw) for i, kw in enumerate(specs)]
+    frames = 3
+    counts_alone = [[] for _ in specs]
+    for i, (w, s) in enumerate(alone):
+        for f in range(frames):
+            set_poses(w, s, f)
+            counts_alone[i].append(s.advance(w.frame_dt).substeps)
+    batch = SceneBatch([s for _, s in mixed])
+    next_frame = [0] * len(specs)
+    counts_mixed = [[] for _ in specs]
+    fdt = np.zeros(len(specs))
+    for i, (w, s) in enumerate(mixed):
+        set_poses(w, s, 0)
+        fdt[i] = w.frame_dt
+    batch.begin(fdt)
+    with pytest.raises(ValueError):
+        mixed[0][1].save_state()  # in the middle of a batched frame
+    overlapped = False
+    while any(n < frames for n in next_frame):
+        done, total = batch.progress()
+        _, ended = batch.run(-1)
+        done2, total2 = batch.progress()
+        if len(ended) < sum(1 for i in range(len(specs)) if done[i] < total[i]):
+            overlapped = True
+        fdt = np.zeros(len(specs))
+        for i, st in ended.items():
+            counts_mixed[i].append(st.substeps)
+            next_frame[i] += 1
+            if next_frame[i] < frames:
+                w, s = mixed[i]
+                set_poses(w, s, next_frame[i])
+                fdt[i] = w.frame_dt
+        if fdt.any():
+            batch.begin(fdt)
+    assert overlapped, "some frames should have ended while others were in flight"
+    assert counts_mixed == counts_alone
+    for (w, a), (_, m) in zip(alone, mixed):
+        assert_aggregates_close(m.save_state(), a.save_state())
