@@ -13,6 +13,13 @@ A second measurement (skipped with --no-batch) runs the batched engine on the 25
 0-2 colliders each): the same frames advanced once one scene at a time (``Solver.advance`` per scene, in
 turn) and once through one ``SceneBatch``; both counts are sums of particles x substeps actually run.
 
+A third measurement (skipped with --no-serve) runs the bake server (mpm_core.serve.BakeServer) on
+requests arriving as a Poisson process at 20 per second, each drawn like the scene mix with a
+lognormal frame count (median 48) and 20-200 substeps per frame, admitted against a 250 GB HBM budget:
+once continuously batched and once as serial plain runs (one bake at a time, Solver.advance), with the
+same arrival times. Frames are downloaded to the host every frame as a bake streams them (no files are
+written in the measurement). Reported: per-frame latency p50/p99 and aggregate particle-substeps/s.
+
 The printed JSON line carries:
   roofline      for the kernel with the largest device time: algorithmic bytes per launch (bytes per
                 particle x particles + bytes per active grid node x active nodes) over its mean launch
@@ -59,6 +66,9 @@ def parse_args(argv=None):
     ap.add_argument("--no-batch", action="store_true", help="skip the batched scene-mix measurement")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-frames", type=int, default=2, help="timed frames of the scene mix")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-requests measurement")
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second")
     return ap.parse_args(argv)
 
 
@@ -165,6 +175,38 @@ def batch_bench(n_scenes: int, frames: int) -> dict:
     return result
 
 
+def serve_bench(n_requests: int, rate: float) -> dict:
+    """Poisson-arriving bake requests, continuously batched against serial plain runs. Throughput is
+    compared over the arrival window (first to last arrival), in which both servers always have work;
+    the batched server then drains its queue (latency over every frame), the serial one is stopped at the
+    end of the window (its backlog would take minutes)."""
+    from mpm_core.serve import BakeRequest, BakeServer
+    from mpm_plugin_amd.workloads import served_requests
+
+    reqs = served_requests(n_requests, rate)
+    window = reqs[-1][0]
+    out = {"requests": n_requests, "rate_per_s": rate, "hbm_budget_gb": 250, "window_s": round(window, 3),
+           "frames": int(sum(s.frame_end - s.frame_start + 1 for _, s in reqs))}
+    for name, kw, stop in (("batched", dict(batched=True), None),
+                           ("serial", dict(batched=False, max_active=1), window)):
+        server = BakeServer(hbm_budget=250e9, **kw)
+        t0 = time.perf_counter()
+        handles = server.run([(off, BakeRequest(scene, name=f"r{i}")) for i, (off, scene) in enumerate(reqs)],
+                             stop_after=stop)
+        rep = server.report(handles)
+        in_window = server.work_between(t0, t0 + window)
+        rep["window_particle_substeps_per_s"] = in_window / window
+        server.close()
+        rep["wall_s"] = round(time.perf_counter() - t0, 3)
+        out[name] = rep
+        log(f"served ({name}): {in_window / window:.3e} particle-substeps/s over the arrival window; frame "
+            f"latency p50 {rep['frame_latency_ms']['p50']:.1f} ms p99 {rep['frame_latency_ms']['p99']:.1f} ms; "
+            f"{rep['done']}/{n_requests} bakes done in {rep['wall_s']} s")
+    out["throughput_speedup"] = (out["batched"]["window_particle_substeps_per_s"]
+                                 / out["serial"]["window_particle_substeps_per_s"])
+    return out
+
+
 def main(argv=None) -> int:
     args = parse_args(argv)
     if args.gpus != 1:
@@ -227,9 +269,12 @@ def main(argv=None) -> int:
         log(f"cpu baseline {baseline['value']:.3e} particle-substeps/s")
 
     batched = None
+    solver.close()
     if not args.no_batch:
-        solver.close()
         batched = batch_bench(args.batch_scenes, args.batch_frames)
+    served = None
+    if not args.no_serve:
+        served = serve_bench(args.serve_requests, args.serve_rate)
 
     value = n * args.steps / elapsed
     line = {
@@ -252,6 +297,7 @@ def main(argv=None) -> int:
         "cpu_baseline": baseline,
         "kernels": kernels,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
     return 0

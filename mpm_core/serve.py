@@ -151,6 +151,8 @@ class BakeServer:
         self.t_last: float | None = None
         self.particle_substeps = 0
         self.launches = 0
+        self.run_seconds = 0.0  # host time spent inside batch runs (device work and its synchronisation)
+        self.work_log: list[tuple[float, int]] = []  # (time, particle-substeps so far) after each step
 
     # ---- requests -------------------------------------------------------------------------------------
     def submit(self, request: BakeRequest, arrival: float | None = None) -> BakeHandle:
@@ -174,13 +176,16 @@ class BakeServer:
         return not self.queue and not self.active
 
     # ---- the loop -------------------------------------------------------------------------------------
-    def run(self, arrivals: Sequence[tuple[float, BakeRequest]] = ()) -> list[BakeHandle]:
-        """Submit each request at its offset (seconds from now) and serve until everything has ended."""
+    def run(self, arrivals: Sequence[tuple[float, BakeRequest]] = (), stop_after: float | None = None) -> list[BakeHandle]:
+        """Submit each request at its offset (seconds from now) and serve until everything has ended, or
+        until `stop_after` seconds have passed (what is still queued or in flight is then left as it is)."""
         t0 = time.perf_counter()
         pending = deque(sorted(arrivals, key=lambda a: a[0]))
         handles = []
         while pending or not self.idle:
             now = time.perf_counter()
+            if stop_after is not None and now - t0 >= stop_after:
+                break
             while pending and t0 + pending[0][0] <= now:
                 off, req = pending.popleft()
                 handles.append(self.submit(req, arrival=t0 + off))
@@ -197,7 +202,9 @@ class BakeServer:
         self._admit()
         if self.batched:
             if any(b.in_piece for b in self.active):
+                t = time.perf_counter()
                 launched, ended = self.batch.run(self.chunk)
+                self.run_seconds += time.perf_counter() - t
                 self.launches += launched
                 for i in sorted(ended):
                     self._piece_ended(self.active[i], ended[i].substeps)
@@ -214,6 +221,7 @@ class BakeServer:
         self._retire()
         self._admit()
         self._begin_pieces()
+        self.work_log.append((time.perf_counter(), self.particle_substeps))
 
     def drain(self) -> None:
         """Wait for every file write in flight."""
@@ -449,6 +457,12 @@ class BakeServer:
             _release(b.solver)
 
     # ---- report ---------------------------------------------------------------------------------------
+    def work_between(self, t0: float, t1: float) -> int:
+        """Particle-substeps of the pieces that ended between t0 and t1."""
+        before = [w for t, w in self.work_log if t <= t0]
+        upto = [w for t, w in self.work_log if t <= t1]
+        return (upto[-1] if upto else 0) - (before[-1] if before else 0)
+
     def report(self, handles: Sequence[BakeHandle] | None = None) -> dict:
         """Latency percentiles over finished frames (first frames apart: they include queueing) and the
         aggregate particle-substeps/s from the first arrival to the last frame."""
@@ -472,6 +486,7 @@ class BakeServer:
             "seconds": span,
             "particle_substeps_per_s": self.particle_substeps / span if span > 0 else None,
             "substep_launches": self.launches,
+            "seconds_in_batch_runs": round(self.run_seconds, 3),
         }
 
 

@@ -155,6 +155,7 @@ struct mpm_ctx {
   // binning
   uint32_t *keys0 = nullptr, *keys1 = nullptr, *keys2 = nullptr, *vals1 = nullptr, *vals2 = nullptr;
   uint32_t *offsets = nullptr, *active = nullptr, *n_active = nullptr;  // offsets: per cell
+  uint32_t* dirty_list = nullptr;  // batched grid update work list
   Scratch hist, scan_sums;
   bool sorted = false;
   int64_t steps_since_sort = 0;
@@ -896,6 +897,10 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   check_launch("k_batch_prefix");
   k_batch_p2g<<<batch_blocks(b->p2g_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
   check_launch("k_batch_p2g");
+  k_batch_grid_scan<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+  check_launch("k_batch_grid_scan");
+  k_batch_dirty_prefix<<<1, kPlanThreads, 0, s>>>(b->d_steps, n, b->d_pre);
+  check_launch("k_batch_dirty_prefix");
   k_batch_grid<<<batch_blocks(b->grid_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n,
                                                                   b->d_frames);
   check_launch("k_batch_grid");
@@ -946,6 +951,7 @@ SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
   d.offsets = c->offsets;
   d.active = c->active;
   d.n_active = c->n_active;
+  d.dirty_list = c->dirty_list;
   d.n = (int32_t)c->n;
   d.n_bricks = c->n_bricks;
   d.n_cells = c->n_cells;
@@ -1342,6 +1348,7 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     c->live = dev_alloc<uint8_t>(bricks);
     c->offsets = dev_alloc<uint32_t>((size_t)c->n_cells + 1);
     c->active = dev_alloc<uint32_t>(bricks);
+    c->dirty_list = dev_alloc<uint32_t>(bricks);
     c->n_active = dev_alloc<uint32_t>(1);
     HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
     HIP_CHECK(hipMemsetAsync(c->vel, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
@@ -1370,7 +1377,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
-                  (void*)c->active, (void*)c->n_active, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
+                  (void*)c->active, (void*)c->dirty_list, (void*)c->n_active, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
                   (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris})
     dev_free(p);
   c->scratch.release();

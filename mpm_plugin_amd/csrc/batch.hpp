@@ -21,7 +21,7 @@
 //   k_batch_plan            per scene: runs?, frame fraction s, sort now?, work prefixes, collider frames
 //   [k_batch_cell_keys, radix passes, k_batch_cell_bounds, k_batch_reorder]   only in the sorting graph
 //   k_batch_prefix          switches scenes that sorted to their reordered set, occupied-brick prefix
-//   k_batch_p2g, k_batch_grid, k_batch_g2p, k_batch_advance
+//   k_batch_p2g, k_batch_grid_scan, k_batch_dirty_prefix, k_batch_grid, k_batch_g2p, k_batch_advance
 #pragma once
 
 #include "binning.hpp"
@@ -48,6 +48,7 @@ struct SceneDev {
   uint32_t* offsets;   // per cell
   uint32_t* active;    // occupied bricks
   uint32_t* n_active;  // their count
+  uint32_t* dirty_list;  // bricks P2G wrote this substep (grid update work list)
   int32_t n;           // particles
   int32_t n_bricks, n_cells, passes;
   int32_t sort_interval;
@@ -62,6 +63,7 @@ struct SceneStep {
   int32_t sorted;
   int32_t cur;          // live particle set
   uint32_t speed_bits;  // max |v| of the scene's last G2P, as float bits (NaN recorded as +inf)
+  uint32_t n_dirty;     // entries of dirty_list this substep
 };
 
 // work-unit prefixes, each n_scenes + 1 entries, in one array: pre + q * (n_scenes + 1)
@@ -72,7 +74,8 @@ enum PrefixKind {
   kPreRadixTiles = 3,  // radix tiles of sorting scenes (histogram segment = 256 * this prefix)
   kPreCellBlocks = 4,  // 256-cell blocks of sorting scenes (cell offsets)
   kPreBricks = 5,      // occupied bricks of running scenes (P2G)
-  kPrefixKinds = 6
+  kPreDirty = 6,       // bricks written by P2G in running scenes (grid update)
+  kPrefixKinds = 7
 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -124,6 +127,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_batch_plan(const SceneDev* __r
       T.sort_now = sort_now ? 1 : 0;
       if (run) {
         T.speed_bits = 0u;
+        T.n_dirty = 0u;
         count[kPreTiles] = ceil_div_u((uint32_t)S.n, kTile);
         count[kPreGroups] = ceil_div_u((uint32_t)S.n_bricks, kGroupBricks);
       }
@@ -393,34 +397,65 @@ __global__ void __launch_bounds__(256) k_batch_p2g(const SceneDev* __restrict__ 
   }
 }
 
-// grid update: one wave per group of 64 bricks; each lane reads one brick's dirty flag, then the wave
-// updates the dirty bricks one after another (grid_brick)
-__global__ void __launch_bounds__(256) k_batch_grid(const SceneDev* __restrict__ sd, const SceneStep* __restrict__ st,
-                                                    const uint32_t* __restrict__ pre, int n_scenes,
-                                                    const ColliderFrame* __restrict__ frames) {
+// Grid update, first pass: one wave per group of 64 bricks of a running scene; each lane reads one
+// brick's dirty flag, records whether the brick holds a fresh velocity, clears the flag, and the dirty
+// bricks are appended to the scene's work list (one atomic per wave).
+__global__ void __launch_bounds__(256) k_batch_grid_scan(const SceneDev* __restrict__ sd, SceneStep* __restrict__ st,
+                                                         const uint32_t* __restrict__ pre, int n_scenes) {
   const uint32_t* P0 = pre + kPreGroups * (n_scenes + 1);
   const uint32_t total = P0[n_scenes];
   const int wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int i = 0;
   for (uint32_t gw = blockIdx.x * (blockDim.x >> 6) + wave; gw < total; gw += gridDim.x * (blockDim.x >> 6)) {
     i = uniform(scene_of(P0, n_scenes, gw, i));
     const SceneDev& S = sd[i];
-    const int first = (int)(gw - P0[i]) * kGroupBricks;
-    const int brick = first + lane;
+    const int brick = (int)(gw - P0[i]) * kGroupBricks + lane;
     bool d = false;
     if (brick < S.n_bricks) {
       d = S.dirty[brick] != 0u;
       S.live[brick] = d ? 1 : 0;
       if (d) S.dirty[brick] = 0u;
     }
-    uint64_t mask = __ballot(d);
-    const float dt = st[i].dt;
-    const ColliderFrame* f = frames + i * kMaxColliders;
-    while (mask) {
-      const int b = first + __builtin_ctzll(mask);
-      mask &= mask - 1;
-      grid_brick(S.g, S.sc, dt, b, S.accum, S.vel, S.cs, f, lane);
-    }
+    const uint64_t mask = __ballot(d);
+    if (mask == 0) continue;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&st[i].n_dirty, (uint32_t)__popcll(mask));
+    base = __shfl(base, 0, 64);
+    if (d) S.dirty_list[base + (uint32_t)__popcll(mask & lt_mask)] = (uint32_t)brick;
+  }
+}
+
+// single workgroup: prefix of the running scenes' dirty-brick counts
+__global__ void __launch_bounds__(kPlanThreads) k_batch_dirty_prefix(const SceneStep* __restrict__ st, int n_scenes,
+                                                                     uint32_t* __restrict__ pre) {
+  __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
+  uint32_t* P0 = pre + kPreDirty * (n_scenes + 1);
+  uint32_t carry = 0;
+  for (int base = 0; base < n_scenes; base += kPlanThreads) {
+    const int i = base + threadIdx.x;
+    const uint32_t count = (i < n_scenes && st[i].run) ? st[i].n_dirty : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(count, s_warp, total);
+    if (i < n_scenes) P0[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) P0[n_scenes] = carry;
+}
+
+// Grid update, second pass: one wave per dirty brick of every running scene (grid_brick)
+__global__ void __launch_bounds__(256) k_batch_grid(const SceneDev* __restrict__ sd, const SceneStep* __restrict__ st,
+                                                    const uint32_t* __restrict__ pre, int n_scenes,
+                                                    const ColliderFrame* __restrict__ frames) {
+  const uint32_t* P0 = pre + kPreDirty * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  const int wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  int i = 0;
+  for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + wave; w < total; w += gridDim.x * (blockDim.x >> 6)) {
+    i = uniform(scene_of(P0, n_scenes, w, i));
+    const SceneDev& S = sd[i];
+    const int b = (int)S.dirty_list[w - P0[i]];
+    grid_brick(S.g, S.sc, st[i].dt, b, S.accum, S.vel, S.cs, frames + i * kMaxColliders, lane);
   }
 }
 

@@ -18,6 +18,10 @@
   the values above (elastic: the elastic block's); random velocity up to 3 m/s; 0-2 icosphere colliders
   (162 vertices) on straight paths; frames of 1/24 s. Bodies are jittered lattices (spacing dx/2, jitter
   +-0.25 of the spacing), generated directly rather than through mesh sampling for speed.
+* ``served_requests``: bake requests for the server, Poisson arrivals (20 per second by default), each a
+  scene description drawn like ``scene_mix`` (the body a fill emitter, sampled by the runner) with a
+  lognormal frame count (median 48) and a frame rate chosen so that the first frame takes a substep
+  count drawn uniformly from [20, 200].
 
 Particles are sampled with mpm_core.geometry.sample_mesh_volume, as the runner emits them.
 """
@@ -243,3 +247,48 @@ def mix_scene(index: int, seed: int = 7, n_range=(5_000, 400_000), edges=MIX_EDG
 def scene_mix(n_scenes: int = 256, seed: int = 7, **kw) -> list[Workload]:
     """The batched mix: ``n_scenes`` scenes of :func:`mix_scene` (about 25 M particles for 256)."""
     return [mix_scene(i, seed, **kw) for i in range(n_scenes)]
+
+
+def _wave_speed(m: MaterialParams) -> float:
+    mu, lam = m.lame()
+    return float(np.sqrt((lam + 2.0 * mu) / m.density))
+
+
+def served_request(index: int, seed: int = 11, n_range=(5_000, 400_000), edges=MIX_EDGES,
+                   median_frames: float = 48.0, max_frames: int = 150):
+    """Scene description of request `index` of the served workload (deterministic in (seed, index))."""
+    from mpm_core.scene import Collider, Emitter, SceneDescription
+
+    w = mix_scene(index, seed, n_range=n_range, edges=edges)
+    rng = np.random.default_rng([seed, index, 1])
+    frames = int(np.clip(round(median_frames * np.exp(rng.normal(0.0, 0.5))), 4, max_frames))
+    target = float(rng.uniform(20, 200))
+    speed = float(np.linalg.norm(w.velocities[0]))
+    cfl_dt = 0.4 * w.domain.dx / (_wave_speed(w.materials[0]) + speed)
+    fps = 1.0 / (target * cfl_dt)
+    # the body as a fill emitter mesh of the same extent as the lattice body
+    lo, hi = w.positions.min(0), w.positions.max(0)
+    centre, ext = (lo + hi) / 2, (hi - lo)
+    if rng.random() < 0.5:
+        verts, tris = icosphere(3)
+        xf = _translate(*centre) @ _scale(*(ext / 2))
+    else:
+        verts, tris = box_mesh()
+        xf = _translate(*(centre - ext / 2)) @ _scale(*ext)
+    emitter = Emitter(name="body", kind="fill", material=0, mesh=MeshData(verts, tris),
+                      transforms=np.repeat(xf[None], frames, 0), velocity=tuple(float(v) for v in w.velocities[0]))
+    colliders = []
+    for c, (geom, (a, b)) in enumerate(zip(w.colliders, w.collider_path)):
+        sc = w.collider_scale[c]
+        xfs = np.stack([_translate(*(a + (b - a) * (k / max(frames - 1, 1)))) @ _scale(sc) for k in range(frames)])
+        colliders.append(Collider(name=f"collider_{c}", mesh=geom.mesh, transforms=xfs, friction=geom.friction,
+                                  closed=geom.closed))
+    return SceneDescription(domain=w.domain, materials=w.materials, emitters=[emitter], colliders=colliders,
+                            gravity=w.gravity, fps=fps, frame_start=1, frame_end=frames, seed=seed + index)
+
+
+def served_requests(n_requests: int, rate: float = 20.0, seed: int = 11, **kw):
+    """[(arrival offset in seconds, scene description)] of a Poisson arrival process at `rate` per second."""
+    rng = np.random.default_rng([seed, 2])
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    return [(float(t[i]), served_request(i, seed, **kw)) for i in range(n_requests)]
