@@ -66,6 +66,7 @@ def parse_args(argv=None):
     ap.add_argument("--no-batch", action="store_true", help="skip the batched scene-mix measurement")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-frames", type=int, default=2, help="timed frames of the scene mix")
+    ap.add_argument("--no-tune", action="store_true", help="batched scene mix with default launch shapes")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-requests measurement")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second")
@@ -133,7 +134,7 @@ def run_frames(pairs, frames: int, first_frame: int, advance_all) -> tuple[float
     return time.perf_counter() - t0, work
 
 
-def batch_bench(n_scenes: int, frames: int) -> dict:
+def batch_bench(n_scenes: int, frames: int, tune: bool = True) -> dict:
     """The scene mix advanced one scene at a time and through one SceneBatch, same frames."""
     from mpm_core.backends.hip.batch import SceneBatch
     from mpm_plugin_amd.workloads import scene_mix
@@ -163,8 +164,17 @@ def batch_bench(n_scenes: int, frames: int) -> dict:
         return batch.advance([w.frame_dt for w, _ in ps])
 
     run_frames(pairs, 1, 0, batched)
+    if tune:
+        from mpm_plugin_amd.autotune import tune_batch
+
+        tuned = tune_batch(batch, [w.frame_dt for w, _ in pairs], use_cache=False)
+        result["tuned"] = tuned
+        log(f"autotuned launch shapes: {tuned['config']} ({tuned['tuning_seconds']:.1f} s)")
+        first = 2
+    else:
+        first = 1
     launched = batch.substeps_launched
-    secs, work = run_frames(pairs, frames, 1, batched)
+    secs, work = run_frames(pairs, frames, first, batched)
     result["batched"] = {"value": work / secs, "seconds": round(secs, 3), "particle_substeps": int(work),
                          "substep_launches": int(batch.substeps_launched - launched)}
     log(f"batched: {work / secs:.3e} particle-substeps/s ({secs:.2f} s)")
@@ -271,7 +281,7 @@ def main(argv=None) -> int:
     batched = None
     solver.close()
     if not args.no_batch:
-        batched = batch_bench(args.batch_scenes, args.batch_frames)
+        batched = batch_bench(args.batch_scenes, args.batch_frames, tune=not args.no_tune)
     served = None
     if not args.no_serve:
         served = serve_bench(args.serve_requests, args.serve_rate)

@@ -195,6 +195,7 @@ int32_t mpm_batch_progress(const mpm_batch* batch, int32_t* substeps_done, int32
 #define MPM_BATCH_OPT_SUBSTEPS_LAUNCHED 3 /* read only: substep launches since creation */
 #define MPM_BATCH_OPT_G2P_BLOCKS 4        /* workgroups of the persistent G2P launch (0: automatic) */
 #define MPM_BATCH_OPT_GRID_BLOCKS 5       /* workgroups of the persistent grid-update launch (0: automatic) */
+#define MPM_BATCH_OPT_G2P_TILE 6          /* particles per G2P workgroup: 64, 128 or 256 (default) */
 int32_t mpm_batch_set_option(mpm_batch* batch, int32_t option, int64_t value);
 int64_t mpm_batch_get_option(const mpm_batch* batch, int32_t option);
 

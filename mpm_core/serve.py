@@ -17,7 +17,8 @@ bakes in flight on one GPU instead:
   checkpoints, progress; writer threads do the file work.
 
 The server reports per-frame latency (time between a bake's consecutive finished frames; for the first
-frame, from arrival) at p50/p99 and the aggregate particle-substeps/s. With ``batched=False`` and
+frame, from admission) at p50/p99, the queueing delay (arrival to admission), and the aggregate
+particle-substeps/s. With ``batched=False`` and
 ``max_active=1`` it is the serial runner (one bake at a time, ``Solver.advance``), the baseline.
 """
 
@@ -294,8 +295,8 @@ class BakeServer:
         h.status = "running"
         h.admitted = time.perf_counter()
         b = _Bake(h, solver, writer, progress, hash_, footprint)
-        b.frame_t0 = h.arrival
-        b.last_output = h.arrival
+        b.frame_t0 = h.admitted
+        b.last_output = h.admitted
         try:
             for i in _active_emitters(scene, scene.frame_start):
                 _emit(scene, solver, i, scene.frame_start, Fraction(1), progress)
@@ -468,6 +469,7 @@ class BakeServer:
         aggregate particle-substeps/s from the first arrival to the last frame."""
         hs = list(handles) if handles is not None else list(self.finished)
         first = [h.frame_latency[0] for h in hs if h.frame_latency]
+        waits = [h.admitted - h.arrival for h in hs if h.admitted is not None]
         later = [x for h in hs for x in h.frame_latency[1:]]
         span = (self.t_last - self.t_first) if self.t_first is not None and self.t_last is not None else 0.0
 
@@ -482,6 +484,7 @@ class BakeServer:
             "frame_latency_ms": {"p50": pct(later, 50), "p99": pct(later, 99),
                                  "mean": float(np.mean(later)) * 1000.0 if later else None},
             "first_frame_ms": {"p50": pct(first, 50), "p99": pct(first, 99)},
+            "queue_wait_ms": {"p50": pct(waits, 50), "p99": pct(waits, 99)},
             "particle_substeps": int(self.particle_substeps),
             "seconds": span,
             "particle_substeps_per_s": self.particle_substeps / span if span > 0 else None,

@@ -18,7 +18,7 @@ K_P2G, K_GRID, K_G2P, K_BIN, K_COLLIDERS, K_OTHER = range(6)
 OPT_P2G_MODE, OPT_SORT_INTERVAL, OPT_P2G_BLOCKS = range(3)
 P2G_BINNED, P2G_ATOMIC = 0, 1
 (BATCH_OPT_P2G_BLOCKS, BATCH_OPT_GRAPH, BATCH_OPT_SORT_INTERVAL, BATCH_OPT_SUBSTEPS_LAUNCHED, BATCH_OPT_G2P_BLOCKS,
- BATCH_OPT_GRID_BLOCKS) = range(6)
+ BATCH_OPT_GRID_BLOCKS, BATCH_OPT_G2P_TILE) = range(7)
 KERNEL_NAMES = ("p2g", "grid", "g2p", "bin", "colliders", "other")
 
 

@@ -240,6 +240,7 @@ struct mpm_batch {
   int64_t p2g_blocks = 0;
   int64_t g2p_blocks = 0;
   int64_t grid_blocks = 0;
+  int64_t g2p_tile = 256;
   int64_t use_graph = 1;
   int64_t sort_interval = 0;  // 0: each context's own
   int64_t substeps_launched = 0;
@@ -875,7 +876,8 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   const int n = (int)b->scenes.size();
   hipStream_t s = b->stream;
   // persistent grids: a few workgroups per CU, striding over the running scenes' work
-  k_batch_plan<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, b->d_frames, n, sorting ? 1 : 0, b->d_pre);
+  k_batch_plan<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, b->d_frames, n, sorting ? 1 : 0,
+                                          (int)b->g2p_tile, b->d_pre);
   check_launch("k_batch_plan");
   if (sorting) {
     k_batch_cell_keys<<<2048, kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
@@ -904,8 +906,8 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   k_batch_grid<<<batch_blocks(b->grid_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n,
                                                                   b->d_frames);
   check_launch("k_batch_grid");
-  k_batch_g2p<<<batch_blocks(b->g2p_blocks, 2048), kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n,
-                                                                  b->d_frames);
+  k_batch_g2p<<<batch_blocks(b->g2p_blocks, 2048), (unsigned)b->g2p_tile, 0, s>>>(b->d_scenes, b->d_steps,
+                                                                                  b->d_pre, n, b->d_frames);
   check_launch("k_batch_g2p");
   k_batch_advance<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(b->d_steps, n);
   check_launch("k_batch_advance");
@@ -1818,27 +1820,36 @@ int32_t mpm_batch_progress(const mpm_batch* b, int32_t* substeps_done, int32_t* 
 int32_t mpm_batch_set_option(mpm_batch* b, int32_t option, int64_t value) {
   if (!b) return MPM_ERR_VALUE;
   return run_on(b->device, [&] {
+    int64_t* slot = nullptr;
     switch (option) {
       case MPM_BATCH_OPT_P2G_BLOCKS:
       case MPM_BATCH_OPT_G2P_BLOCKS:
       case MPM_BATCH_OPT_GRID_BLOCKS:
         if (value < 0 || value > (1 << 20)) fail(MPM_ERR_VALUE, "workgroup counts must be in [0, 2^20]");
-        (option == MPM_BATCH_OPT_P2G_BLOCKS ? b->p2g_blocks
-                                            : option == MPM_BATCH_OPT_G2P_BLOCKS ? b->g2p_blocks : b->grid_blocks) = value;
+        slot = option == MPM_BATCH_OPT_P2G_BLOCKS ? &b->p2g_blocks
+                                                  : option == MPM_BATCH_OPT_G2P_BLOCKS ? &b->g2p_blocks : &b->grid_blocks;
         break;
       case MPM_BATCH_OPT_GRAPH:
         if (value != 0 && value != 1) fail(MPM_ERR_VALUE, "graph option is 0 or 1");
-        b->use_graph = value;
+        slot = &b->use_graph;
         break;
       case MPM_BATCH_OPT_SORT_INTERVAL:
         if (value < 0) fail(MPM_ERR_VALUE, "sort interval must be >= 0");
-        b->sort_interval = value;
+        slot = &b->sort_interval;
+        break;
+      case MPM_BATCH_OPT_G2P_TILE:
+        if (value != 64 && value != 128 && value != 256) fail(MPM_ERR_VALUE, "G2P tile must be 64, 128 or 256");
+        slot = &b->g2p_tile;
         break;
       default:
         fail(MPM_ERR_VALUE, "unknown batch option");
     }
+    if (*slot == value) return;
     HIP_CHECK(hipStreamSynchronize(b->stream));
+    *slot = value;
     batch_drop_graphs(b);
+    // the sort cadence lives in the scene records
+    for (size_t i = 0; i < b->scenes.size(); ++i) b->rec[i].sort_interval = scene_record(b, b->scenes[i]).sort_interval;
   });
 }
 
@@ -1851,6 +1862,8 @@ int64_t mpm_batch_get_option(const mpm_batch* b, int32_t option) {
       return batch_blocks(b->g2p_blocks, 2048);
     case MPM_BATCH_OPT_GRID_BLOCKS:
       return batch_blocks(b->grid_blocks, 2048);
+    case MPM_BATCH_OPT_G2P_TILE:
+      return b->g2p_tile;
     case MPM_BATCH_OPT_GRAPH:
       return b->use_graph;
     case MPM_BATCH_OPT_SORT_INTERVAL:

@@ -68,7 +68,7 @@ struct SceneStep {
 
 // work-unit prefixes, each n_scenes + 1 entries, in one array: pre + q * (n_scenes + 1)
 enum PrefixKind {
-  kPreTiles = 0,       // particle tiles of running scenes (G2P)
+  kPreTiles = 0,       // particle tiles of running scenes (G2P; the tile is the G2P workgroup size)
   kPreGroups = 1,      // brick groups of running scenes (grid update)
   kPreSortTiles = 2,   // particle tiles of sorting scenes (keys, reorder)
   kPreRadixTiles = 3,  // radix tiles of sorting scenes (histogram segment = 256 * this prefix)
@@ -111,7 +111,7 @@ __device__ __forceinline__ const uint32_t* sorted_vals(const SceneDev& S) {
 __global__ void __launch_bounds__(kPlanThreads) k_batch_plan(const SceneDev* __restrict__ sd,
                                                              SceneStep* __restrict__ st,
                                                              ColliderFrame* __restrict__ frames, int n_scenes,
-                                                             int allow_sort, uint32_t* __restrict__ pre) {
+                                                             int allow_sort, int g2p_tile, uint32_t* __restrict__ pre) {
   __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
   const int stride = n_scenes + 1;
   uint32_t carry[5] = {0, 0, 0, 0, 0};
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_batch_plan(const SceneDev* __r
       if (run) {
         T.speed_bits = 0u;
         T.n_dirty = 0u;
-        count[kPreTiles] = ceil_div_u((uint32_t)S.n, kTile);
+        count[kPreTiles] = ceil_div_u((uint32_t)S.n, (uint32_t)g2p_tile);
         count[kPreGroups] = ceil_div_u((uint32_t)S.n_bricks, kGroupBricks);
       }
       if (sort_now) {
@@ -459,18 +459,20 @@ __global__ void __launch_bounds__(256) k_batch_grid(const SceneDev* __restrict__
   }
 }
 
-// G2P of every running scene (g2p_particle); each tile's speed maximum goes to the scene's record
-__global__ void __launch_bounds__(kTile) k_batch_g2p(const SceneDev* __restrict__ sd, SceneStep* __restrict__ st,
-                                                     const uint32_t* __restrict__ pre, int n_scenes,
-                                                     const ColliderFrame* __restrict__ frames) {
+// G2P of every running scene (g2p_particle), one workgroup per tile of blockDim.x particles (64, 128 or
+// 256: a launch-shape choice); each tile's speed maximum goes to the scene's record
+__global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ sd, SceneStep* __restrict__ st,
+                                                   const uint32_t* __restrict__ pre, int n_scenes,
+                                                   const ColliderFrame* __restrict__ frames) {
   const uint32_t* P0 = pre + kPreTiles * (n_scenes + 1);
   const uint32_t total = P0[n_scenes];
-  __shared__ float s_max[kTile / 64];
+  const int tile = (int)blockDim.x, waves = tile >> 6;
+  __shared__ float s_max[4];
   int i = 0;
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     i = uniform(scene_of(P0, n_scenes, u, i));
     const SceneDev& S = sd[i];
-    const int p = (int)(u - P0[i]) * kTile + threadIdx.x;
+    const int p = (int)(u - P0[i]) * tile + threadIdx.x;
     const int cur = st[i].cur;
     Particles P = S.set[cur];
     float speed = 0.0f;
@@ -478,12 +480,15 @@ __global__ void __launch_bounds__(kTile) k_batch_g2p(const SceneDev* __restrict_
       g2p_particle(P, S.mats, S.g, S.sc, st[i].dt, S.vel, S.tau[cur], p, S.cs, frames + i * kMaxColliders, speed);
     if (!(speed == speed)) speed = __int_as_float(0x7f800000);
     const float wm = wave_max(speed);
+    if (waves == 1) {
+      if (threadIdx.x == 0) atomicMax(&st[i].speed_bits, __float_as_uint(wm));
+      continue;
+    }
     if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
     __syncthreads();
     if (threadIdx.x == 0) {
       float b = s_max[0];
-#pragma unroll
-      for (int w = 1; w < kTile / 64; ++w) b = fmaxf(b, s_max[w]);
+      for (int w = 1; w < waves; ++w) b = fmaxf(b, s_max[w]);
       // non-negative floats order like their bit patterns
       atomicMax(&st[i].speed_bits, __float_as_uint(b));
     }

@@ -20,8 +20,10 @@
   +-0.25 of the spacing), generated directly rather than through mesh sampling for speed.
 * ``served_requests``: bake requests for the server, Poisson arrivals (20 per second by default), each a
   scene description drawn like ``scene_mix`` (the body a fill emitter, sampled by the runner) with a
-  lognormal frame count (median 48) and a frame rate chosen so that the first frame takes a substep
-  count drawn uniformly from [20, 200].
+  lognormal frame count (median 48, sigma 0.5, clamped to [1, 240]) and a frame rate chosen so that the
+  first frame takes a substep count drawn uniformly from [20, 200] at the scene's CFL step (rather than
+  fixing dt = (1/24)/k, which could undercut the CFL step; the runner's Solver.advance chooses substeps
+  by CFL either way).
 
 Particles are sampled with mpm_core.geometry.sample_mesh_volume, as the runner emits them.
 """
@@ -208,7 +210,7 @@ def lattice_body(shape: str, center, size: float, spacing: float, rng: np.random
     return g.astype(np.float32)
 
 
-def mix_scene(index: int, seed: int = 7, n_range=(5_000, 400_000), edges=MIX_EDGES, model: str | None = None,
+def mix_scene(index: int, seed: int = 3, n_range=(5_000, 400_000), edges=MIX_EDGES, model: str | None = None,
               n_colliders: int | None = None, walls: dict | None = None) -> Workload:
     """Scene `index` of the batched mix (deterministic in (seed, index)); `model`, `n_colliders` and
     `walls` pin what is otherwise drawn (all walls closed by default)."""
@@ -244,7 +246,7 @@ def mix_scene(index: int, seed: int = 7, n_range=(5_000, 400_000), edges=MIX_EDG
     return w
 
 
-def scene_mix(n_scenes: int = 256, seed: int = 7, **kw) -> list[Workload]:
+def scene_mix(n_scenes: int = 256, seed: int = 3, **kw) -> list[Workload]:
     """The batched mix: ``n_scenes`` scenes of :func:`mix_scene` (about 25 M particles for 256)."""
     return [mix_scene(i, seed, **kw) for i in range(n_scenes)]
 
@@ -254,14 +256,14 @@ def _wave_speed(m: MaterialParams) -> float:
     return float(np.sqrt((lam + 2.0 * mu) / m.density))
 
 
-def served_request(index: int, seed: int = 11, n_range=(5_000, 400_000), edges=MIX_EDGES,
-                   median_frames: float = 48.0, max_frames: int = 150):
+def served_request(index: int, seed: int = 4, n_range=(5_000, 400_000), edges=MIX_EDGES,
+                   median_frames: float = 48.0, max_frames: int = 240):
     """Scene description of request `index` of the served workload (deterministic in (seed, index))."""
     from mpm_core.scene import Collider, Emitter, SceneDescription
 
     w = mix_scene(index, seed, n_range=n_range, edges=edges)
     rng = np.random.default_rng([seed, index, 1])
-    frames = int(np.clip(round(median_frames * np.exp(rng.normal(0.0, 0.5))), 4, max_frames))
+    frames = int(np.clip(round(median_frames * np.exp(rng.normal(0.0, 0.5))), 1, max_frames))
     target = float(rng.uniform(20, 200))
     speed = float(np.linalg.norm(w.velocities[0]))
     cfl_dt = 0.4 * w.domain.dx / (_wave_speed(w.materials[0]) + speed)
@@ -287,7 +289,7 @@ def served_request(index: int, seed: int = 11, n_range=(5_000, 400_000), edges=M
                             gravity=w.gravity, fps=fps, frame_start=1, frame_end=frames, seed=seed + index)
 
 
-def served_requests(n_requests: int, rate: float = 20.0, seed: int = 11, **kw):
+def served_requests(n_requests: int, rate: float = 20.0, seed: int = 4, **kw):
     """[(arrival offset in seconds, scene description)] of a Poisson arrival process at `rate` per second."""
     rng = np.random.default_rng([seed, 2])
     t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
