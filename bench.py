@@ -63,6 +63,8 @@ def parse_args(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-collider", action="store_true")
+    ap.add_argument("--precision", default="fallback", choices=["fp32", "fallback", "fp64"],
+                    help="SVD/return-map precision of the single-scene run (library default: fallback)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched scene-mix measurement")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-frames", type=int, default=2, help="timed frames of the scene mix")
@@ -227,6 +229,10 @@ def main(argv=None) -> int:
     w = build_workload(args.workload, with_collider=not args.no_collider)
     log(f"workload {w.name}: {w.n} particles, grid {w.domain.grid_shape}, dt {w.dt}")
     solver = w.make_solver(device="cuda:0")
+    from mpm_plugin_amd import _native as N
+
+    solver.set_option(N.OPT_PRECISION, {"fp32": N.PRECISION_FP32, "fallback": N.PRECISION_FALLBACK,
+                                        "fp64": N.PRECISION_FP64}[args.precision])
     t_sim = 0.0
 
     def step():
@@ -302,7 +308,7 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": w.name, "particles": n, "grid": list(w.domain.grid_shape), "dt": w.dt,
                    "materials": [m.model for m in w.materials], "colliders": len(w.colliders),
-                   "substeps_per_step": 1},
+                   "substeps_per_step": 1, "svd_precision": args.precision},
         "roofline": roofline,
         "cpu_baseline": baseline,
         "kernels": kernels,

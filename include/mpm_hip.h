@@ -141,10 +141,18 @@ int32_t mpm_get_binning(mpm_ctx* ctx, int32_t* keys, int32_t* perm, int32_t* off
 int32_t mpm_debug_sort(mpm_ctx* ctx);
 
 /* ---- options ------------------------------------------------------------------------------ */
-#define MPM_OPT_P2G_MODE 0       /* 0: binned LDS-window P2G (default); 1: one thread per particle,
-                                    27 x 4 global atomics (the reference's data flow) */
+#define MPM_OPT_P2G_MODE 0       /* MPM_P2G_BINNED (default), MPM_P2G_ATOMIC or MPM_P2G_FIXED */
 #define MPM_OPT_SORT_INTERVAL 1  /* substeps between brick re-sorts (default 8; >= 1) */
-#define MPM_OPT_P2G_BLOCKS 2    /* workgroups of the binned P2G launch (0: automatic) */
+#define MPM_OPT_P2G_BLOCKS 2     /* workgroups of the binned P2G launch (0: automatic) */
+#define MPM_OPT_PRECISION 3      /* SVD, return maps and stress: MPM_PRECISION_FP32, _FALLBACK (default:
+                                    FP64 only where FP32 cannot resolve F: sigma_min/sigma_max < 1e-2 or
+                                    J = Jp det F < 0.25), _FP64 */
+#define MPM_P2G_BINNED 0         /* per-brick LDS window, global FP32 atomics for the flush */
+#define MPM_P2G_ATOMIC 1         /* one thread per particle, 27 x 4 FP32 global atomics (reference flow) */
+#define MPM_P2G_FIXED 2          /* as BINNED, accumulated in 64-bit fixed point: run-to-run bit-exact grids */
+#define MPM_PRECISION_FP32 0
+#define MPM_PRECISION_FALLBACK 1
+#define MPM_PRECISION_FP64 2
 int32_t mpm_set_option(mpm_ctx* ctx, int32_t option, int64_t value);
 int64_t mpm_get_option(const mpm_ctx* ctx, int32_t option);
 

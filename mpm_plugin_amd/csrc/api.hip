@@ -156,13 +156,17 @@ struct mpm_ctx {
   uint32_t *keys0 = nullptr, *keys1 = nullptr, *keys2 = nullptr, *vals1 = nullptr, *vals2 = nullptr;
   uint32_t *offsets = nullptr, *active = nullptr, *n_active = nullptr;  // offsets: per cell
   uint32_t* dirty_list = nullptr;  // batched grid update work list
+  uint32_t* fix_list = nullptr;    // particles deferred to the FP64 pass (cap entries)
+  uint32_t* n_fix = nullptr;       // their count
   Scratch hist, scan_sums;
   bool sorted = false;
   int64_t steps_since_sort = 0;
   uint32_t* last_perm = nullptr;  // the sorted values of the last sort (points into vals1/vals2)
 
   // options
-  int p2g_mode = 0;
+  int p2g_mode = 0;  // MPM_P2G_BINNED, MPM_P2G_ATOMIC or MPM_P2G_FIXED
+  unsigned long long* fx = nullptr;  // fixed-point accumulator (MPM_P2G_FIXED only)
+  double fx_scale = 0.0;
   int64_t sort_interval = 8;
   int64_t p2g_blocks = 0;
 
@@ -185,6 +189,9 @@ struct mpm_ctx {
   bool capturing = false;  // inside a graph capture: no events, no allocations, no host waits
 
   Particles parts() const { return A.view(); }
+  FixedAccum fixed() const {
+    return FixedAccum{p2g_mode == MPM_P2G_FIXED ? fx : nullptr, fx_scale, fx_scale > 0.0 ? 1.0 / fx_scale : 0.0};
+  }
   ColliderSet collider_set() const {
     ColliderSet cs{};
     cs.count = (int)colliders.size();
@@ -242,6 +249,7 @@ struct mpm_batch {
   int64_t grid_blocks = 0;
   int64_t g2p_tile = 256;
   int64_t use_graph = 1;
+  bool fp64_pass = false;  // some scene defers particles to the FP64 pass (part of the captured sequence)
   int64_t sort_interval = 0;  // 0: each context's own
   int64_t substeps_launched = 0;
 };
@@ -321,8 +329,9 @@ void free_particles(mpm_ctx* c) {
   dev_free(c->A.base);
   dev_free(c->B.base);
   for (void* p : {(void*)c->emitter, (void*)c->keys0, (void*)c->keys1, (void*)c->keys2, (void*)c->vals1,
-                  (void*)c->vals2})
+                  (void*)c->vals2, (void*)c->fix_list})
     dev_free(p);
+  c->fix_list = nullptr;
   c->A = ParticleSet{};
   c->B = ParticleSet{};
   c->emitter = nullptr;
@@ -346,6 +355,7 @@ void reserve(mpm_ctx* c, int64_t need) {
   uint32_t* k2 = dev_alloc<uint32_t>(cap);
   uint32_t* v1 = dev_alloc<uint32_t>(cap);
   uint32_t* v2 = dev_alloc<uint32_t>(cap);
+  uint32_t* fl = dev_alloc<uint32_t>(cap);
   if (c->n) {
     for (int k = 0; k < kFloatPlanes + kIntPlanes; ++k)
       HIP_CHECK(hipMemcpyAsync(A.base + (size_t)k * cap * 4, c->A.base + (size_t)k * c->cap * 4, c->n * 4,
@@ -363,6 +373,7 @@ void reserve(mpm_ctx* c, int64_t need) {
   c->keys2 = k2;
   c->vals1 = v1;
   c->vals2 = v2;
+  c->fix_list = fl;
   c->cap = cap;
   c->n = n;
 }
@@ -389,6 +400,13 @@ void download_planes(mpm_ctx* c, const float* planes, float* host, int comps) {
 void check_materials(mpm_ctx* c, const int32_t* mat, int64_t n) {
   for (int64_t i = 0; i < n; ++i)
     if (mat[i] < 0 || mat[i] >= (int32_t)c->mats.size()) fail(MPM_ERR_VALUE, "material id out of range");
+}
+
+// empty P2G accumulators (float and fixed point) and no pending bricks, as between two substeps
+void clear_grid(mpm_ctx* c) {
+  HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
+  HIP_CHECK(hipMemsetAsync(c->dirty, 0, (size_t)c->n_bricks * 4, c->stream));
+  if (c->fx) HIP_CHECK(hipMemsetAsync(c->fx, 0, (size_t)c->n_nodes * 4 * sizeof(unsigned long long), c->stream));
 }
 
 void state_changed(mpm_ctx* c) {
@@ -457,7 +475,7 @@ void sort_particles(mpm_ctx* c) {
 
 // ---- substep phases --------------------------------------------------------------------------
 void run_p2g(mpm_ctx* c, float dt) {
-  if (c->p2g_mode == 1) {
+  if (c->p2g_mode == MPM_P2G_ATOMIC) {
     if (c->n == 0) return;
     Timed t(c, MPM_K_P2G);
     k_p2g_atomic<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt,
@@ -472,20 +490,21 @@ void run_p2g(mpm_ctx* c, float dt) {
   int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 2048;
   blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, (std::min<int64_t>(c->n_bricks, c->n) + 3) / 4));
   k_p2g_cells<<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->A.tau(), c->d_mats, c->geom, c->sc.p_vol, dt,
-                                                       c->offsets, c->active, c->n_active, c->accum, c->dirty);
+                                                       c->offsets, c->active, c->n_active, c->accum, c->dirty,
+                                                       c->fixed());
   check_launch("k_p2g_cells");
 }
 
 void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   Timed t(c, MPM_K_GRID);
-  if (c->p2g_mode == 1) {
+  if (c->p2g_mode == MPM_P2G_ATOMIC) {
     k_grid_dense<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->sc, dt, c->n_nodes, c->accum, c->vel,
                                                                   c->live, c->collider_set(), s, inv_frame_dt);
     check_launch("k_grid_dense");
   } else {
     const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
     k_grid_bricks<<<blocks, 256, 0, c->stream>>>(c->geom, c->sc, dt, c->n_bricks, c->dirty, c->live, c->accum,
-                                                 c->vel, c->collider_set(), s, inv_frame_dt);
+                                                 c->vel, c->collider_set(), s, inv_frame_dt, c->fixed());
     check_launch("k_grid_bricks");
   }
 }
@@ -493,9 +512,21 @@ void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
 void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   if (c->n == 0) return;
   Timed t(c, MPM_K_G2P);
+  const bool fix = c->sc.precision != kPrecisionFp32;
+  if (fix) {
+    k_zero_u32<<<1, 64, 0, c->stream>>>(c->n_fix, 1);
+    check_launch("k_zero_u32");
+  }
   k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel,
-                                                  c->A.tau(), block_max(c), c->collider_set(), s, inv_frame_dt);
+                                                  c->A.tau(), block_max(c), c->collider_set(), s, inv_frame_dt,
+                                                  FixList{c->fix_list, c->n_fix});
   check_launch("k_g2p");
+  if (fix) {
+    // a persistent grid: the deferred particles are usually few
+    k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 1024), kBlock, 0, c->stream>>>(
+        c->parts(), c->d_mats, c->fix_list, c->n_fix, c->A.tau());
+    check_launch("k_fp64_fixup");
+  }
   c->block_max_count = grid_for(c->n);
   c->vmax_valid = true;
 }
@@ -540,6 +571,7 @@ void load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const flo
   if (n > 0 && (!x || !v || !Cm || !F || !Jp || !mat || !emitter)) fail(MPM_ERR_VALUE, "null state array");
   check_materials(c, mat, n);
   if (n > (int64_t)INT32_MAX) fail(MPM_ERR_VALUE, "too many particles");
+  clear_grid(c);  // a grid left by a debug phase hook belongs to the old state
   c->n = 0;
   reserve(c, n);
   upload_planes(c, c->A.x(), x, 3, n, 0);
@@ -552,7 +584,7 @@ void load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const flo
     HIP_CHECK(hipMemcpyAsync(c->emitter, emitter, n * 4, hipMemcpyHostToDevice, c->stream));
     k_iota<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.pid(), (int)n);
     check_launch("k_iota");
-    k_stress<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, (int)n, c->A.tau());
+    k_stress<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, (int)n, c->sc.precision, c->A.tau());
     check_launch("k_stress");
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
@@ -909,6 +941,12 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   k_batch_g2p<<<batch_blocks(b->g2p_blocks, 2048), (unsigned)b->g2p_tile, 0, s>>>(b->d_scenes, b->d_steps,
                                                                                   b->d_pre, n, b->d_frames);
   check_launch("k_batch_g2p");
+  if (b->fp64_pass) {
+    k_batch_fix_prefix<<<1, kPlanThreads, 0, s>>>(b->d_steps, n, b->d_pre);
+    check_launch("k_batch_fix_prefix");
+    k_batch_fixup<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+    check_launch("k_batch_fixup");
+  }
   k_batch_advance<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(b->d_steps, n);
   check_launch("k_batch_advance");
 }
@@ -954,6 +992,7 @@ SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
   d.active = c->active;
   d.n_active = c->n_active;
   d.dirty_list = c->dirty_list;
+  d.fix_list = c->fix_list;
   d.n = (int32_t)c->n;
   d.n_bricks = c->n_bricks;
   d.n_cells = c->n_cells;
@@ -974,7 +1013,7 @@ void batch_begin(mpm_batch* b, const double* frame_dt) {
     if (!(frame_dt[i] > 0.0)) continue;
     mpm_ctx* c = b->scenes[i];
     if (in_frame(b->step[i]) || b->empty_frame[i]) fail(MPM_ERR_VALUE, "a scene of the batch is still in a frame");
-    if (c->p2g_mode != 0) fail(MPM_ERR_VALUE, "batched scenes use the binned P2G (option P2G_MODE 0)");
+    if (c->p2g_mode != MPM_P2G_BINNED) fail(MPM_ERR_VALUE, "batched scenes use the binned P2G (option P2G_MODE 0)");
     starting.push_back(i);
   }
   if (starting.empty()) return;
@@ -1088,6 +1127,13 @@ void batch_run(mpm_batch* b, int64_t max_launches, int64_t* launched, int32_t* f
   const int64_t L = max_launches > 0 ? std::min(max_launches, most) : (max_launches == 0 ? most : least);
   batch_reserve_tables(b, n);
   batch_layout(b);
+  bool fp64_pass = false;
+  for (const SceneDev& r : b->rec) fp64_pass = fp64_pass || r.sc.precision != kPrecisionFp32;
+  if (fp64_pass != b->fp64_pass) {
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    batch_drop_graphs(b);
+    b->fp64_pass = fp64_pass;
+  }
   std::copy(b->rec.begin(), b->rec.end(), b->h_scenes);
   std::copy(b->step.begin(), b->step.end(), b->h_steps);
   HIP_CHECK(hipMemcpyAsync(b->d_scenes, b->h_scenes, n * sizeof(SceneDev), hipMemcpyHostToDevice, b->stream));
@@ -1337,6 +1383,7 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
       s.clamp_hi[a] = domain->clamp_hi[a];
     }
     s.wall_friction = domain->wall_friction;
+    s.precision = kPrecisionFallback;
     const int64_t bricks = (int64_t)g.bdims[0] * g.bdims[1] * g.bdims[2];
     if (bricks * 64 > ((int64_t)1 << 30)) fail(MPM_ERR_VALUE, "grid too large");
     c->n_bricks = (int)bricks;
@@ -1352,6 +1399,7 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     c->active = dev_alloc<uint32_t>(bricks);
     c->dirty_list = dev_alloc<uint32_t>(bricks);
     c->n_active = dev_alloc<uint32_t>(1);
+    c->n_fix = dev_alloc<uint32_t>(1);
     HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
     HIP_CHECK(hipMemsetAsync(c->vel, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
     HIP_CHECK(hipMemsetAsync(c->dirty, 0, bricks * 4, c->stream));
@@ -1379,7 +1427,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
-                  (void*)c->active, (void*)c->dirty_list, (void*)c->n_active, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
+                  (void*)c->active, (void*)c->dirty_list, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
                   (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris})
     dev_free(p);
   c->scratch.release();
@@ -1587,7 +1635,7 @@ int32_t mpm_debug_p2g(mpm_ctx* c, double dt) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
     require_idle(c);
-    HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
+    clear_grid(c);
     c->sorted = false;  // binned path: bin from the current positions, as a first substep does
     run_p2g(c, (float)dt);
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1619,8 +1667,16 @@ int32_t mpm_get_grid(mpm_ctx* c, int32_t which, float* grid_v, float* grid_m) {
   return run(c, [&] {
     require_idle(c);
     const int64_t total = (int64_t)c->geom.dims[0] * c->geom.dims[1] * c->geom.dims[2];
-    float* stage = static_cast<float*>(c->scratch.get(total * 4 * sizeof(float)));
-    k_grid_to_ref<<<grid_for(total), kBlock, 0, c->stream>>>(c->geom, which == 0 ? c->accum : c->vel,
+    float4* src = which == 0 ? c->accum : c->vel;
+    if (which == 0 && c->p2g_mode == MPM_P2G_FIXED) {
+      src = static_cast<float4*>(c->scratch.get((size_t)c->n_nodes * sizeof(float4) + total * 4 * sizeof(float)));
+      k_fixed_to_float<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->fx, c->n_nodes, 1.0 / c->fx_scale, src);
+      check_launch("k_fixed_to_float");
+    }
+    float* stage = which == 0 && c->p2g_mode == MPM_P2G_FIXED
+                       ? reinterpret_cast<float*>(src + c->n_nodes)
+                       : static_cast<float*>(c->scratch.get(total * 4 * sizeof(float)));
+    k_grid_to_ref<<<grid_for(total), kBlock, 0, c->stream>>>(c->geom, src,
                                                              which == 0 ? nullptr : c->live, stage, stage + 3 * total);
     check_launch("k_grid_to_ref");
     HIP_CHECK(hipMemcpyAsync(grid_v, stage, total * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -1676,15 +1732,43 @@ int32_t mpm_set_option(mpm_ctx* c, int32_t option, int64_t value) {
     require_idle(c);
     switch (option) {
       case MPM_OPT_P2G_MODE:
-        if (value != 0 && value != 1) fail(MPM_ERR_VALUE, "p2g mode must be 0 or 1");
+        if (value != MPM_P2G_BINNED && value != MPM_P2G_ATOMIC && value != MPM_P2G_FIXED)
+          fail(MPM_ERR_VALUE, "p2g mode must be 0, 1 or 2");
         if (value != c->p2g_mode) {
           // the dense path leaves no dirty flags; hand over a clean accumulator either way
           HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
           HIP_CHECK(hipMemsetAsync(c->dirty, 0, (size_t)c->n_bricks * 4, c->stream));
+          if (value == MPM_P2G_FIXED && !c->fx) {
+            c->fx = dev_alloc<unsigned long long>((size_t)c->n_nodes * 4);
+            HIP_CHECK(hipMemsetAsync(c->fx, 0, (size_t)c->n_nodes * 4 * sizeof(unsigned long long), c->stream));
+            // one particle's mass maps to 2^32: contributions keep ~1e-10 of it, sums of 2^31 particle
+            // masses at speeds up to ~1e3 still fit in int64
+            float pm = 0.0f;
+            for (const auto& m : c->mats) pm = std::max(pm, m.p_mass);
+            c->fx_scale = std::ldexp(1.0, 32) / (double)(pm > 0.0f ? pm : 1.0f);
+          }
+          if (value != MPM_P2G_FIXED && c->fx) {
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            dev_free(c->fx);
+            c->fx = nullptr;
+          }
           HIP_CHECK(hipStreamSynchronize(c->stream));
         }
         c->p2g_mode = (int)value;
         c->sorted = false;
+        break;
+      case MPM_OPT_PRECISION:
+        if (value < kPrecisionFp32 || value > kPrecisionFp64) fail(MPM_ERR_VALUE, "precision must be 0, 1 or 2");
+        if (value != c->sc.precision) {
+          c->sc.precision = (int)value;
+          // the stored stress belongs to the old precision
+          if (c->n) {
+            k_stress<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, (int)c->n, c->sc.precision,
+                                                              c->A.tau());
+            check_launch("k_stress");
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+          }
+        }
         break;
       case MPM_OPT_SORT_INTERVAL:
         if (value < 1) fail(MPM_ERR_VALUE, "sort interval must be >= 1");
@@ -1704,6 +1788,7 @@ int64_t mpm_get_option(const mpm_ctx* c, int32_t option) {
   if (!c) return -1;
   switch (option) {
     case MPM_OPT_P2G_MODE: return c->p2g_mode;
+    case MPM_OPT_PRECISION: return c->sc.precision;
     case MPM_OPT_SORT_INTERVAL: return c->sort_interval;
     case MPM_OPT_P2G_BLOCKS: return c->p2g_blocks;
     default: return -1;

@@ -21,7 +21,9 @@
 //   k_batch_plan            per scene: runs?, frame fraction s, sort now?, work prefixes, collider frames
 //   [k_batch_cell_keys, radix passes, k_batch_cell_bounds, k_batch_reorder]   only in the sorting graph
 //   k_batch_prefix          switches scenes that sorted to their reordered set, occupied-brick prefix
-//   k_batch_p2g, k_batch_grid_scan, k_batch_dirty_prefix, k_batch_grid, k_batch_g2p, k_batch_advance
+//   k_batch_p2g, k_batch_grid_scan, k_batch_dirty_prefix, k_batch_grid, k_batch_g2p,
+//   [k_batch_fix_prefix, k_batch_fixup]   when a scene defers particles to the FP64 pass
+//   k_batch_advance
 #pragma once
 
 #include "binning.hpp"
@@ -49,6 +51,7 @@ struct SceneDev {
   uint32_t* active;    // occupied bricks
   uint32_t* n_active;  // their count
   uint32_t* dirty_list;  // bricks P2G wrote this substep (grid update work list)
+  uint32_t* fix_list;    // particles whose material update is redone in FP64 this substep
   int32_t n;           // particles
   int32_t n_bricks, n_cells, passes;
   int32_t sort_interval;
@@ -64,6 +67,7 @@ struct SceneStep {
   int32_t cur;          // live particle set
   uint32_t speed_bits;  // max |v| of the scene's last G2P, as float bits (NaN recorded as +inf)
   uint32_t n_dirty;     // entries of dirty_list this substep
+  uint32_t n_fix;       // entries of fix_list this substep
 };
 
 // work-unit prefixes, each n_scenes + 1 entries, in one array: pre + q * (n_scenes + 1)
@@ -75,7 +79,8 @@ enum PrefixKind {
   kPreCellBlocks = 4,  // 256-cell blocks of sorting scenes (cell offsets)
   kPreBricks = 5,      // occupied bricks of running scenes (P2G)
   kPreDirty = 6,       // bricks written by P2G in running scenes (grid update)
-  kPrefixKinds = 7
+  kPreFix = 7,         // particles deferred to the FP64 pass in running scenes
+  kPrefixKinds = 8
 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -128,6 +133,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_batch_plan(const SceneDev* __r
       if (run) {
         T.speed_bits = 0u;
         T.n_dirty = 0u;
+        T.n_fix = 0u;
         count[kPreTiles] = ceil_div_u((uint32_t)S.n, (uint32_t)g2p_tile);
         count[kPreGroups] = ceil_div_u((uint32_t)S.n_bricks, kGroupBricks);
       }
@@ -393,7 +399,7 @@ __global__ void __launch_bounds__(256) k_batch_p2g(const SceneDev* __restrict__ 
     const int cur = st[i].cur;
     const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
     p2g_brick(S.set[cur], S.tau[cur], S.mats, S.g, k, S.offsets, S.active[w - P0[i]], s_win[wave], S.accum,
-              S.dirty, lane);
+              S.dirty, lane, FixedAccum{nullptr, 0.0, 0.0});
   }
 }
 
@@ -455,7 +461,8 @@ __global__ void __launch_bounds__(256) k_batch_grid(const SceneDev* __restrict__
     i = uniform(scene_of(P0, n_scenes, w, i));
     const SceneDev& S = sd[i];
     const int b = (int)S.dirty_list[w - P0[i]];
-    grid_brick(S.g, S.sc, st[i].dt, b, S.accum, S.vel, S.cs, frames + i * kMaxColliders, lane);
+    grid_brick(S.g, S.sc, st[i].dt, b, S.accum, S.vel, S.cs, frames + i * kMaxColliders, lane,
+               FixedAccum{nullptr, 0.0, 0.0});
   }
 }
 
@@ -476,8 +483,11 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ 
     const int cur = st[i].cur;
     Particles P = S.set[cur];
     float speed = 0.0f;
+    bool defer = false;
     if (p < S.n)
-      g2p_particle(P, S.mats, S.g, S.sc, st[i].dt, S.vel, S.tau[cur], p, S.cs, frames + i * kMaxColliders, speed);
+      defer = g2p_particle(P, S.mats, S.g, S.sc, st[i].dt, S.vel, S.tau[cur], p, S.cs, frames + i * kMaxColliders,
+                           speed);
+    if (S.sc.precision != kPrecisionFp32) wave_append(FixList{S.fix_list, &st[i].n_fix}, defer, (uint32_t)p);
     if (!(speed == speed)) speed = __int_as_float(0x7f800000);
     const float wm = wave_max(speed);
     if (waves == 1) {
@@ -493,6 +503,39 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ 
       atomicMax(&st[i].speed_bits, __float_as_uint(b));
     }
     __syncthreads();
+  }
+}
+
+// single workgroup: prefix of the running scenes' deferred-particle counts
+__global__ void __launch_bounds__(kPlanThreads) k_batch_fix_prefix(const SceneStep* __restrict__ st, int n_scenes,
+                                                                   uint32_t* __restrict__ pre) {
+  __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
+  uint32_t* P0 = pre + kPreFix * (n_scenes + 1);
+  uint32_t carry = 0;
+  for (int base = 0; base < n_scenes; base += kPlanThreads) {
+    const int i = base + threadIdx.x;
+    const uint32_t count = (i < n_scenes && st[i].run) ? st[i].n_fix : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(count, s_warp, total);
+    if (i < n_scenes) P0[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) P0[n_scenes] = carry;
+}
+
+// the FP64 pass over every running scene's deferred particles (k_fp64_fixup)
+__global__ void __launch_bounds__(256) k_batch_fixup(const SceneDev* __restrict__ sd, const SceneStep* __restrict__ st,
+                                                     const uint32_t* __restrict__ pre, int n_scenes) {
+  const uint32_t* P0 = pre + kPreFix * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  int i = 0;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
+    i = scene_of(P0, n_scenes, u, i);
+    const SceneDev& S = sd[i];
+    const int cur = st[i].cur;
+    Particles P = S.set[cur];
+    const int64_t p = S.fix_list[u - P0[i]];
+    material_update_fp64(S.mats[P.mat[p]], P, S.tau[cur], p);
   }
 }
 

@@ -49,7 +49,46 @@ struct StepConsts {
   float wall_lo[3], wall_hi[3];
   float clamp_lo[3], clamp_hi[3];
   float wall_friction;
+  int precision;  // kPrecisionFp32, kPrecisionFallback or kPrecisionFp64 (SVD and return maps)
 };
+
+// Where the 3x3 SVD, the return maps and the stress run in FP64: never, always (a reference-quality mode
+// for error measurements), or as a fallback for the particles FP32 cannot resolve:
+//   ill-conditioned F, sigma_min / sigma_max < 1e-2: F^T F squares the ratio, and below ~1e-3 the small
+//     singular value sinks under the rounding of the large one;
+//   collapsed volume, J = Jp det(F) < 0.25 (compressed snow; Jp is 1 for the other models), where the
+//     hardened stiffness multiplies the rounding of the clamped singular values.
+// Near-degenerate singular values alone do not qualify: U is then ill-determined, but the stress
+// U diag(d) U^T is not, because the principal stresses d are as close as the singular values (measured:
+// singular values equal to 1e-6 give the same FP32 error as FP64, 3e-7 on the grid momentum).
+enum : int { kPrecisionFp32 = 0, kPrecisionFallback = 1, kPrecisionFp64 = 2 };
+
+__device__ __forceinline__ bool needs_fp64(int precision, const float (&sig)[3], float jp) {
+  if (precision == kPrecisionFp64) return true;
+  if (precision != kPrecisionFallback) return false;
+  const float a0 = fabsf(sig[0]), a1 = fabsf(sig[1]), a2 = fabsf(sig[2]);
+  return a2 < 1.0e-2f * a0 || jp * a0 * a1 * a2 < 0.25f;
+}
+
+// Work list of particles whose material update is redone in FP64 after the FP32 pass. Appends are
+// aggregated per wave (every lane of the wave must call, flag or not).
+struct FixList {
+  uint32_t* list;
+  uint32_t* count;
+};
+
+__device__ __forceinline__ void wave_append(const FixList& fl, bool flag, uint32_t value) {
+  const uint64_t mask = __ballot(flag);
+  if (mask == 0) return;
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0;
+  if (lane == __builtin_ctzll(mask)) base = atomicAdd(fl.count, (uint32_t)__popcll(mask));
+  base = __shfl(base, __builtin_ctzll(mask), 64);
+  if (flag) {
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    fl.list[base + (uint32_t)__popcll(mask & lt)] = value;
+  }
+}
 
 __global__ void k_zero_u32(uint32_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -106,43 +145,34 @@ __device__ __forceinline__ uint32_t cell_key(const GridGeom& g, const int (&base
 
 // Kirchhoff stress of one particle (kernels.py:115-124): Hencky for von Mises and Drucker-Prager,
 // fixed corotated for snow (with hardening h = min(exp(xi (1 - Jp)), h_max)) and elastic.
-__device__ __forceinline__ M3<float> particle_tau(const MatDev& m, const M3<float>& F, float Jp) {
-  M3<float> U, V;
-  float sig[3];
-  svd3(F, U, sig, V);
-  if (m.model == kVonMises || m.model == kDruckerPrager) return hencky_tau(U, sig, m.mu, m.lam);
-  float mu = m.mu, lam = m.lam;
-  if (m.model == kSnow) {
-    const float h = fminf(expf(m.hardening * (1.0f - Jp)), m.max_hardening);
-    mu *= h;
-    lam *= h;
-  }
-  return fixed_corotated_tau(F, U, sig, V, mu, lam);
-}
 
 // Kirchhoff stress from an SVD F = U diag(sig) V^T, as the symmetric U diag(d) U^T (6 entries
 // xx yy zz xy xz yz). Fixed corotated: (F - R) F^T = U (diag(sig) - I) diag(sig) U^T, so
 // d_i = 2 mu (sig_i - 1) sig_i + lam J (J - 1); Hencky: d_i = 2 mu eps_i + lam tr(eps).
-__device__ __forceinline__ void stress_from_svd(const MatDev& m, const M3<float>& U, const float (&sig)[3], float Jp,
-                                                float (&tau)[6]) {
-  float d[3];
+// Kirchhoff stress from the SVD of F as 6 symmetric entries (xx, yy, zz, xy, xz, yz): U diag(d) U^T with
+// d the Hencky (kernels.py:41-60) or fixed-corotated (kernels.py:30-38, snow hardening :120-122) principal
+// stress; for fixed-corotated (F - R) F^T = U (S - I) S U^T.
+template <class T>
+__device__ __forceinline__ void stress_from_svd(const MatDev& m, const M3<T>& U, const T (&sig)[3], T Jp,
+                                                T (&tau)[6]) {
+  T d[3];
   if (m.model == kVonMises || m.model == kDruckerPrager) {
-    const float e0 = safe_log(sig[0]), e1 = safe_log(sig[1]), e2 = safe_log(sig[2]);
-    const float tr = e0 + e1 + e2;
-    d[0] = 2.0f * m.mu * e0 + m.lam * tr;
-    d[1] = 2.0f * m.mu * e1 + m.lam * tr;
-    d[2] = 2.0f * m.mu * e2 + m.lam * tr;
+    const T e0 = safe_log(sig[0]), e1 = safe_log(sig[1]), e2 = safe_log(sig[2]);
+    const T tr = e0 + e1 + e2;
+    d[0] = T(2) * T(m.mu) * e0 + T(m.lam) * tr;
+    d[1] = T(2) * T(m.mu) * e1 + T(m.lam) * tr;
+    d[2] = T(2) * T(m.mu) * e2 + T(m.lam) * tr;
   } else {
-    float mu = m.mu, lam = m.lam;
+    T mu = T(m.mu), lam = T(m.lam);
     if (m.model == kSnow) {
-      const float h = fminf(expf(m.hardening * (1.0f - Jp)), m.max_hardening);
+      const T h = min_t(exp_t(T(m.hardening) * (T(1) - Jp)), T(m.max_hardening));
       mu *= h;
       lam *= h;
     }
-    const float J = sig[0] * sig[1] * sig[2];
-    const float vol = lam * J * (J - 1.0f);
+    const T J = sig[0] * sig[1] * sig[2];
+    const T vol = lam * J * (J - T(1));
 #pragma unroll
-    for (int a = 0; a < 3; ++a) d[a] = 2.0f * mu * (sig[a] - 1.0f) * sig[a] + vol;
+    for (int a = 0; a < 3; ++a) d[a] = T(2) * mu * (sig[a] - T(1)) * sig[a] + vol;
   }
   const auto& u = U.a;
   tau[0] = u[0][0] * d[0] * u[0][0] + u[0][1] * d[1] * u[0][1] + u[0][2] * d[2] * u[0][2];
@@ -153,22 +183,41 @@ __device__ __forceinline__ void stress_from_svd(const MatDev& m, const M3<float>
   tau[5] = u[1][0] * d[0] * u[2][0] + u[1][1] * d[1] * u[2][1] + u[1][2] * d[2] * u[2][2];
 }
 
+// stress of a particle state in the requested precision (6 entries, FP32 out)
+__device__ __forceinline__ void particle_stress(const MatDev& m, const M3<float>& F, float Jp, int precision,
+                                                float (&t)[6]) {
+  M3<float> U, V;
+  float sig[3];
+  svd3(F, U, sig, V);
+  if (needs_fp64(precision, sig, m.model == kSnow ? Jp : 1.0f)) {
+    M3<double> Fd, Ud, Vd;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Fd.a[e / 3][e % 3] = (double)F.a[e / 3][e % 3];
+    double sd[3], td[6];
+    svd3(Fd, Ud, sd, Vd);
+    stress_from_svd(m, Ud, sd, (double)Jp, td);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) t[e] = (float)td[e];
+    return;
+  }
+  stress_from_svd(m, U, sig, Jp, t);
+}
+
 __device__ __forceinline__ void store_tau(float* tau, int64_t cap, int64_t p, const float (&t)[6]) {
 #pragma unroll
   for (int e = 0; e < 6; ++e) tau[e * cap + p] = t[e];
 }
 
 // stress of every particle from its F (and Jp): after load_state, when tau is not yet known
-__global__ void __launch_bounds__(256) k_stress(Particles P, const MatDev* __restrict__ mats, int n,
+__global__ void __launch_bounds__(256) k_stress(Particles P, const MatDev* __restrict__ mats, int n, int precision,
                                                 float* __restrict__ tau) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const MatDev m = mats[P.mat[p]];
-  M3<float> F, U, V;
+  M3<float> F;
   load_m3(P.F, P.cap, p, F);
-  float sig[3], t[6];
-  svd3(F, U, sig, V);
-  stress_from_svd(m, U, sig, P.Jp[p], t);
+  float t[6];
+  particle_stress(m, F, P.Jp[p], precision, t);
   store_tau(tau, P.cap, p, t);
 }
 
@@ -183,7 +232,13 @@ __global__ void __launch_bounds__(256) k_p2g_atomic(Particles P, const MatDev* _
   M3<float> F, Cm;
   load_m3(P.F, cap, p, F);
   load_m3(P.C, cap, p, Cm);
-  const M3<float> tau = particle_tau(m, F, P.Jp[p]);
+  float t6[6];
+  particle_stress(m, F, P.Jp[p], sc.precision, t6);
+  M3<float> tau;
+  tau.a[0][0] = t6[0]; tau.a[1][1] = t6[1]; tau.a[2][2] = t6[2];
+  tau.a[0][1] = tau.a[1][0] = t6[3];
+  tau.a[0][2] = tau.a[2][0] = t6[4];
+  tau.a[1][2] = tau.a[2][1] = t6[5];
   const float pm = m.p_mass;
   const float k = -dt * sc.p_vol * 4.0f * g.inv_dx * g.inv_dx;
   M3<float> aff;
@@ -298,6 +353,16 @@ __global__ void k_grid_to_ref(GridGeom g, const float4* __restrict__ src, const 
   v3[3 * (int64_t)idx + 1] = a.y;
   v3[3 * (int64_t)idx + 2] = a.z;
   m[idx] = a.w;
+}
+
+// fixed-point accumulator (4 int64 per node) to float4 (momentum, mass) for the grid download
+__global__ void k_fixed_to_float(const unsigned long long* __restrict__ fx, int n_nodes, double inv_scale,
+                                 float4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const unsigned long long* f = fx + 4 * (int64_t)i;
+  out[i] = make_float4((float)((double)(long long)f[0] * inv_scale), (float)((double)(long long)f[1] * inv_scale),
+                       (float)((double)(long long)f[2] * inv_scale), (float)((double)(long long)f[3] * inv_scale));
 }
 
 __global__ void k_grid_from_ref(GridGeom g, const float* __restrict__ v3, const float* __restrict__ m,

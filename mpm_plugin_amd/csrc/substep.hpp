@@ -22,6 +22,19 @@ namespace mpm {
 constexpr int kWin = 8;
 constexpr int kWinNodes = kWin * kWin * kWin;
 
+// Optional fixed-point P2G accumulator: 4 int64 per node (momentum xyz, mass) scaled by `scale`. Every
+// contribution reaching it is rounded to an integer first and integer addition is associative, so the
+// grid is bit-identical from run to run whatever order the atomics land in (the per-brick FP32 partial
+// sums before it are formed in a fixed order). fx == nullptr selects the FP32 accumulator.
+struct FixedAccum {
+  unsigned long long* fx;
+  double scale, inv_scale;
+};
+
+__device__ __forceinline__ void fixed_add(const FixedAccum& fa, int node, int c, float v) {
+  atomicAdd(&fa.fx[4 * (int64_t)node + c], (unsigned long long)__double2ll_rn((double)v * fa.scale));
+}
+
 __device__ __forceinline__ void mark_brick(const GridGeom& g, uint32_t* dirty, int bx, int by, int bz) {
   if (bx < 0 || by < 0 || bz < 0 || bx >= g.bdims[0] || by >= g.bdims[1] || bz >= g.bdims[2]) return;
   dirty[(bx * g.bdims[1] + by) * g.bdims[2] + bz] = 1u;
@@ -73,9 +86,11 @@ __device__ __forceinline__ void load_p2g_particle(const Particles& P, const floa
 // atomics into the wave's window when the stencil fits the window, else straight to HBM.
 __device__ __forceinline__ void p2g_scatter_atomic(const P2GParticle& q, const GridGeom& g, const int (&wo)[3],
                                                 float (*win)[kWinNodes], float4* __restrict__ accum,
-                                                uint32_t* __restrict__ dirty) {
+                                                uint32_t* __restrict__ dirty, const FixedAccum& fa) {
   const int l0 = q.base[0] - wo[0], l1 = q.base[1] - wo[1], l2 = q.base[2] - wo[2];
-  const bool in_window = l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 <= kWin - 3 && l1 <= kWin - 3 && l2 <= kWin - 3;
+  // fixed point: straight to the integer grid (the window's float atomics would not add in a fixed order)
+  const bool in_window = fa.fx == nullptr && l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 <= kWin - 3 &&
+                         l1 <= kWin - 3 && l2 <= kWin - 3;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       const float wij = q.wx[0][i] * q.wx[1][j];
@@ -92,6 +107,12 @@ __device__ __forceinline__ void p2g_scatter_atomic(const P2GParticle& q, const G
           atomicAdd(&win[1][li], vy);
           atomicAdd(&win[2][li], vz);
           atomicAdd(&win[3][li], vm);
+        } else if (fa.fx) {
+          const int node = node_addr(g, q.base[0] + i, q.base[1] + j, q.base[2] + kk);
+          fixed_add(fa, node, 0, vx);
+          fixed_add(fa, node, 1, vy);
+          fixed_add(fa, node, 2, vz);
+          fixed_add(fa, node, 3, vm);
         } else {
           float4* node = accum + node_addr(g, q.base[0] + i, q.base[1] + j, q.base[2] + kk);
           atomicAdd(&node->x, vx);
@@ -124,7 +145,8 @@ __device__ __forceinline__ void wave_lds_fence() {
 __device__ __forceinline__ void p2g_brick(const Particles& P, const float* __restrict__ tau,
                                           const MatDev* __restrict__ mats, const GridGeom& g, float k,
                                           const uint32_t* __restrict__ cell_off, uint32_t b, float (*win)[kWinNodes],
-                                          float4* __restrict__ accum, uint32_t* __restrict__ dirty, int lane) {
+                                          float4* __restrict__ accum, uint32_t* __restrict__ dirty, int lane,
+                                          const FixedAccum& fa) {
   const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
   const int bz = (int)(b % (uint32_t)g.bdims[2]);
   const int by = (int)((b / (uint32_t)g.bdims[2]) % (uint32_t)g.bdims[1]);
@@ -171,7 +193,7 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const float* __res
         }
       }
     } else {
-      p2g_scatter_atomic(q, g, wo, win, accum, dirty);
+      p2g_scatter_atomic(q, g, wo, win, accum, dirty, fa);
     }
   }
   wave_lds_fence();
@@ -198,6 +220,14 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const float* __res
     if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
     const float vx = win[0][i], vy = win[1][i], vz = win[2][i], vm = win[3][i];
     if (vx == 0.0f && vy == 0.0f && vz == 0.0f && vm == 0.0f) continue;
+    if (fa.fx) {
+      const int nd = node_addr(g, gi, gj, gk);
+      if (vx != 0.0f) fixed_add(fa, nd, 0, vx);
+      if (vy != 0.0f) fixed_add(fa, nd, 1, vy);
+      if (vz != 0.0f) fixed_add(fa, nd, 2, vz);
+      if (vm != 0.0f) fixed_add(fa, nd, 3, vm);
+      continue;
+    }
     float4* node = accum + node_addr(g, gi, gj, gk);
     if (vx != 0.0f) atomicAdd(&node->x, vx);
     if (vy != 0.0f) atomicAdd(&node->y, vy);
@@ -215,25 +245,35 @@ __global__ void __launch_bounds__(256) k_p2g_cells(Particles P, const float* __r
                                                    const uint32_t* __restrict__ cell_off,
                                                    const uint32_t* __restrict__ active,
                                                    const uint32_t* __restrict__ n_active_ptr,
-                                                   float4* __restrict__ accum, uint32_t* __restrict__ dirty) {
+                                                   float4* __restrict__ accum, uint32_t* __restrict__ dirty,
+                                                   FixedAccum fa) {
   constexpr int kWaves = 4;
   __shared__ float s_win[kWaves][4][kWinNodes];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t n_active = *n_active_ptr;
   const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
   for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves)
-    p2g_brick(P, tau, mats, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane);
+    p2g_brick(P, tau, mats, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane, fa);
 }
 
 // grid update of one dirty brick by one wave (lane = node): momentum -> velocity, gravity, walls,
 // collider contact; clears the accumulator behind itself
 __device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& sc, float dt, int brick,
                                            float4* __restrict__ accum, float4* __restrict__ vel,
-                                           const ColliderSet& cs, const ColliderFrame* frames, int lane) {
+                                           const ColliderSet& cs, const ColliderFrame* frames, int lane,
+                                           const FixedAccum& fa) {
   const int addr = brick * 64 + lane;
   int i, j, k;
   node_ijk(g, addr, i, j, k);
-  const float4 a = accum[addr];
+  float4 a;
+  if (fa.fx) {
+    unsigned long long* f = fa.fx + 4 * (int64_t)addr;
+    a = make_float4((float)((double)(long long)f[0] * fa.inv_scale), (float)((double)(long long)f[1] * fa.inv_scale),
+                    (float)((double)(long long)f[2] * fa.inv_scale), (float)((double)(long long)f[3] * fa.inv_scale));
+    f[0] = f[1] = f[2] = f[3] = 0ull;
+  } else {
+    a = accum[addr];
+  }
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (i < g.dims[0] && j < g.dims[1] && k < g.dims[2]) {
     out = update_node(g, sc, dt, i, j, k, a);
@@ -247,7 +287,7 @@ __device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& 
 __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, float dt, int n_bricks,
                                                      uint32_t* __restrict__ dirty, uint8_t* __restrict__ live,
                                                      float4* __restrict__ accum, float4* __restrict__ vel,
-                                                     ColliderSet cs, float s, float inv_frame_dt) {
+                                                     ColliderSet cs, float s, float inv_frame_dt, FixedAccum fa) {
   __shared__ ColliderFrame frames[kMaxColliders];
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -256,7 +296,7 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
   const bool d = dirty[brick] != 0u;
   if (lane == 0) live[brick] = d ? 1 : 0;
   if (!d) return;
-  grid_brick(g, sc, dt, brick, accum, vel, cs, frames, lane);
+  grid_brick(g, sc, dt, brick, accum, vel, cs, frames, lane, fa);
   if (lane == 0) dirty[brick] = 0u;
 }
 
@@ -293,7 +333,65 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-__device__ __forceinline__ void g2p_particle(Particles& P, const MatDev* __restrict__ mats, const GridGeom& g,
+// Per-model return map of the new deformation gradient Fn = U diag(sig) V^T (kernels.py:242-263, 63-89)
+// and the stress of the result, in precision T; F, Jp and tau are stored in FP32.
+template <class T>
+__device__ __forceinline__ void material_update(const MatDev& m, const M3<T>& Fn, const M3<T>& U, const T (&sig)[3],
+                                                const M3<T>& V, Particles& P, float* __restrict__ tau_out,
+                                                int64_t p) {
+  const int64_t cap = P.cap;
+  T t[6];
+  if (m.model == kElastic) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) P.F[e * cap + p] = (float)Fn.a[e / 3][e % 3];
+    stress_from_svd(m, U, sig, T(1), t);
+  } else {
+    T ns[3];
+    T jp = T(1);
+    if (m.model == kVonMises) {
+      project_von_mises(sig, T(m.mu), T(m.yield_stress), ns);
+    } else if (m.model == kDruckerPrager) {
+      project_drucker_prager(sig, T(m.mu), T(m.lam), T(m.alpha), ns);
+    } else {  // snow (Stomakhin et al. 2013)
+      const T lo = T(1) - T(m.crit_c), hi = T(1) + T(m.crit_s);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) ns[a] = min_t(max_t(sig[a], lo), hi);
+      jp = T(P.Jp[p]) * (sig[0] * sig[1] * sig[2]) / (ns[0] * ns[1] * ns[2]);
+      P.Jp[p] = (float)jp;
+    }
+    const M3<T> Fe = usv(U, ns, V);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) P.F[e * cap + p] = (float)Fe.a[e / 3][e % 3];
+    stress_from_svd(m, U, ns, jp, t);
+  }
+#pragma unroll
+  for (int e = 0; e < 6; ++e) tau_out[e * cap + p] = (float)t[e];
+}
+
+// The FP64 pass of material_update for one particle whose FP32 pass was deferred: F holds the updated
+// (not yet projected) deformation gradient; SVD, return map and stress in double precision.
+__device__ __forceinline__ void material_update_fp64(const MatDev& m, Particles& P, float* __restrict__ tau_out,
+                                                     int64_t p) {
+  M3<double> Fd, Ud, Vd;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Fd.a[e / 3][e % 3] = (double)P.F[e * P.cap + p];
+  double sd[3];
+  svd3(Fd, Ud, sd, Vd);
+  material_update(m, Fd, Ud, sd, Vd, P, tau_out, p);
+}
+
+// FP64 pass over the deferred particles (grid-stride over the work list)
+__global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* __restrict__ mats,
+                                                    const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count, float* __restrict__ tau) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int64_t p = list[i];
+    material_update_fp64(mats[P.mat[p]], P, tau, p);
+  }
+}
+
+__device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restrict__ mats, const GridGeom& g,
                                              const StepConsts& sc, float dt, const float4* __restrict__ vel,
                                              float* __restrict__ tau_out, int64_t p, const ColliderSet& cs,
                                              const ColliderFrame* frames, float& speed) {
@@ -365,30 +463,15 @@ __device__ __forceinline__ void g2p_particle(Particles& P, const MatDev* __restr
   const M3<float> Fn = mul(G, Fo);
   const MatDev m = mats[P.mat[p]];
   M3<float> U, V;
-  float sig[3], t[6];
+  float sig[3];
   svd3(Fn, U, sig, V);
-  if (m.model == kElastic) {
+  if (needs_fp64(sc.precision, sig, m.model == kSnow ? P.Jp[p] : 1.0f)) {
+    // deferred to the FP64 pass (k_fp64_fixup), which reads Fn back from F
     store_m3(P.F, cap, p, Fn);
-    stress_from_svd(m, U, sig, 1.0f, t);
-    store_tau(tau_out, cap, p, t);
-    return;
+    return true;
   }
-  float ns[3];
-  float jp = 1.0f;
-  if (m.model == kVonMises) {
-    project_von_mises(sig, m.mu, m.yield_stress, ns);
-  } else if (m.model == kDruckerPrager) {
-    project_drucker_prager(sig, m.mu, m.lam, m.alpha, ns);
-  } else {  // snow (Stomakhin et al. 2013)
-    const float lo = 1.0f - m.crit_c, hi = 1.0f + m.crit_s;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) ns[a] = fminf(fmaxf(sig[a], lo), hi);
-    jp = P.Jp[p] * (sig[0] * sig[1] * sig[2]) / (ns[0] * ns[1] * ns[2]);
-    P.Jp[p] = jp;
-  }
-  store_m3(P.F, cap, p, usv(U, ns, V));
-  stress_from_svd(m, U, ns, jp, t);
-  store_tau(tau_out, cap, p, t);
+  material_update(m, Fn, U, sig, V, P, tau_out, p);
+  return false;
 }
 
 // largest speed of the workgroup into block_max[blockIdx.x]; NaN (which fmaxf would drop) counts as +inf,
@@ -409,12 +492,14 @@ __device__ __forceinline__ void block_max_speed(float speed, float* __restrict__
 __global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restrict__ mats, GridGeom g,
                                              StepConsts sc, int n, float dt, const float4* __restrict__ vel,
                                              float* __restrict__ tau, float* __restrict__ block_max,
-                                             ColliderSet cs, float s, float inv_frame_dt) {
+                                             ColliderSet cs, float s, float inv_frame_dt, FixList fix) {
   __shared__ ColliderFrame frames[kMaxColliders];
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float speed = 0.0f;
-  if (p < n) g2p_particle(P, mats, g, sc, dt, vel, tau, p, cs, frames, speed);
+  bool defer = false;
+  if (p < n) defer = g2p_particle(P, mats, g, sc, dt, vel, tau, p, cs, frames, speed);
+  if (sc.precision != kPrecisionFp32) wave_append(fix, defer, (uint32_t)p);
   block_max_speed(speed, block_max);
 }
 

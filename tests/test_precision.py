@@ -1,0 +1,186 @@
+"""Precision modes against the FP64 oracle, and the fixed-point P2G.
+
+Two states stress FP32 (128^3 grid):
+  A  snow whose plastic part has collapsed the volume: Jp in [0.2, 0.3], elastic singular values spread
+     over the clamp range [1 - theta_c, 1 + theta_s], so J = Jp det(Fe) is about 0.2 (the descending-plane
+     compression reached through load_state rather than simulated, so that the FP64 oracle, which has no
+     FP64 collider path, can follow every substep);
+  B  elastic F = R diag(s, s(1 + d), s(1 - d)) Q^T with s ~ U(0.9, 1.1), d ~ U(-1e-6, 1e-6), R, Q random
+     rotations: singular values equal to 1e-6.
+For each substep k of an FP64 oracle trajectory, the GPU is loaded with the oracle's state k and runs one
+substep in each mode; the error is ||X - X64|| / ||X64|| for the grid momentum after P2G and for the
+particle F after the substep. Modes: FP32 (every SVD and return map in FP32), the FP64 fallback (FP64 only
+for sigma_min/sigma_max < 1e-2 or J = Jp det F < 0.25), FP64 (every SVD and return map), and the
+fixed-point P2G (with the fallback). Set MPM_PRECISION_REPORT=path to write the errors as JSON and MPM_PRECISION_N to
+change the particle count per state (default 60,000; 500,000 for the full-size report).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mpm_core.materials import MaterialParams
+from mpm_core.scene import Domain
+from mpm_plugin_amd import _native as N
+from mpm_plugin_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+RES = 128
+N_STATE = int(os.environ.get("MPM_PRECISION_N", "60000"))
+SUBSTEPS = 3
+DT = 5.0e-5
+SNOW = MaterialParams("snow", "snow", 1.4e5, 0.2, 400.0, hardening=10.0, critical_compression=2.5e-2,
+                      critical_stretch=7.5e-3, max_hardening=5.0)
+ELASTIC = MaterialParams("elastic", "elastic", 1.0e4, 0.3, 1000.0)
+MODES = {"fp32": (N.PRECISION_FP32, N.P2G_BINNED), "fp64_fallback": (N.PRECISION_FALLBACK, N.P2G_BINNED),
+         "fp64": (N.PRECISION_FP64, N.P2G_BINNED), "fixed_point_p2g": (N.PRECISION_FALLBACK, N.P2G_FIXED)}
+
+
+def rotations(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                     np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                     np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def block(n, rng):
+    dx = 1.0 / RES
+    side = (n / 8) ** (1 / 3) * dx
+    lo = np.array([0.5 - side / 2, 0.5 - side / 2, 0.1])
+    return workloads.box_points(lo, lo + side, dx / 2, int(rng.integers(1 << 30)))
+
+
+def phase_a(n, seed=2):
+    rng = np.random.default_rng([seed, 0])
+    x = block(n, rng)
+    m = len(x)
+    sig = rng.uniform(1 - 2.5e-2, 1 + 7.5e-3, (m, 3))
+    F = rotations(rng, m) @ (sig[:, :, None] * np.swapaxes(rotations(rng, m), 1, 2))
+    return {"x": x, "v": rng.normal(scale=0.3, size=(m, 3)).astype(np.float32),
+            "C": rng.normal(scale=0.5, size=(m, 3, 3)).astype(np.float32), "F": F.astype(np.float32),
+            "Jp": rng.uniform(0.2, 0.3, m).astype(np.float32), "mat": np.zeros(m, np.int32),
+            "emitter": np.zeros(m, np.int32)}
+
+
+def phase_b(n, seed=2):
+    rng = np.random.default_rng([seed, 1])
+    x = block(n, rng)
+    m = len(x)
+    s = rng.uniform(0.9, 1.1, m)
+    d = rng.uniform(-1e-6, 1e-6, m)
+    sig = np.stack([s, s * (1 + d), s * (1 - d)], -1)
+    F = rotations(rng, m) @ (sig[:, :, None] * np.swapaxes(rotations(rng, m), 1, 2))
+    return {"x": x, "v": rng.normal(scale=0.3, size=(m, 3)).astype(np.float32),
+            "C": rng.normal(scale=0.5, size=(m, 3, 3)).astype(np.float32), "F": F.astype(np.float32),
+            "Jp": np.ones(m, np.float32), "mat": np.zeros(m, np.int32), "emitter": np.zeros(m, np.int32)}
+
+
+def rel(a, b) -> float:
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def gpu_solver(material, precision, p2g_mode):
+    from mpm_core.backends import create_solver
+
+    d = Domain((0, 0, 0), (1, 1, 1), RES)
+    s = create_solver("hip", domain=d, materials=[material], gravity=(0, 0, -9.81), device="cuda:0")
+    s.set_option(N.OPT_P2G_MODE, p2g_mode)
+    s.set_option(N.OPT_PRECISION, precision)
+    return s
+
+
+def errors_along_trajectory(state, material) -> dict:
+    from oracle import oracle as O
+
+    d = Domain((0, 0, 0), (1, 1, 1), RES)
+    ora = O.OracleSolver(d, [material], (0, 0, -9.81), precision="f64")
+    ora.load_state(state)
+    solvers = {name: gpu_solver(material, *mode) for name, mode in MODES.items()}
+    out = {name: {"grid_momentum": [], "F": []} for name in MODES}
+    for _ in range(SUBSTEPS):
+        st = ora.save_state()
+        gv64, _ = ora.p2g(DT)
+        ora.load_state(st)
+        ora.substep(DT)
+        F64 = ora.save_state()["F"]
+        for name, s in solvers.items():
+            s.load_state(st)
+            gv, _ = s.debug_p2g(DT)
+            out[name]["grid_momentum"].append(rel(gv, gv64))
+            s.load_state(st)
+            s.substep(DT)
+            out[name]["F"].append(rel(s.save_state()["F"], F64))
+    return out
+
+
+@pytest.fixture(scope="module")
+def report():
+    rep = {"particles_per_state": None, "substeps": SUBSTEPS, "dt": DT, "grid": RES,
+           "A_snow_J_0.2": None, "B_elastic_near_degenerate": None}
+    a, b = phase_a(N_STATE), phase_b(N_STATE)
+    rep["particles_per_state"] = [len(a["x"]), len(b["x"])]
+    rep["A_snow_J_0.2"] = errors_along_trajectory(a, SNOW)
+    rep["B_elastic_near_degenerate"] = errors_along_trajectory(b, ELASTIC)
+    path = os.environ.get("MPM_PRECISION_REPORT")
+    if path:
+        with open(path, "w") as f:
+            json.dump(rep, f, indent=1)
+    return rep
+
+
+@pytest.mark.parametrize("phase", ["A_snow_J_0.2", "B_elastic_near_degenerate"])
+def test_every_mode_meets_the_fp32_bar(report, phase):
+    """Single-substep agreement bar (grid momentum 1e-5, F 1e-4) in every mode, against FP64."""
+    for name, errs in report[phase].items():
+        assert max(errs["grid_momentum"]) <= 1e-5, (name, errs)
+        assert max(errs["F"]) <= 1e-4, (name, errs)
+
+
+@pytest.mark.parametrize("phase", ["A_snow_J_0.2", "B_elastic_near_degenerate"])
+def test_fp64_paths_are_no_worse_than_fp32(report, phase):
+    r = report[phase]
+    for q in ("grid_momentum", "F"):
+        base = max(r["fp32"][q])
+        assert max(r["fp64"][q]) <= 1.5 * base + 1e-9, q
+        assert max(r["fp64_fallback"][q]) <= 1.5 * base + 1e-9, q
+
+
+def test_fixed_point_p2g_is_bit_exact_run_to_run():
+    """Two fixed-point P2G runs of one state, with different launch shapes, give the same bits; the
+    FP32 accumulator generally does not (atomic order), and both agree with each other to 1e-5."""
+    state = phase_b(30000, seed=5)
+    grids = []
+    for blocks in (0, 37):
+        s = gpu_solver(ELASTIC, N.PRECISION_FALLBACK, N.P2G_FIXED)
+        s.set_option(N.OPT_P2G_BLOCKS, blocks)
+        s.load_state(state)
+        grids.append(s.debug_p2g(DT))
+        s.load_state(state)
+        again = s.debug_p2g(DT)
+        np.testing.assert_array_equal(again[0], grids[-1][0])
+        np.testing.assert_array_equal(again[1], grids[-1][1])
+    np.testing.assert_array_equal(grids[0][0], grids[1][0])
+    np.testing.assert_array_equal(grids[0][1], grids[1][1])
+    f = gpu_solver(ELASTIC, N.PRECISION_FALLBACK, N.P2G_BINNED)
+    f.load_state(state)
+    gv, gm = f.debug_p2g(DT)
+    assert rel(grids[0][0], gv) <= 1e-5 and rel(grids[0][1], gm) <= 1e-6
+
+
+def test_fixed_point_substeps_run():
+    """A few full substeps in fixed-point mode stay finite and conserve particle count."""
+    state = phase_a(20000, seed=6)
+    s = gpu_solver(SNOW, N.PRECISION_FALLBACK, N.P2G_FIXED)
+    s.load_state(state)
+    for k in range(5):
+        s.substep(DT, (k + 1) / 5, 5 * DT)
+    out = s.save_state()
+    assert len(out["x"]) == len(state["x"]) and np.isfinite(out["x"]).all() and np.isfinite(out["F"]).all()
