@@ -90,18 +90,42 @@ def material_table(materials: Sequence[MaterialParams], domain: Domain, particle
     return arr
 
 
-def collider_poses(start: np.ndarray, end: np.ndarray, dx: float):
+def _pose_parts(m: np.ndarray):
+    """decompose_transform(m) and the smallest singular value of its linear part (the margin scale)."""
+    return decompose_transform(m), float(np.linalg.svd(np.asarray(m)[:3, :3], compute_uv=False).min())
+
+
+class _PoseCache:
+    """_pose_parts of the last few matrices: consecutive advances share their boundary pose (the end of one
+    is the start of the next), so every pose is decomposed once."""
+
+    def __init__(self, size: int = 16):
+        self.size = size
+        self.items: dict[bytes, tuple] = {}
+
+    def __call__(self, m: np.ndarray):
+        key = np.ascontiguousarray(m, np.float32).tobytes()
+        hit = self.items.get(key)
+        if hit is None:
+            hit = _pose_parts(m)
+            if len(self.items) >= self.size:
+                self.items.pop(next(iter(self.items)))
+            self.items[key] = hit
+        return hit
+
+
+def collider_poses(start: np.ndarray, end: np.ndarray, dx: float, parts=_pose_parts):
     """Decomposed start/end poses and AABB margins (solver.py:154-172, 258-260)."""
     poses = (N.ColliderPose * max(1, len(start)))()
     for i, (a, b) in enumerate(zip(start, end)):
-        (t0, q0, S0), (t1, q1, S1) = decompose_transform(a), decompose_transform(b)
+        ((t0, q0, S0), smin0), ((t1, q1, S1), smin1) = parts(a), parts(b)
         if np.dot(q0, q1) < 0:
             q1 = -q1  # slerp the short way round
         p = poses[i]
         p.t0[:], p.t1[:] = t0.tolist(), t1.tolist()
         p.q0[:], p.q1[:] = q0.tolist(), q1.tolist()
         p.S0[:], p.S1[:] = S0.ravel().tolist(), S1.ravel().tolist()
-        inv_scale = max(1.0 / max(np.linalg.svd(m[:3, :3], compute_uv=False).min(), 1e-12) for m in (a, b))
+        inv_scale = max(1.0 / max(smin0, 1e-12), 1.0 / max(smin1, 1e-12))
         p.margin = 3.0 * dx * inv_scale
     return poses
 
@@ -123,6 +147,7 @@ class HipSolver(Solver):
         self._materials_desc = material_table(self.materials, domain, particles_per_cell)
         self.dims = tuple(self._domain_desc.dims)
         self._n_colliders = 0
+        self._poses = _PoseCache()
         handle = C.c_void_p()
         N.check(self._lib.mpm_create(self.ordinal, C.byref(self._domain_desc), self._materials_desc,
                                      len(self.materials), C.byref(handle)))
@@ -210,7 +235,7 @@ class HipSolver(Solver):
         end = np.asarray(end, np.float32).reshape(-1, 4, 4)
         if len(start) != self._n_colliders or len(end) != self._n_colliders:
             raise ValueError(f"expected {self._n_colliders} collider transforms")
-        poses = collider_poses(start, end, self.dx)
+        poses = collider_poses(start, end, self.dx, self._poses)
         N.check(self._lib.mpm_set_collider_poses(self._ctx, self._n_colliders, poses))
 
     # -- stepping --------------------------------------------------------------------------------

@@ -96,9 +96,10 @@ struct Timing {
   int64_t launches[MPM_K_COUNT] = {};
 };
 
-// all per-slot particle planes of one buffer set, in one allocation:
-// x 3, v 3, C 9, F 9, Jp 1, tau 6 float planes, then mat, pid int planes
-constexpr int kFloatPlanes = 31;
+// all per-slot particle arrays of one buffer set, in one allocation: x 3, v 3, F 9, Jp 1 float planes,
+// the 16-float records (C, stress, mass; 64 bytes per slot), then the mat and pid int planes
+constexpr int kFloatPlanes = 16;
+constexpr int kRecFloats = 16;
 constexpr int kIntPlanes = 2;
 
 struct ParticleSet {
@@ -106,14 +107,13 @@ struct ParticleSet {
   int64_t cap = 0;
   float* x() const { return reinterpret_cast<float*>(base); }
   float* v() const { return x() + 3 * cap; }
-  float* C() const { return x() + 6 * cap; }
-  float* F() const { return x() + 15 * cap; }
-  float* Jp() const { return x() + 24 * cap; }
-  float* tau() const { return x() + 25 * cap; }
-  int32_t* mat() const { return reinterpret_cast<int32_t*>(x() + kFloatPlanes * cap); }
+  float* F() const { return x() + 6 * cap; }
+  float* Jp() const { return x() + 15 * cap; }
+  float4* rec() const { return reinterpret_cast<float4*>(x() + kFloatPlanes * cap); }
+  int32_t* mat() const { return reinterpret_cast<int32_t*>(x() + (kFloatPlanes + kRecFloats) * cap); }
   int32_t* pid() const { return mat() + cap; }
-  Particles view() const { return Particles{x(), v(), C(), F(), Jp(), mat(), pid(), cap}; }
-  static size_t bytes(int64_t cap) { return (size_t)cap * 4 * (kFloatPlanes + kIntPlanes); }
+  Particles view() const { return Particles{x(), v(), F(), Jp(), mat(), pid(), rec(), cap}; }
+  static size_t bytes(int64_t cap) { return (size_t)cap * 4 * (kFloatPlanes + kRecFloats + kIntPlanes); }
 };
 
 struct ColliderHost {
@@ -357,9 +357,11 @@ void reserve(mpm_ctx* c, int64_t need) {
   uint32_t* v2 = dev_alloc<uint32_t>(cap);
   uint32_t* fl = dev_alloc<uint32_t>(cap);
   if (c->n) {
-    for (int k = 0; k < kFloatPlanes + kIntPlanes; ++k)
-      HIP_CHECK(hipMemcpyAsync(A.base + (size_t)k * cap * 4, c->A.base + (size_t)k * c->cap * 4, c->n * 4,
-                               hipMemcpyDeviceToDevice, c->stream));
+    for (int k = 0; k < kFloatPlanes; ++k)
+      HIP_CHECK(hipMemcpyAsync(A.x() + k * cap, c->A.x() + k * c->cap, c->n * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(A.rec(), c->A.rec(), c->n * kRecFloats * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(A.mat(), c->A.mat(), c->n * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(A.pid(), c->A.pid(), c->n * 4, hipMemcpyDeviceToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(em, c->emitter, c->n * 4, hipMemcpyDeviceToDevice, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
@@ -394,6 +396,16 @@ void download_planes(mpm_ctx* c, const float* planes, float* host, int comps) {
   k_pack<<<grid_for(c->n), kBlock, 0, c->stream>>>(planes, c->cap, c->A.pid(), comps, (int)c->n, stage);
   check_launch("k_pack");
   HIP_CHECK(hipMemcpyAsync(host, stage, c->n * comps * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// C of every particle into host rows in insertion order
+void download_C(mpm_ctx* c, float* host) {
+  if (c->n == 0) return;
+  float* stage = static_cast<float*>(c->scratch.get(c->n * 9 * sizeof(float)));
+  k_pack_C<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), (int)c->n, stage);
+  check_launch("k_pack_C");
+  HIP_CHECK(hipMemcpyAsync(host, stage, c->n * 9 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -465,7 +477,7 @@ void sort_particles(mpm_ctx* c) {
   check_launch("k_cell_bounds");
   k_active_bricks<<<grid_for(c->n_bricks), kBlock, 0, c->stream>>>(c->offsets, c->n_bricks, c->active, c->n_active);
   check_launch("k_active_bricks");
-  k_reorder<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.view(), c->B.view(), c->A.tau(), c->B.tau(), vin, (int)n);
+  k_reorder<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.view(), c->B.view(), vin, (int)n);
   check_launch("k_reorder");
   std::swap(c->A, c->B);
   c->last_perm = const_cast<uint32_t*>(vin);
@@ -489,9 +501,8 @@ void run_p2g(mpm_ctx* c, float dt) {
   // four waves (four bricks at a time) per workgroup, grid-stride over the device-side brick list
   int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 2048;
   blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, (std::min<int64_t>(c->n_bricks, c->n) + 3) / 4));
-  k_p2g_cells<<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->A.tau(), c->d_mats, c->geom, c->sc.p_vol, dt,
-                                                       c->offsets, c->active, c->n_active, c->accum, c->dirty,
-                                                       c->fixed());
+  k_p2g_cells<<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets, c->active,
+                                                       c->n_active, c->accum, c->dirty, c->fixed());
   check_launch("k_p2g_cells");
 }
 
@@ -518,13 +529,13 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
     check_launch("k_zero_u32");
   }
   k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel,
-                                                  c->A.tau(), block_max(c), c->collider_set(), s, inv_frame_dt,
+                                                  block_max(c), c->collider_set(), s, inv_frame_dt,
                                                   FixList{c->fix_list, c->n_fix});
   check_launch("k_g2p");
   if (fix) {
     // a persistent grid: the deferred particles are usually few
     k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 1024), kBlock, 0, c->stream>>>(
-        c->parts(), c->d_mats, c->fix_list, c->n_fix, c->A.tau());
+        c->parts(), c->d_mats, c->fix_list, c->n_fix);
     check_launch("k_fp64_fixup");
   }
   c->block_max_count = grid_for(c->n);
@@ -576,7 +587,12 @@ void load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const flo
   reserve(c, n);
   upload_planes(c, c->A.x(), x, 3, n, 0);
   upload_planes(c, c->A.v(), v, 3, n, 0);
-  upload_planes(c, c->A.C(), Cm, 9, n, 0);
+  if (n) {
+    float* stage = static_cast<float*>(c->scratch.get(n * 9 * sizeof(float)));
+    HIP_CHECK(hipMemcpyAsync(stage, Cm, n * 9 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    k_unpack_C<<<grid_for(n), kBlock, 0, c->stream>>>(stage, (int)n, 0, c->parts());
+    check_launch("k_unpack_C");
+  }
   upload_planes(c, c->A.F(), F, 9, n, 0);
   upload_planes(c, c->A.Jp(), Jp, 1, n, 0);
   if (n) {
@@ -584,7 +600,7 @@ void load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const flo
     HIP_CHECK(hipMemcpyAsync(c->emitter, emitter, n * 4, hipMemcpyHostToDevice, c->stream));
     k_iota<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.pid(), (int)n);
     check_launch("k_iota");
-    k_stress<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, (int)n, c->sc.precision, c->A.tau());
+    k_stress<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, (int)n, c->sc.precision);
     check_launch("k_stress");
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
@@ -629,7 +645,7 @@ int64_t remove_escaped(mpm_ctx* c) {
   std::vector<int32_t> mat(n), em(n);
   download_planes(c, c->A.x(), x.data(), 3);
   download_planes(c, c->A.v(), v.data(), 3);
-  download_planes(c, c->A.C(), C.data(), 9);
+  download_C(c, C.data());
   download_planes(c, c->A.F(), F.data(), 9);
   download_planes(c, c->A.Jp(), Jp.data(), 1);
   download_planes(c, reinterpret_cast<float*>(c->A.mat()), reinterpret_cast<float*>(mat.data()), 1);
@@ -973,8 +989,6 @@ SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
   SceneDev d{};
   d.set[0] = c->A.view();
   d.set[1] = c->B.view();
-  d.tau[0] = c->A.tau();
-  d.tau[1] = c->B.tau();
   d.mats = c->d_mats;
   d.g = c->geom;
   d.sc = c->sc;
@@ -1460,7 +1474,7 @@ int32_t mpm_add_particles(mpm_ctx* c, int64_t n, const float* x, const float* v,
     upload_planes(c, c->A.v(), v, 3, n, p0);
     HIP_CHECK(hipMemcpyAsync(c->A.mat() + p0, mat, n * 4, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(c->emitter + p0, emitter, n * 4, hipMemcpyHostToDevice, c->stream));
-    k_init_new<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->A.tau(), p0, (int)n);
+    k_init_new<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, p0, (int)n);
     check_launch("k_init_new");
     HIP_CHECK(hipStreamSynchronize(c->stream));
     c->n = p0 + n;
@@ -1485,7 +1499,7 @@ int32_t mpm_get_state(mpm_ctx* c, float* x, float* v, float* Cm, float* F, float
     require_idle(c);
     if (x) download_planes(c, c->A.x(), x, 3);
     if (v) download_planes(c, c->A.v(), v, 3);
-    if (Cm) download_planes(c, c->A.C(), Cm, 9);
+    if (Cm) download_C(c, Cm);
     if (F) download_planes(c, c->A.F(), F, 9);
     if (Jp) download_planes(c, c->A.Jp(), Jp, 1);
     if (mat) download_planes(c, reinterpret_cast<const float*>(c->A.mat()), reinterpret_cast<float*>(mat), 1);
@@ -1589,9 +1603,10 @@ int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
     const int64_t steps = (int64_t)std::ceil(frame_dt / cfl_dt - 1e-9);
     const double dt = frame_dt / (double)steps;
     for (int64_t k = 0; k < steps; ++k) substep(c, dt, (double)(k + 1) / (double)steps, frame_dt);
+    // the removal count needs the positions (a host wait, only with open sides); otherwise the substeps
+    // stay in flight and the host prepares the next call meanwhile (every read of the state waits for them)
     const int64_t removed = remove_escaped(c);
     if (removed) refresh_present(c);
-    HIP_CHECK(hipStreamSynchronize(c->stream));
     *out = mpm_advance_stats{steps, dt, removed};
   });
 }
@@ -1763,8 +1778,7 @@ int32_t mpm_set_option(mpm_ctx* c, int32_t option, int64_t value) {
           c->sc.precision = (int)value;
           // the stored stress belongs to the old precision
           if (c->n) {
-            k_stress<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, (int)c->n, c->sc.precision,
-                                                              c->A.tau());
+            k_stress<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, (int)c->n, c->sc.precision);
             check_launch("k_stress");
             HIP_CHECK(hipStreamSynchronize(c->stream));
           }

@@ -37,7 +37,6 @@ constexpr int kPlanThreads = 1024;
 
 struct SceneDev {
   Particles set[2];  // set[cur] is live, the other is the reorder target
-  float* tau[2];
   const MatDev* mats;
   GridGeom g;
   StepConsts sc;
@@ -334,8 +333,6 @@ __global__ void __launch_bounds__(kTile) k_batch_reorder(const SceneDev* __restr
     if (p >= S.n) continue;
     const int cur = st[i].cur;
     const Particles src = S.set[cur], dst = S.set[1 - cur];
-    const float* tau_src = S.tau[cur];
-    float* tau_dst = S.tau[1 - cur];
     const int64_t s = sorted_vals(S)[p];
     const int64_t cs = src.cap, cd = dst.cap;
 #pragma unroll
@@ -344,12 +341,9 @@ __global__ void __launch_bounds__(kTile) k_batch_reorder(const SceneDev* __restr
       dst.v[a * cd + p] = src.v[a * cs + s];
     }
 #pragma unroll
-    for (int e = 0; e < 9; ++e) {
-      dst.C[e * cd + p] = src.C[e * cs + s];
-      dst.F[e * cd + p] = src.F[e * cs + s];
-    }
+    for (int e = 0; e < 9; ++e) dst.F[e * cd + p] = src.F[e * cs + s];
 #pragma unroll
-    for (int e = 0; e < 6; ++e) tau_dst[e * cd + p] = tau_src[e * cs + s];
+    for (int e = 0; e < 4; ++e) dst.rec[4 * (int64_t)p + e] = src.rec[4 * s + e];
     dst.Jp[p] = src.Jp[s];
     dst.mat[p] = src.mat[s];
     dst.pid[p] = src.pid[s];
@@ -398,8 +392,8 @@ __global__ void __launch_bounds__(256) k_batch_p2g(const SceneDev* __restrict__ 
     const SceneDev& S = sd[i];
     const int cur = st[i].cur;
     const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
-    p2g_brick(S.set[cur], S.tau[cur], S.mats, S.g, k, S.offsets, S.active[w - P0[i]], s_win[wave], S.accum,
-              S.dirty, lane, FixedAccum{nullptr, 0.0, 0.0});
+    p2g_brick(S.set[cur], S.g, k, S.offsets, S.active[w - P0[i]], s_win[wave], S.accum, S.dirty, lane,
+              FixedAccum{nullptr, 0.0, 0.0});
   }
 }
 
@@ -485,8 +479,7 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ 
     float speed = 0.0f;
     bool defer = false;
     if (p < S.n)
-      defer = g2p_particle(P, S.mats, S.g, S.sc, st[i].dt, S.vel, S.tau[cur], p, S.cs, frames + i * kMaxColliders,
-                           speed);
+      defer = g2p_particle(P, S.mats, S.g, S.sc, st[i].dt, S.vel, p, S.cs, frames + i * kMaxColliders, speed);
     if (S.sc.precision != kPrecisionFp32) wave_append(FixList{S.fix_list, &st[i].n_fix}, defer, (uint32_t)p);
     if (!(speed == speed)) speed = __int_as_float(0x7f800000);
     const float wm = wave_max(speed);
@@ -535,7 +528,7 @@ __global__ void __launch_bounds__(256) k_batch_fixup(const SceneDev* __restrict_
     const int cur = st[i].cur;
     Particles P = S.set[cur];
     const int64_t p = S.fix_list[u - P0[i]];
-    material_update_fp64(S.mats[P.mat[p]], P, S.tau[cur], p);
+    material_update_fp64(S.mats[P.mat[p]], P, p);
   }
 }
 

@@ -201,8 +201,7 @@ __global__ void __launch_bounds__(256) k_active_bricks(const uint32_t* __restric
 }
 
 // gather every per-slot particle array into sorted order: dst[i] = src[perm[i]]
-__global__ void __launch_bounds__(256) k_reorder(Particles src, Particles dst, float* __restrict__ tau_src,
-                                                 float* __restrict__ tau_dst, const uint32_t* __restrict__ perm,
+__global__ void __launch_bounds__(256) k_reorder(Particles src, Particles dst, const uint32_t* __restrict__ perm,
                                                  int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -214,12 +213,9 @@ __global__ void __launch_bounds__(256) k_reorder(Particles src, Particles dst, f
     dst.v[a * cd + i] = src.v[a * cs + s];
   }
 #pragma unroll
-  for (int e = 0; e < 9; ++e) {
-    dst.C[e * cd + i] = src.C[e * cs + s];
-    dst.F[e * cd + i] = src.F[e * cs + s];
-  }
+  for (int e = 0; e < 9; ++e) dst.F[e * cd + i] = src.F[e * cs + s];
 #pragma unroll
-  for (int e = 0; e < 6; ++e) tau_dst[e * cd + i] = tau_src[e * cs + s];
+  for (int e = 0; e < 4; ++e) dst.rec[4 * (int64_t)i + e] = src.rec[4 * s + e];
   dst.Jp[i] = src.Jp[s];
   dst.mat[i] = src.mat[s];
   dst.pid[i] = src.pid[s];

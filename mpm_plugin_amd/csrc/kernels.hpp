@@ -23,14 +23,18 @@ struct MatDev {
   float mu, lam, yield_stress, alpha, hardening, max_hardening, crit_c, crit_s, p_mass;
 };
 
+// Particle slots: x, v, F, Jp as planes of `cap` floats; per slot one 64-byte record `rec` (4 float4)
+// holding what P2G reads besides x and v: C (row-major, 9), the Kirchhoff stress of F (xx, yy, zz, xy,
+// xz, yz) and the particle mass. One record per lane makes P2G's loads whole cache-line pieces where
+// planes would scatter them (its lanes walk different cells, not consecutive slots).
 struct Particles {
   float* x;
   float* v;
-  float* C;
   float* F;
   float* Jp;
   int32_t* mat;
   int32_t* pid;
+  float4* rec;
   int64_t cap;
 };
 
@@ -203,14 +207,29 @@ __device__ __forceinline__ void particle_stress(const MatDev& m, const M3<float>
   stress_from_svd(m, U, sig, Jp, t);
 }
 
-__device__ __forceinline__ void store_tau(float* tau, int64_t cap, int64_t p, const float (&t)[6]) {
-#pragma unroll
-  for (int e = 0; e < 6; ++e) tau[e * cap + p] = t[e];
+__device__ __forceinline__ void load_C(const Particles& P, int64_t p, M3<float>& C) {
+  const float4 a = P.rec[4 * p], b = P.rec[4 * p + 1], c = P.rec[4 * p + 2];
+  C.a[0][0] = a.x; C.a[0][1] = a.y; C.a[0][2] = a.z; C.a[1][0] = a.w;
+  C.a[1][1] = b.x; C.a[1][2] = b.y; C.a[2][0] = b.z; C.a[2][1] = b.w;
+  C.a[2][2] = c.x;
+}
+__device__ __forceinline__ void store_rec(const Particles& P, int64_t p, const M3<float>& C, const float (&t)[6],
+                                          float pm) {
+  P.rec[4 * p] = make_float4(C.a[0][0], C.a[0][1], C.a[0][2], C.a[1][0]);
+  P.rec[4 * p + 1] = make_float4(C.a[1][1], C.a[1][2], C.a[2][0], C.a[2][1]);
+  P.rec[4 * p + 2] = make_float4(C.a[2][2], t[0], t[1], t[2]);
+  P.rec[4 * p + 3] = make_float4(t[3], t[4], t[5], pm);
+}
+// stress and mass of a record whose C stays
+__device__ __forceinline__ void store_tau_pm(const Particles& P, int64_t p, const float (&t)[6], float pm) {
+  const float c22 = P.rec[4 * p + 2].x;
+  P.rec[4 * p + 2] = make_float4(c22, t[0], t[1], t[2]);
+  P.rec[4 * p + 3] = make_float4(t[3], t[4], t[5], pm);
 }
 
 // stress of every particle from its F (and Jp): after load_state, when tau is not yet known
-__global__ void __launch_bounds__(256) k_stress(Particles P, const MatDev* __restrict__ mats, int n, int precision,
-                                                float* __restrict__ tau) {
+// stress and particle mass of every record from F, Jp and the material (after a state upload)
+__global__ void __launch_bounds__(256) k_stress(Particles P, const MatDev* __restrict__ mats, int n, int precision) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const MatDev m = mats[P.mat[p]];
@@ -218,7 +237,7 @@ __global__ void __launch_bounds__(256) k_stress(Particles P, const MatDev* __res
   load_m3(P.F, P.cap, p, F);
   float t[6];
   particle_stress(m, F, P.Jp[p], precision, t);
-  store_tau(tau, P.cap, p, t);
+  store_tau_pm(P, p, t, m.p_mass);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -231,7 +250,7 @@ __global__ void __launch_bounds__(256) k_p2g_atomic(Particles P, const MatDev* _
   const MatDev m = mats[P.mat[p]];
   M3<float> F, Cm;
   load_m3(P.F, cap, p, F);
-  load_m3(P.C, cap, p, Cm);
+  load_C(P, p, Cm);
   float t6[6];
   particle_stress(m, F, P.Jp[p], sc.precision, t6);
   M3<float> tau;
@@ -318,19 +337,41 @@ __global__ void k_unpack(const float* __restrict__ src, int comps, int n, int64_
 }
 
 // new particles: C = 0, F = I, Jp = 1, tau = 0 (the stress of F = I), pid = insertion index
-__global__ void k_init_new(Particles P, float* __restrict__ tau, int64_t p0, int n) {
+// fresh particles (solver.py:100-125): C = 0, F = I, Jp = 1, zero stress (F = I is unstressed in every model)
+__global__ void k_init_new(Particles P, const MatDev* __restrict__ mats, int64_t p0, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t p = p0 + i;
 #pragma unroll
-  for (int e = 0; e < 9; ++e) {
-    P.C[e * P.cap + p] = 0.0f;
-    P.F[e * P.cap + p] = (e % 4 == 0) ? 1.0f : 0.0f;
-  }
-#pragma unroll
-  for (int e = 0; e < 6; ++e) tau[e * P.cap + p] = 0.0f;
+  for (int e = 0; e < 9; ++e) P.F[e * P.cap + p] = (e % 4 == 0) ? 1.0f : 0.0f;
+  const float pm = mats[P.mat[p]].p_mass;
+  P.rec[4 * p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  P.rec[4 * p + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  P.rec[4 * p + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  P.rec[4 * p + 3] = make_float4(0.0f, 0.0f, 0.0f, pm);
   P.Jp[p] = 1.0f;
   P.pid[p] = (int32_t)p;
+}
+
+// C of every slot into host rows (insertion order through pid) and back; uploads leave stress and mass
+// to k_stress
+__global__ void k_pack_C(Particles P, int n, float* __restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  M3<float> C;
+  load_C(P, p, C);
+  const int64_t q = P.pid[p];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) dst[q * 9 + e] = C.a[e / 3][e % 3];
+}
+__global__ void k_unpack_C(const float* __restrict__ src, int n, int64_t p0, Particles P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  M3<float> C;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) C.a[e / 3][e % 3] = src[(int64_t)i * 9 + e];
+  const float z[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  store_rec(P, p0 + i, C, z, 0.0f);
 }
 
 __global__ void k_iota(int32_t* __restrict__ out, int n) {

@@ -50,29 +50,42 @@ struct P2GParticle {
   float wx[3][3], dp[3][3];
 };
 
-__device__ __forceinline__ void load_p2g_particle(const Particles& P, const float* __restrict__ tau,
-                                                  const MatDev* __restrict__ mats, const GridGeom& g, float k,
-                                                  int64_t p, P2GParticle& q) {
+// the per-particle inputs of P2G as stored: x, v from their planes, C, stress and mass from the record
+struct P2GRaw {
+  float x[3], v[3];
+  float4 r[4];
+};
+
+__device__ __forceinline__ void load_p2g_raw(const Particles& P, int64_t p, P2GRaw& r) {
   const int64_t cap = P.cap;
-  const float pm = mats[P.mat[p]].p_mass;
-  const float t00 = tau[p], t11 = tau[cap + p], t22 = tau[2 * cap + p];
-  const float t01 = tau[3 * cap + p], t02 = tau[4 * cap + p], t12 = tau[5 * cap + p];
-  q.A[0][0] = k * t00 + pm * P.C[p];
-  q.A[0][1] = k * t01 + pm * P.C[cap + p];
-  q.A[0][2] = k * t02 + pm * P.C[2 * cap + p];
-  q.A[1][0] = k * t01 + pm * P.C[3 * cap + p];
-  q.A[1][1] = k * t11 + pm * P.C[4 * cap + p];
-  q.A[1][2] = k * t12 + pm * P.C[5 * cap + p];
-  q.A[2][0] = k * t02 + pm * P.C[6 * cap + p];
-  q.A[2][1] = k * t12 + pm * P.C[7 * cap + p];
-  q.A[2][2] = k * t22 + pm * P.C[8 * cap + p];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    r.x[a] = P.x[a * cap + p];
+    r.v[a] = P.v[a * cap + p];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) r.r[e] = P.rec[4 * p + e];
+}
+
+__device__ __forceinline__ void make_p2g_particle(const P2GRaw& r, const GridGeom& g, float k, P2GParticle& q) {
+  const float C[9] = {r.r[0].x, r.r[0].y, r.r[0].z, r.r[0].w, r.r[1].x, r.r[1].y, r.r[1].z, r.r[1].w, r.r[2].x};
+  const float t00 = r.r[2].y, t11 = r.r[2].z, t22 = r.r[2].w, t01 = r.r[3].x, t02 = r.r[3].y, t12 = r.r[3].z;
+  const float pm = r.r[3].w;
+  q.A[0][0] = k * t00 + pm * C[0];
+  q.A[0][1] = k * t01 + pm * C[1];
+  q.A[0][2] = k * t02 + pm * C[2];
+  q.A[1][0] = k * t01 + pm * C[3];
+  q.A[1][1] = k * t11 + pm * C[4];
+  q.A[1][2] = k * t12 + pm * C[5];
+  q.A[2][0] = k * t02 + pm * C[6];
+  q.A[2][1] = k * t12 + pm * C[7];
+  q.A[2][2] = k * t22 + pm * C[8];
   q.pm = pm;
-  const float xo[3] = {P.x[p], P.x[cap + p], P.x[2 * cap + p]};
-  q.mv[0] = pm * P.v[p];
-  q.mv[1] = pm * P.v[cap + p];
-  q.mv[2] = pm * P.v[2 * cap + p];
+  q.mv[0] = pm * r.v[0];
+  q.mv[1] = pm * r.v[1];
+  q.mv[2] = pm * r.v[2];
   float fx[3];
-  stencil_base(g, xo, q.base, fx);
+  stencil_base(g, r.x, q.base, fx);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -142,8 +155,7 @@ __device__ __forceinline__ void wave_lds_fence() {
 // of a wave own 64 different cells, so when they then add their sums into the wave's private LDS window
 // every store of one instruction hits a different node: plain read-add-write, no LDS atomics. The window
 // is flushed to the bricked HBM grid with global atomics (neighbouring windows overlap).
-__device__ __forceinline__ void p2g_brick(const Particles& P, const float* __restrict__ tau,
-                                          const MatDev* __restrict__ mats, const GridGeom& g, float k,
+__device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g, float k,
                                           const uint32_t* __restrict__ cell_off, uint32_t b, float (*win)[kWinNodes],
                                           float4* __restrict__ accum, uint32_t* __restrict__ dirty, int lane,
                                           const FixedAccum& fa) {
@@ -170,9 +182,14 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const float* __res
       for (int kk = 0; kk < 3; ++kk)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[i][j][kk][c] = 0.0f;
+  // the next particle's inputs load while the current one is scattered
+  P2GRaw next;
+  if (p0 < p1) load_p2g_raw(P, (int64_t)p0, next);
   for (uint32_t p = p0; p < p1; ++p) {
+    const P2GRaw cur = next;
+    if (p + 1 < p1) load_p2g_raw(P, (int64_t)p + 1, next);
     P2GParticle q;
-    load_p2g_particle(P, tau, mats, g, k, (int64_t)p, q);
+    make_p2g_particle(cur, g, k, q);
     if (q.base[0] == cb[0] && q.base[1] == cb[1] && q.base[2] == cb[2]) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -240,8 +257,7 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const float* __res
 
 // P2G over the device-side list of occupied bricks: four waves (four bricks at a time) per workgroup,
 // grid-stride, so the launch needs no host-side count
-__global__ void __launch_bounds__(256) k_p2g_cells(Particles P, const float* __restrict__ tau,
-                                                   const MatDev* __restrict__ mats, GridGeom g, float p_vol, float dt,
+__global__ void __launch_bounds__(256) k_p2g_cells(Particles P, GridGeom g, float p_vol, float dt,
                                                    const uint32_t* __restrict__ cell_off,
                                                    const uint32_t* __restrict__ active,
                                                    const uint32_t* __restrict__ n_active_ptr,
@@ -253,7 +269,7 @@ __global__ void __launch_bounds__(256) k_p2g_cells(Particles P, const float* __r
   const uint32_t n_active = *n_active_ptr;
   const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
   for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves)
-    p2g_brick(P, tau, mats, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane, fa);
+    p2g_brick(P, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane, fa);
 }
 
 // grid update of one dirty brick by one wave (lane = node): momentum -> velocity, gravity, walls,
@@ -337,8 +353,7 @@ __device__ __forceinline__ float wave_max(float v) {
 // and the stress of the result, in precision T; F, Jp and tau are stored in FP32.
 template <class T>
 __device__ __forceinline__ void material_update(const MatDev& m, const M3<T>& Fn, const M3<T>& U, const T (&sig)[3],
-                                                const M3<T>& V, Particles& P, float* __restrict__ tau_out,
-                                                int64_t p) {
+                                                const M3<T>& V, Particles& P, float (&tau_out)[6], int64_t p) {
   const int64_t cap = P.cap;
   T t[6];
   if (m.model == kElastic) {
@@ -365,35 +380,36 @@ __device__ __forceinline__ void material_update(const MatDev& m, const M3<T>& Fn
     stress_from_svd(m, U, ns, jp, t);
   }
 #pragma unroll
-  for (int e = 0; e < 6; ++e) tau_out[e * cap + p] = (float)t[e];
+  for (int e = 0; e < 6; ++e) tau_out[e] = (float)t[e];
 }
 
 // The FP64 pass of material_update for one particle whose FP32 pass was deferred: F holds the updated
 // (not yet projected) deformation gradient; SVD, return map and stress in double precision.
-__device__ __forceinline__ void material_update_fp64(const MatDev& m, Particles& P, float* __restrict__ tau_out,
-                                                     int64_t p) {
+__device__ __forceinline__ void material_update_fp64(const MatDev& m, Particles& P, int64_t p) {
   M3<double> Fd, Ud, Vd;
 #pragma unroll
   for (int e = 0; e < 9; ++e) Fd.a[e / 3][e % 3] = (double)P.F[e * P.cap + p];
   double sd[3];
   svd3(Fd, Ud, sd, Vd);
-  material_update(m, Fd, Ud, sd, Vd, P, tau_out, p);
+  float t[6];
+  material_update(m, Fd, Ud, sd, Vd, P, t, p);
+  store_tau_pm(P, p, t, m.p_mass);
 }
 
 // FP64 pass over the deferred particles (grid-stride over the work list)
 __global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* __restrict__ mats,
                                                     const uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ count, float* __restrict__ tau) {
+                                                    const uint32_t* __restrict__ count) {
   const uint32_t n = *count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t p = list[i];
-    material_update_fp64(mats[P.mat[p]], P, tau, p);
+    material_update_fp64(mats[P.mat[p]], P, p);
   }
 }
 
 __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restrict__ mats, const GridGeom& g,
                                              const StepConsts& sc, float dt, const float4* __restrict__ vel,
-                                             float* __restrict__ tau_out, int64_t p, const ColliderSet& cs,
+                                             int64_t p, const ColliderSet& cs,
                                              const ColliderFrame* frames, float& speed) {
   const int64_t cap = P.cap;
   const float xo[3] = {P.x[p], P.x[cap + p], P.x[2 * cap + p]};
@@ -434,7 +450,10 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
       }
     }
   }
-  store_m3(P.C, cap, p, nC);
+  // the record's C part now (only C22 stays live: it shares a float4 with the stress written last)
+  P.rec[4 * p] = make_float4(nC.a[0][0], nC.a[0][1], nC.a[0][2], nC.a[1][0]);
+  P.rec[4 * p + 1] = make_float4(nC.a[1][1], nC.a[1][2], nC.a[2][0], nC.a[2][1]);
+  const float c22 = nC.a[2][2];
   float nx[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -465,12 +484,17 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
   M3<float> U, V;
   float sig[3];
   svd3(Fn, U, sig, V);
+  float t[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (needs_fp64(sc.precision, sig, m.model == kSnow ? P.Jp[p] : 1.0f)) {
-    // deferred to the FP64 pass (k_fp64_fixup), which reads Fn back from F
+    // deferred to the FP64 pass (k_fp64_fixup), which reads Fn back from F and writes the stress
     store_m3(P.F, cap, p, Fn);
+    P.rec[4 * p + 2] = make_float4(c22, 0.0f, 0.0f, 0.0f);
+    P.rec[4 * p + 3] = make_float4(0.0f, 0.0f, 0.0f, m.p_mass);
     return true;
   }
-  material_update(m, Fn, U, sig, V, P, tau_out, p);
+  material_update(m, Fn, U, sig, V, P, t, p);
+  P.rec[4 * p + 2] = make_float4(c22, t[0], t[1], t[2]);
+  P.rec[4 * p + 3] = make_float4(t[3], t[4], t[5], m.p_mass);
   return false;
 }
 
@@ -491,14 +515,14 @@ __device__ __forceinline__ void block_max_speed(float speed, float* __restrict__
 
 __global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restrict__ mats, GridGeom g,
                                              StepConsts sc, int n, float dt, const float4* __restrict__ vel,
-                                             float* __restrict__ tau, float* __restrict__ block_max,
+                                             float* __restrict__ block_max,
                                              ColliderSet cs, float s, float inv_frame_dt, FixList fix) {
   __shared__ ColliderFrame frames[kMaxColliders];
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float speed = 0.0f;
   bool defer = false;
-  if (p < n) defer = g2p_particle(P, mats, g, sc, dt, vel, tau, p, cs, frames, speed);
+  if (p < n) defer = g2p_particle(P, mats, g, sc, dt, vel, p, cs, frames, speed);
   if (sc.precision != kPrecisionFp32) wave_append(fix, defer, (uint32_t)p);
   block_max_speed(speed, block_max);
 }
