@@ -156,6 +156,9 @@ struct mpm_ctx {
   uint32_t *keys0 = nullptr, *keys1 = nullptr, *keys2 = nullptr, *vals1 = nullptr, *vals2 = nullptr;
   uint32_t *offsets = nullptr, *active = nullptr, *n_active = nullptr;  // offsets: per cell
   uint32_t* dirty_list = nullptr;  // batched grid update work list
+  int32_t* slot = nullptr;         // per brick: its position in `active`, or -1
+  float4* stage = nullptr;         // P2G windows of the occupied bricks (stage_cap x 512 nodes)
+  int64_t stage_cap = 0;
   uint32_t* fix_list = nullptr;    // particles deferred to the FP64 pass (cap entries)
   uint32_t* n_fix = nullptr;       // their count
   Scratch hist, scan_sums;
@@ -377,6 +380,14 @@ void reserve(mpm_ctx* c, int64_t need) {
   c->vals2 = v2;
   c->fix_list = fl;
   c->cap = cap;
+  // one staged window per occupied brick: at most one brick per particle
+  const int64_t stage_need = std::min<int64_t>(c->n_bricks, cap);
+  if (stage_need > c->stage_cap) {
+    dev_free(c->stage);
+    c->stage = nullptr;
+    c->stage = dev_alloc<float4>((size_t)stage_need * kWinNodes);
+    c->stage_cap = stage_need;
+  }
   c->n = n;
 }
 
@@ -451,6 +462,9 @@ void sort_particles(mpm_ctx* c) {
   if (n == 0) {
     k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(c->keys1, 0, c->n_cells, c->offsets);
     check_launch("k_cell_bounds");
+    k_active_bricks<<<grid_for(c->n_bricks), kBlock, 0, c->stream>>>(c->offsets, c->n_bricks, c->active,
+                                                                     c->n_active, c->slot);
+    check_launch("k_active_bricks");
     c->sorted = true;
     c->steps_since_sort = 0;
     return;
@@ -475,7 +489,8 @@ void sort_particles(mpm_ctx* c) {
   }
   k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(kin, (int)n, c->n_cells, c->offsets);
   check_launch("k_cell_bounds");
-  k_active_bricks<<<grid_for(c->n_bricks), kBlock, 0, c->stream>>>(c->offsets, c->n_bricks, c->active, c->n_active);
+  k_active_bricks<<<grid_for(c->n_bricks), kBlock, 0, c->stream>>>(c->offsets, c->n_bricks, c->active, c->n_active,
+                                                                   c->slot);
   check_launch("k_active_bricks");
   k_reorder<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.view(), c->B.view(), vin, (int)n);
   check_launch("k_reorder");
@@ -502,7 +517,7 @@ void run_p2g(mpm_ctx* c, float dt) {
   int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 2048;
   blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, (std::min<int64_t>(c->n_bricks, c->n) + 3) / 4));
   k_p2g_cells<<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets, c->active,
-                                                       c->n_active, c->accum, c->dirty, c->fixed());
+                                                       c->n_active, c->accum, c->dirty, c->fixed(), c->stage);
   check_launch("k_p2g_cells");
 }
 
@@ -515,7 +530,8 @@ void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   } else {
     const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
     k_grid_bricks<<<blocks, 256, 0, c->stream>>>(c->geom, c->sc, dt, c->n_bricks, c->dirty, c->live, c->accum,
-                                                 c->vel, c->collider_set(), s, inv_frame_dt, c->fixed());
+                                                 c->vel, c->collider_set(), s, inv_frame_dt, c->fixed(), c->stage,
+                                                 c->slot);
     check_launch("k_grid_bricks");
   }
 }
@@ -1006,6 +1022,8 @@ SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
   d.active = c->active;
   d.n_active = c->n_active;
   d.dirty_list = c->dirty_list;
+  d.slot = c->slot;
+  d.stage = c->stage;
   d.fix_list = c->fix_list;
   d.n = (int32_t)c->n;
   d.n_bricks = c->n_bricks;
@@ -1412,6 +1430,8 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     c->offsets = dev_alloc<uint32_t>((size_t)c->n_cells + 1);
     c->active = dev_alloc<uint32_t>(bricks);
     c->dirty_list = dev_alloc<uint32_t>(bricks);
+    c->slot = dev_alloc<int32_t>(bricks);
+    HIP_CHECK(hipMemsetAsync(c->slot, 0xff, bricks * 4, c->stream));
     c->n_active = dev_alloc<uint32_t>(1);
     c->n_fix = dev_alloc<uint32_t>(1);
     HIP_CHECK(hipMemsetAsync(c->accum, 0, (size_t)c->n_nodes * sizeof(float4), c->stream));
@@ -1441,7 +1461,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
-                  (void*)c->active, (void*)c->dirty_list, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
+                  (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
                   (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris})
     dev_free(p);
   c->scratch.release();
@@ -1683,14 +1703,21 @@ int32_t mpm_get_grid(mpm_ctx* c, int32_t which, float* grid_v, float* grid_m) {
     require_idle(c);
     const int64_t total = (int64_t)c->geom.dims[0] * c->geom.dims[1] * c->geom.dims[2];
     float4* src = which == 0 ? c->accum : c->vel;
-    if (which == 0 && c->p2g_mode == MPM_P2G_FIXED) {
+    const bool resolve = which == 0 && c->p2g_mode != MPM_P2G_ATOMIC;
+    if (resolve) {
+      // the P2G output lives in the fixed-point accumulator, or in the accumulator plus the staged windows
       src = static_cast<float4*>(c->scratch.get((size_t)c->n_nodes * sizeof(float4) + total * 4 * sizeof(float)));
-      k_fixed_to_float<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->fx, c->n_nodes, 1.0 / c->fx_scale, src);
-      check_launch("k_fixed_to_float");
+      if (c->p2g_mode == MPM_P2G_FIXED) {
+        k_fixed_to_float<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->fx, c->n_nodes, 1.0 / c->fx_scale, src);
+        check_launch("k_fixed_to_float");
+      } else {
+        k_p2g_resolve<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->accum, c->n_nodes, c->stage,
+                                                                     c->slot, src);
+        check_launch("k_p2g_resolve");
+      }
     }
-    float* stage = which == 0 && c->p2g_mode == MPM_P2G_FIXED
-                       ? reinterpret_cast<float*>(src + c->n_nodes)
-                       : static_cast<float*>(c->scratch.get(total * 4 * sizeof(float)));
+    float* stage = resolve ? reinterpret_cast<float*>(src + c->n_nodes)
+                           : static_cast<float*>(c->scratch.get(total * 4 * sizeof(float)));
     k_grid_to_ref<<<grid_for(total), kBlock, 0, c->stream>>>(c->geom, src,
                                                              which == 0 ? nullptr : c->live, stage, stage + 3 * total);
     check_launch("k_grid_to_ref");

@@ -51,6 +51,8 @@ struct SceneDev {
   uint32_t* n_active;  // their count
   uint32_t* dirty_list;  // bricks P2G wrote this substep (grid update work list)
   uint32_t* fix_list;    // particles whose material update is redone in FP64 this substep
+  int32_t* slot;         // per brick: position in `active`, or -1
+  float4* stage;         // per occupied brick: its P2G window (8^3 nodes)
   int32_t n;           // particles
   int32_t n_bricks, n_cells, passes;
   int32_t sort_interval;
@@ -314,7 +316,12 @@ __global__ void __launch_bounds__(256) k_batch_cell_bounds(const SceneDev* __res
     S.offsets[c] = lo;
     if ((c & 63) == 0 && c < S.n_cells) {
       const uint32_t hi = lower_bound_u32(keys, S.n, (uint32_t)(c + 64));
-      if (hi > lo) S.active[atomicAdd(S.n_active, 1u)] = (uint32_t)(c >> 6);
+      int32_t sl = -1;
+      if (hi > lo) {
+        sl = (int32_t)atomicAdd(S.n_active, 1u);
+        S.active[sl] = (uint32_t)(c >> 6);
+      }
+      S.slot[c >> 6] = sl;
     }
   }
 }
@@ -393,7 +400,7 @@ __global__ void __launch_bounds__(256) k_batch_p2g(const SceneDev* __restrict__ 
     const int cur = st[i].cur;
     const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
     p2g_brick(S.set[cur], S.g, k, S.offsets, S.active[w - P0[i]], s_win[wave], S.accum, S.dirty, lane,
-              FixedAccum{nullptr, 0.0, 0.0});
+              FixedAccum{nullptr, 0.0, 0.0}, S.stage, w - P0[i]);
   }
 }
 
@@ -456,7 +463,7 @@ __global__ void __launch_bounds__(256) k_batch_grid(const SceneDev* __restrict__
     const SceneDev& S = sd[i];
     const int b = (int)S.dirty_list[w - P0[i]];
     grid_brick(S.g, S.sc, st[i].dt, b, S.accum, S.vel, S.cs, frames + i * kMaxColliders, lane,
-               FixedAccum{nullptr, 0.0, 0.0});
+               FixedAccum{nullptr, 0.0, 0.0}, S.stage, S.slot);
   }
 }
 

@@ -192,12 +192,18 @@ __global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict_
 }
 
 // bricks holding at least one particle are appended to `active` (their order there is irrelevant to
-// every result)
+// every result); slot[b] = the brick's position in that list, or -1
 __global__ void __launch_bounds__(256) k_active_bricks(const uint32_t* __restrict__ offsets, int n_bricks,
-                                                       uint32_t* __restrict__ active, uint32_t* __restrict__ n_active) {
+                                                       uint32_t* __restrict__ active, uint32_t* __restrict__ n_active,
+                                                       int32_t* __restrict__ slot) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n_bricks) return;
-  if (offsets[64 * (b + 1)] > offsets[64 * b]) active[atomicAdd(n_active, 1u)] = (uint32_t)b;
+  int32_t s = -1;
+  if (offsets[64 * (b + 1)] > offsets[64 * b]) {
+    s = (int32_t)atomicAdd(n_active, 1u);
+    active[s] = (uint32_t)b;
+  }
+  slot[b] = s;
 }
 
 // gather every per-slot particle array into sorted order: dst[i] = src[perm[i]]

@@ -158,7 +158,7 @@ __device__ __forceinline__ void wave_lds_fence() {
 __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g, float k,
                                           const uint32_t* __restrict__ cell_off, uint32_t b, float (*win)[kWinNodes],
                                           float4* __restrict__ accum, uint32_t* __restrict__ dirty, int lane,
-                                          const FixedAccum& fa) {
+                                          const FixedAccum& fa, float4* __restrict__ stage, uint32_t slot) {
   const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
   const int bz = (int)(b % (uint32_t)g.bdims[2]);
   const int by = (int)((b / (uint32_t)g.bdims[2]) % (uint32_t)g.bdims[1]);
@@ -232,7 +232,12 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
         }
         wave_lds_fence();
       }
-  for (int i = lane; i < kWinNodes; i += 64) {
+  if (stage && !fa.fx) {
+    // the window goes to the brick's staging slot with plain stores; the grid update sums the (up to 8)
+    // windows that cover each node, in a fixed order
+    float4* out = stage + (int64_t)slot * kWinNodes;
+    for (int i = lane; i < kWinNodes; i += 64) out[i] = make_float4(win[0][i], win[1][i], win[2][i], win[3][i]);
+  } else for (int i = lane; i < kWinNodes; i += 64) {
     const int gi = wo[0] + (i >> 6), gj = wo[1] + ((i >> 3) & 7), gk = wo[2] + (i & 7);
     if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
     const float vx = win[0][i], vy = win[1][i], vz = win[2][i], vm = win[3][i];
@@ -262,22 +267,64 @@ __global__ void __launch_bounds__(256) k_p2g_cells(Particles P, GridGeom g, floa
                                                    const uint32_t* __restrict__ active,
                                                    const uint32_t* __restrict__ n_active_ptr,
                                                    float4* __restrict__ accum, uint32_t* __restrict__ dirty,
-                                                   FixedAccum fa) {
+                                                   FixedAccum fa, float4* __restrict__ stage) {
   constexpr int kWaves = 4;
   __shared__ float s_win[kWaves][4][kWinNodes];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t n_active = *n_active_ptr;
   const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
   for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves)
-    p2g_brick(P, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane, fa);
+    p2g_brick(P, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane, fa, stage, w);
 }
 
 // grid update of one dirty brick by one wave (lane = node): momentum -> velocity, gravity, walls,
 // collider contact; clears the accumulator behind itself
+// P2G result of node (i, j, k): what reached the HBM accumulator (drifted particles) plus, when windows
+// are staged, the windows of the occupied bricks whose 8^3 window covers the node. Window B' spans nodes
+// 4B' - 1 .. 4B' + 6 per axis, so node 4B + r is covered by bricks B - 1 and B (r <= 2) or B and B + 1
+// (r = 3); the sum runs in that fixed order.
+__device__ __forceinline__ float4 staged_sum(const GridGeom& g, int i, int j, int k, float4 a,
+                                             const float4* __restrict__ stage, const int32_t* __restrict__ slot) {
+  const int n[3] = {i, j, k};
+  int b0[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) b0[ax] = (n[ax] >> 2) - ((n[ax] & 3) <= 2 ? 1 : 0);
+#pragma unroll
+  for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dz = 0; dz < 2; ++dz) {
+        const int bx = b0[0] + dx, by = b0[1] + dy, bz = b0[2] + dz;
+        if (bx < 0 || by < 0 || bz < 0 || bx >= g.bdims[0] || by >= g.bdims[1] || bz >= g.bdims[2]) continue;
+        const int32_t s = slot[(bx * g.bdims[1] + by) * g.bdims[2] + bz];
+        if (s < 0) continue;
+        const int li = ((i - 4 * bx + 1) * kWin + (j - 4 * by + 1)) * kWin + (k - 4 * bz + 1);
+        const float4 w = stage[(int64_t)s * kWinNodes + li];
+        a.x += w.x;
+        a.y += w.y;
+        a.z += w.z;
+        a.w += w.w;
+      }
+  return a;
+}
+
+// the P2G result of every node (accumulator plus staged windows) as float4, for the grid download
+__global__ void __launch_bounds__(256) k_p2g_resolve(GridGeom g, const float4* __restrict__ accum, int n_nodes,
+                                                     const float4* __restrict__ stage,
+                                                     const int32_t* __restrict__ slot, float4* __restrict__ out) {
+  const int addr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (addr >= n_nodes) return;
+  int i, j, k;
+  node_ijk(g, addr, i, j, k);
+  out[addr] = staged_sum(g, i, j, k, accum[addr], stage, slot);
+}
+
 __device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& sc, float dt, int brick,
                                            float4* __restrict__ accum, float4* __restrict__ vel,
                                            const ColliderSet& cs, const ColliderFrame* frames, int lane,
-                                           const FixedAccum& fa) {
+                                           const FixedAccum& fa, const float4* __restrict__ stage,
+                                           const int32_t* __restrict__ slot) {
   const int addr = brick * 64 + lane;
   int i, j, k;
   node_ijk(g, addr, i, j, k);
@@ -290,20 +337,25 @@ __device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& 
   } else {
     a = accum[addr];
   }
+  const float4 drift = a;
+  if (stage && !fa.fx) a = staged_sum(g, i, j, k, a, stage, slot);
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (i < g.dims[0] && j < g.dims[1] && k < g.dims[2]) {
     out = update_node(g, sc, dt, i, j, k, a);
     if (cs.count > 0 && out.w > 0.0f) collide_node_frames(cs, frames, g, i, j, k, out);
   }
   vel[addr] = out;
-  if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f || a.w != 0.0f) accum[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (drift.x != 0.0f || drift.y != 0.0f || drift.z != 0.0f || drift.w != 0.0f)
+    accum[addr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // one wave per brick; non-dirty bricks are skipped (and flagged stale for the grid download)
 __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, float dt, int n_bricks,
                                                      uint32_t* __restrict__ dirty, uint8_t* __restrict__ live,
                                                      float4* __restrict__ accum, float4* __restrict__ vel,
-                                                     ColliderSet cs, float s, float inv_frame_dt, FixedAccum fa) {
+                                                     ColliderSet cs, float s, float inv_frame_dt, FixedAccum fa,
+                                                     const float4* __restrict__ stage,
+                                                     const int32_t* __restrict__ slot) {
   __shared__ ColliderFrame frames[kMaxColliders];
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -312,7 +364,7 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
   const bool d = dirty[brick] != 0u;
   if (lane == 0) live[brick] = d ? 1 : 0;
   if (!d) return;
-  grid_brick(g, sc, dt, brick, accum, vel, cs, frames, lane, fa);
+  grid_brick(g, sc, dt, brick, accum, vel, cs, frames, lane, fa, stage, slot);
   if (lane == 0) dirty[brick] = 0u;
 }
 
