@@ -41,11 +41,14 @@ from mpm_core.geometry import mesh_is_closed
 from mpm_core.scene import SceneDescription, scene_hash
 from mpm_core.solver import ColliderGeometry, ParticleData
 
-# device bytes per particle slot (two particle sets of 31 float + 2 int planes, emitter ids, 5 sort
-# arrays) and per grid node (two float4 arrays, cell offsets), plus per brick (flags, occupied list)
-BYTES_PER_SLOT = 2 * 33 * 4 + 4 + 5 * 4
+# device bytes per particle slot (two particle sets of 32 float + 2 int planes, emitter ids, 5 sort
+# arrays, the FP64 work list, the drift-routing flag), per grid node (two float4 arrays, cell offsets),
+# per brick (flags, occupied list, slot map) and per occupied brick (its staged 8^3 P2G window and the
+# drift-routing counters and lists of its 64 cells); a scene holds min(bricks, particle slots) of those
+BYTES_PER_SLOT = 2 * 34 * 4 + 4 + 6 * 4 + 1
 BYTES_PER_NODE = 2 * 16 + 4
-BYTES_PER_BRICK = 4 + 1 + 4
+BYTES_PER_BRICK = 4 + 1 + 4 + 4
+BYTES_PER_OCCUPIED_BRICK = 512 * 16 + 64 * 4 + 64 * 4 * 4
 GROWTH = 1.5  # capacity grows by half when particles are added
 
 
@@ -73,7 +76,8 @@ def hbm_footprint(scene: SceneDescription) -> int:
         particles += per
     cells = float(np.prod(scene.domain.grid_shape))
     particles = min(particles, 2.0 * cells * scene.particles_per_cell)
-    return int(grid + BYTES_PER_SLOT * (GROWTH * particles + 256))
+    slots = GROWTH * particles + 256
+    return int(grid + BYTES_PER_SLOT * slots + BYTES_PER_OCCUPIED_BRICK * min(bricks, slots))
 
 
 @dataclass

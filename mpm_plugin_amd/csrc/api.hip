@@ -159,6 +159,9 @@ struct mpm_ctx {
   int32_t* slot = nullptr;         // per brick: its position in `active`, or -1
   float4* stage = nullptr;         // P2G windows of the occupied bricks (stage_cap x 512 nodes)
   int64_t stage_cap = 0;
+  uint32_t* in_cnt = nullptr;      // drift routing: per cell of an occupied brick (stage_cap x 64)
+  uint32_t* in_lst = nullptr;      // its incoming particles (stage_cap x 64 x kInPerCell)
+  uint8_t* fwd = nullptr;          // per particle: routed to its current cell (cap)
   uint32_t* fix_list = nullptr;    // particles deferred to the FP64 pass (cap entries)
   uint32_t* n_fix = nullptr;       // their count
   Scratch hist, scan_sums;
@@ -332,9 +335,10 @@ void free_particles(mpm_ctx* c) {
   dev_free(c->A.base);
   dev_free(c->B.base);
   for (void* p : {(void*)c->emitter, (void*)c->keys0, (void*)c->keys1, (void*)c->keys2, (void*)c->vals1,
-                  (void*)c->vals2, (void*)c->fix_list})
+                  (void*)c->vals2, (void*)c->fix_list, (void*)c->fwd})
     dev_free(p);
   c->fix_list = nullptr;
+  c->fwd = nullptr;
   c->A = ParticleSet{};
   c->B = ParticleSet{};
   c->emitter = nullptr;
@@ -359,6 +363,7 @@ void reserve(mpm_ctx* c, int64_t need) {
   uint32_t* v1 = dev_alloc<uint32_t>(cap);
   uint32_t* v2 = dev_alloc<uint32_t>(cap);
   uint32_t* fl = dev_alloc<uint32_t>(cap);
+  uint8_t* fw = dev_alloc<uint8_t>(cap);
   if (c->n) {
     for (int k = 0; k < kFloatPlanes; ++k)
       HIP_CHECK(hipMemcpyAsync(A.x() + k * cap, c->A.x() + k * c->cap, c->n * 4, hipMemcpyDeviceToDevice, c->stream));
@@ -379,13 +384,18 @@ void reserve(mpm_ctx* c, int64_t need) {
   c->vals1 = v1;
   c->vals2 = v2;
   c->fix_list = fl;
+  c->fwd = fw;
   c->cap = cap;
-  // one staged window per occupied brick: at most one brick per particle
+  // one staged window and one set of incoming lists per occupied brick: at most one brick per particle
   const int64_t stage_need = std::min<int64_t>(c->n_bricks, cap);
   if (stage_need > c->stage_cap) {
-    dev_free(c->stage);
+    for (void* p : {(void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst}) dev_free(p);
     c->stage = nullptr;
+    c->in_cnt = c->in_lst = nullptr;
     c->stage = dev_alloc<float4>((size_t)stage_need * kWinNodes);
+    c->in_cnt = dev_alloc<uint32_t>((size_t)stage_need * 64);
+    c->in_lst = dev_alloc<uint32_t>((size_t)stage_need * 64 * kInPerCell);
+    HIP_CHECK(hipMemsetAsync(c->in_cnt, 0, (size_t)stage_need * 64 * sizeof(uint32_t), c->stream));
     c->stage_cap = stage_need;
   }
   c->n = n;
@@ -500,6 +510,11 @@ void sort_particles(mpm_ctx* c) {
   c->steps_since_sort = 0;
 }
 
+// the cell keys of the last sort, in slot order: the output of its last radix pass
+const uint32_t* ctx_sorted_keys(const mpm_ctx* c) {
+  return ((c->key_bits + 7) / 8 - 1) % 2 == 0 ? c->keys1 : c->keys2;
+}
+
 // ---- substep phases --------------------------------------------------------------------------
 void run_p2g(mpm_ctx* c, float dt) {
   if (c->p2g_mode == MPM_P2G_ATOMIC) {
@@ -510,14 +525,24 @@ void run_p2g(mpm_ctx* c, float dt) {
     check_launch("k_p2g_atomic");
     return;
   }
-  if ((!c->sorted || c->steps_since_sort >= c->sort_interval)) sort_particles(c);
+  const bool sorting = !c->sorted || c->steps_since_sort >= c->sort_interval;
+  if (sorting) sort_particles(c);
   if (c->n == 0) return;
   Timed t(c, MPM_K_P2G);
+  // between sorts, drifted particles go to their current cell's lane (not in fixed point: the lists'
+  // order is that of atomics, and the fixed-point sums must not depend on it)
+  DriftLists dl{};
+  if (!sorting && c->p2g_mode == MPM_P2G_BINNED) {
+    k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)c->n, ctx_sorted_keys(c), c->slot,
+                                                      c->in_cnt, c->in_lst, c->fwd);
+    check_launch("k_drift");
+    dl = DriftLists{c->in_cnt, c->in_lst, c->fwd};
+  }
   // four waves (four bricks at a time) per workgroup, grid-stride over the device-side brick list
   int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 2048;
   blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, (std::min<int64_t>(c->n_bricks, c->n) + 3) / 4));
   k_p2g_cells<<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets, c->active,
-                                                       c->n_active, c->accum, c->dirty, c->fixed(), c->stage);
+                                                       c->n_active, c->accum, c->dirty, c->fixed(), c->stage, dl);
   check_launch("k_p2g_cells");
 }
 
@@ -961,6 +986,8 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   }
   k_batch_prefix<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, n, b->d_pre);
   check_launch("k_batch_prefix");
+  k_batch_drift<<<2048, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n, (int)b->g2p_tile);
+  check_launch("k_batch_drift");
   k_batch_p2g<<<batch_blocks(b->p2g_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
   check_launch("k_batch_p2g");
   k_batch_grid_scan<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
@@ -1024,6 +1051,9 @@ SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
   d.dirty_list = c->dirty_list;
   d.slot = c->slot;
   d.stage = c->stage;
+  d.in_cnt = c->in_cnt;
+  d.in_lst = c->in_lst;
+  d.fwd = c->fwd;
   d.fix_list = c->fix_list;
   d.n = (int32_t)c->n;
   d.n_bricks = c->n_bricks;
@@ -1461,7 +1491,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
-                  (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
+                  (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
                   (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris})
     dev_free(p);
   c->scratch.release();

@@ -21,6 +21,7 @@
 //   k_batch_plan            per scene: runs?, frame fraction s, sort now?, work prefixes, collider frames
 //   [k_batch_cell_keys, radix passes, k_batch_cell_bounds, k_batch_reorder]   only in the sorting graph
 //   k_batch_prefix          switches scenes that sorted to their reordered set, occupied-brick prefix
+//   k_batch_drift           scenes that did not sort: drifted particles to their current cell's lane
 //   k_batch_p2g, k_batch_grid_scan, k_batch_dirty_prefix, k_batch_grid, k_batch_g2p,
 //   [k_batch_fix_prefix, k_batch_fixup]   when a scene defers particles to the FP64 pass
 //   k_batch_advance
@@ -53,6 +54,9 @@ struct SceneDev {
   uint32_t* fix_list;    // particles whose material update is redone in FP64 this substep
   int32_t* slot;         // per brick: position in `active`, or -1
   float4* stage;         // per occupied brick: its P2G window (8^3 nodes)
+  uint32_t* in_cnt;      // drift routing (DriftLists): per cell of an occupied brick
+  uint32_t* in_lst;
+  uint8_t* fwd;          // per particle
   int32_t n;           // particles
   int32_t n_bricks, n_cells, passes;
   int32_t sort_interval;
@@ -385,6 +389,24 @@ __global__ void __launch_bounds__(kPlanThreads) k_batch_prefix(const SceneDev* _
   if (threadIdx.x == 0) P0[n_scenes] = carry;
 }
 
+// drift routing (k_drift) of the running scenes that did not sort this substep, over their G2P tiles
+__global__ void __launch_bounds__(256) k_batch_drift(const SceneDev* __restrict__ sd,
+                                                     const SceneStep* __restrict__ st,
+                                                     const uint32_t* __restrict__ pre, int n_scenes, int tile) {
+  const uint32_t* P0 = pre + kPreTiles * (n_scenes + 1);
+  const uint32_t total = P0[n_scenes];
+  int i = 0;
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    i = uniform(scene_of(P0, n_scenes, u, i));
+    if (st[i].sort_now) continue;
+    const SceneDev& S = sd[i];
+    const Particles P = S.set[st[i].cur];
+    const int end = min(S.n, (int)(u - P0[i] + 1) * tile);
+    for (int p = (int)(u - P0[i]) * tile + threadIdx.x; p < end; p += blockDim.x)
+      drift_route(P, S.g, p, sorted_keys(S), S.slot, S.in_cnt, S.in_lst, S.fwd);
+  }
+}
+
 // P2G over every running scene's occupied bricks, one wave per brick (p2g_brick)
 __global__ void __launch_bounds__(256) k_batch_p2g(const SceneDev* __restrict__ sd, const SceneStep* __restrict__ st,
                                                    const uint32_t* __restrict__ pre, int n_scenes) {
@@ -399,8 +421,10 @@ __global__ void __launch_bounds__(256) k_batch_p2g(const SceneDev* __restrict__ 
     const SceneDev& S = sd[i];
     const int cur = st[i].cur;
     const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
+    // a scene that sorted this substep has no drifted particles (and k_batch_drift skipped it)
+    const DriftLists dl = st[i].sort_now ? DriftLists{} : DriftLists{S.in_cnt, S.in_lst, S.fwd};
     p2g_brick(S.set[cur], S.g, k, S.offsets, S.active[w - P0[i]], s_win[wave], S.accum, S.dirty, lane,
-              FixedAccum{nullptr, 0.0, 0.0}, S.stage, w - P0[i]);
+              FixedAccum{nullptr, 0.0, 0.0}, S.stage, w - P0[i], dl);
   }
 }
 

@@ -144,6 +144,54 @@ __device__ __forceinline__ void p2g_scatter_atomic(const P2GParticle& q, const G
   }
 }
 
+// ---- drift routing --------------------------------------------------------------------------------
+// Between two sorts a particle stays in the slot of the cell it was sorted into, but its stencil follows
+// its position. Before P2G, k_drift hands every particle whose cell changed ("drifted") to the incoming
+// list of its current cell (kInPerCell entries per cell of an occupied brick, indexed by the brick's
+// slot), so the P2G lane that owns that cell sums it in registers like its own particles; `fwd` marks the
+// forwarded ones for the lane that holds them in sorted order, which then skips them. A drifted particle
+// whose cell is in an unoccupied brick, or whose cell's list is full, is left to the holder's atomic
+// scatter. P2G clears the counters it reads, so they start every substep at zero.
+constexpr int kInPerCell = 4;
+
+struct DriftLists {
+  uint32_t* cnt;  // per slot cell: particles routed to it (may exceed kInPerCell); null: no routing
+  uint32_t* lst;  // per slot cell: kInPerCell particle indices
+  const uint8_t* fwd;  // per particle: routed to its current cell's list
+};
+
+__device__ __forceinline__ void drift_route(const Particles& P, const GridGeom& g, int p,
+                                            const uint32_t* __restrict__ keys, const int32_t* __restrict__ slot,
+                                            uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ in_lst,
+                                            uint8_t* __restrict__ fwd) {
+  const float x[3] = {P.x[p], P.x[P.cap + p], P.x[2 * P.cap + p]};
+  int base[3];
+  float fx[3];
+  stencil_base(g, x, base, fx);
+  const uint32_t key = cell_key(g, base);
+  uint8_t routed = 0;
+  if (key != keys[p]) {
+    const int32_t s = slot[key >> 6];
+    if (s >= 0) {
+      const uint32_t c = (uint32_t)s * 64u + (key & 63u);
+      const uint32_t pos = atomicAdd(&in_cnt[c], 1u);
+      if (pos < (uint32_t)kInPerCell) {
+        in_lst[c * kInPerCell + pos] = (uint32_t)p;
+        routed = 1;
+      }
+    }
+  }
+  fwd[p] = routed;
+}
+
+// keys: the cell keys of the last sort, in slot order
+__global__ void __launch_bounds__(256) k_drift(Particles P, GridGeom g, int n, const uint32_t* __restrict__ keys,
+                                               const int32_t* __restrict__ slot, uint32_t* __restrict__ in_cnt,
+                                               uint32_t* __restrict__ in_lst, uint8_t* __restrict__ fwd) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) drift_route(P, g, p, keys, slot, in_cnt, in_lst, fwd);
+}
+
 // orders this wave's LDS accesses across lanes (a wave's LDS operations complete in issue order)
 __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -158,7 +206,8 @@ __device__ __forceinline__ void wave_lds_fence() {
 __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g, float k,
                                           const uint32_t* __restrict__ cell_off, uint32_t b, float (*win)[kWinNodes],
                                           float4* __restrict__ accum, uint32_t* __restrict__ dirty, int lane,
-                                          const FixedAccum& fa, float4* __restrict__ stage, uint32_t slot) {
+                                          const FixedAccum& fa, float4* __restrict__ stage, uint32_t slot,
+                                          const DriftLists& dl) {
   const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
   const int bz = (int)(b % (uint32_t)g.bdims[2]);
   const int by = (int)((b / (uint32_t)g.bdims[2]) % (uint32_t)g.bdims[1]);
@@ -173,6 +222,38 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
   wave_lds_fence();
   const int cb[3] = {4 * bx + cx, 4 * by + cy, 4 * bz + cz};
   const uint32_t p0 = cell_off[64 * b + lane], p1 = cell_off[64 * b + lane + 1];
+  // the lane's particles: its cell's sorted ones, then those routed to its cell since the last sort
+  const uint32_t own_n = p1 - p0;
+  uint32_t n_in = 0;
+  uint32_t* in_list = nullptr;
+  if (dl.cnt) {
+    const uint32_t c = slot * 64u + (uint32_t)lane;
+    const uint32_t m = dl.cnt[c];
+    n_in = m < (uint32_t)kInPerCell ? m : (uint32_t)kInPerCell;
+    in_list = dl.lst + (size_t)c * kInPerCell;
+    dl.cnt[c] = 0u;
+    if (n_in > 1) {
+      // in particle order, so the lane's sums do not depend on the order of the routing atomics
+      static_assert(kInPerCell == 4, "sorting network for 4 entries");
+      uint32_t e[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) e[j] = (uint32_t)j < n_in ? in_list[j] : 0xffffffffu;
+      const auto order = [](uint32_t& a, uint32_t& b) {
+        const uint32_t lo = a < b ? a : b;
+        b = a < b ? b : a;
+        a = lo;
+      };
+      order(e[0], e[1]);
+      order(e[2], e[3]);
+      order(e[0], e[2]);
+      order(e[1], e[3]);
+      order(e[1], e[2]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if ((uint32_t)j < n_in) in_list[j] = e[j];
+    }
+  }
+  const uint32_t total = own_n + n_in;
   float acc[3][3][3][4];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -184,10 +265,20 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
         for (int c = 0; c < 4; ++c) acc[i][j][kk][c] = 0.0f;
   // the next particle's inputs load while the current one is scattered
   P2GRaw next;
-  if (p0 < p1) load_p2g_raw(P, (int64_t)p0, next);
-  for (uint32_t p = p0; p < p1; ++p) {
+  bool next_fwd = false;
+  if (total > 0) {
+    const uint32_t pn = own_n > 0 ? p0 : in_list[0];
+    load_p2g_raw(P, (int64_t)pn, next);
+    if (dl.cnt && own_n > 0) next_fwd = dl.fwd[pn] != 0;
+  }
+  for (uint32_t t = 0; t < total; ++t) {
     const P2GRaw cur = next;
-    if (p + 1 < p1) load_p2g_raw(P, (int64_t)p + 1, next);
+    const bool cur_skip = t < own_n && next_fwd;  // routed to its current cell's lane
+    if (t + 1 < total) {
+      const uint32_t pn = t + 1 < own_n ? p0 + t + 1 : in_list[t + 1 - own_n];
+      load_p2g_raw(P, (int64_t)pn, next);
+      next_fwd = dl.cnt && t + 1 < own_n && dl.fwd[pn] != 0;
+    }
     P2GParticle q;
     make_p2g_particle(cur, g, k, q);
     if (q.base[0] == cb[0] && q.base[1] == cb[1] && q.base[2] == cb[2]) {
@@ -209,7 +300,7 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
           }
         }
       }
-    } else {
+    } else if (!cur_skip) {
       p2g_scatter_atomic(q, g, wo, win, accum, dirty, fa);
     }
   }
@@ -218,7 +309,7 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
   // offset, so the 27 read-add-writes must stay in program order across lanes: the fence after each
   // offset keeps the compiler from batching the reads of later offsets ahead of earlier writes (a
   // wave's LDS operations then complete in issue order). Lanes without particles add nothing.
-  const bool any = p1 > p0;
+  const bool any = total > 0;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -267,14 +358,14 @@ __global__ void __launch_bounds__(256) k_p2g_cells(Particles P, GridGeom g, floa
                                                    const uint32_t* __restrict__ active,
                                                    const uint32_t* __restrict__ n_active_ptr,
                                                    float4* __restrict__ accum, uint32_t* __restrict__ dirty,
-                                                   FixedAccum fa, float4* __restrict__ stage) {
+                                                   FixedAccum fa, float4* __restrict__ stage, DriftLists dl) {
   constexpr int kWaves = 4;
   __shared__ float s_win[kWaves][4][kWinNodes];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t n_active = *n_active_ptr;
   const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
   for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves)
-    p2g_brick(P, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane, fa, stage, w);
+    p2g_brick(P, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane, fa, stage, w, dl);
 }
 
 // grid update of one dirty brick by one wave (lane = node): momentum -> velocity, gravity, walls,
