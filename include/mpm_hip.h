@@ -142,7 +142,7 @@ int32_t mpm_debug_sort(mpm_ctx* ctx);
 
 /* ---- options ------------------------------------------------------------------------------ */
 #define MPM_OPT_P2G_MODE 0       /* MPM_P2G_BINNED (default), MPM_P2G_ATOMIC or MPM_P2G_FIXED */
-#define MPM_OPT_SORT_INTERVAL 1  /* substeps between brick re-sorts (default 8; >= 1) */
+#define MPM_OPT_SORT_INTERVAL 1  /* substeps between brick re-sorts (default 16; >= 1) */
 #define MPM_OPT_P2G_BLOCKS 2     /* workgroups of the binned P2G launch (0: automatic) */
 #define MPM_OPT_PRECISION 3      /* SVD, return maps and stress: MPM_PRECISION_FP32, _FALLBACK (default:
                                     FP64 only where FP32 cannot resolve F: sigma_min/sigma_max < 1e-2 or
@@ -165,8 +165,11 @@ int64_t mpm_get_option(const mpm_ctx* ctx, int32_t option);
 #define MPM_K_COLLIDERS 4
 #define MPM_K_OTHER 5
 #define MPM_K_COUNT 6
-/* record HIP events around every launch on the context's stream (off by default) */
-int32_t mpm_set_timing(mpm_ctx* ctx, int32_t enable);
+#define MPM_TIMING_ALL ((1 << MPM_K_COUNT) - 1)
+/* record HIP events around the launches of the kernel classes in `mask` (bit 1 << MPM_K_*, MPM_TIMING_ALL
+   for all, 0 off; off by default) on the context's stream. Each class brackets its own kernel only: P2G is
+   the P2G kernel (drift routing and the FP64 pass count as OTHER), G2P the G2P kernel. */
+int32_t mpm_set_timing(mpm_ctx* ctx, int32_t mask);
 /* accumulated device time and launch count of one kernel class since the last reset */
 int32_t mpm_get_kernel_time(mpm_ctx* ctx, int32_t kernel, double* total_ms, int64_t* launches);
 int32_t mpm_reset_timing(mpm_ctx* ctx);

@@ -301,8 +301,15 @@ class HipSolver(Solver):
     def get_option(self, option: int) -> int:
         return int(self._lib.mpm_get_option(self._ctx, int(option)))
 
-    def set_timing(self, enabled: bool) -> None:
-        N.check(self._lib.mpm_set_timing(self._ctx, int(bool(enabled))))
+    def set_timing(self, enabled: bool, phases=None) -> None:
+        """Bracket kernel launches with HIP events: every kernel class, or only those named in ``phases``
+        (names of ``N.KERNEL_NAMES``)."""
+        mask = 0
+        if enabled:
+            names = N.KERNEL_NAMES if phases is None else phases
+            for name in names:
+                mask |= 1 << N.KERNEL_NAMES.index(name)
+        N.check(self._lib.mpm_set_timing(self._ctx, mask))
 
     def reset_timing(self) -> None:
         N.check(self._lib.mpm_reset_timing(self._ctx))

@@ -85,7 +85,7 @@ struct Scratch {
 };
 
 struct Timing {
-  bool on = false;
+  uint32_t mask = 0;  // kernel classes (1 << MPM_K_*) bracketed with events
   struct Pending {
     int kernel;
     hipEvent_t a, b;
@@ -164,6 +164,7 @@ struct mpm_ctx {
   uint8_t* fwd = nullptr;          // per particle: routed to its current cell (cap)
   uint32_t* fix_list = nullptr;    // particles deferred to the FP64 pass (cap entries)
   uint32_t* n_fix = nullptr;       // their count
+  bool n_fix_cleared = false;      // the grid update since the last G2P cleared n_fix
   Scratch hist, scan_sums;
   bool sorted = false;
   int64_t steps_since_sort = 0;
@@ -173,7 +174,7 @@ struct mpm_ctx {
   int p2g_mode = 0;  // MPM_P2G_BINNED, MPM_P2G_ATOMIC or MPM_P2G_FIXED
   unsigned long long* fx = nullptr;  // fixed-point accumulator (MPM_P2G_FIXED only)
   double fx_scale = 0.0;
-  int64_t sort_interval = 8;
+  int64_t sort_interval = 16;
   int64_t p2g_blocks = 0;
 
   // colliders
@@ -184,7 +185,9 @@ struct mpm_ctx {
   TriNormals* d_normals = nullptr;
   uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
 
-  float* d_vmax = nullptr;        // reduced maximum speed
+  float* h_vmax = nullptr;        // reduced maximum speed, in pinned host memory the reduction writes to
+  float* h_vmax_dev = nullptr;    // its device address
+  bool vmax_pending = false;      // a reduction into h_vmax of the current state is on the stream
   float* d_block_max = nullptr;   // per-workgroup maxima of the last G2P (or speed pass)
   int64_t block_max_cap = 0;
   int64_t block_max_count = 0;
@@ -288,7 +291,7 @@ struct Timed {
   int kernel;
   hipEvent_t a = nullptr;
   Timed(mpm_ctx* ctx, int k) : c(ctx), kernel(k) {
-    if (c->timing.on && !c->capturing) {
+    if (((c->timing.mask >> kernel) & 1u) && !c->capturing) {
       a = pool_event(c->timing);
       HIP_CHECK(hipEventRecord(a, c->stream));
     }
@@ -446,6 +449,7 @@ void state_changed(mpm_ctx* c) {
   c->version += 1;
   c->sorted = false;
   c->vmax_valid = false;
+  c->vmax_pending = false;
 }
 
 // ---- device-wide exclusive scan --------------------------------------------------------------
@@ -528,17 +532,18 @@ void run_p2g(mpm_ctx* c, float dt) {
   const bool sorting = !c->sorted || c->steps_since_sort >= c->sort_interval;
   if (sorting) sort_particles(c);
   if (c->n == 0) return;
-  Timed t(c, MPM_K_P2G);
   // between sorts, drifted particles go to their current cell's lane (not in fixed point: the lists'
   // order is that of atomics, and the fixed-point sums must not depend on it)
   DriftLists dl{};
   if (!sorting && c->p2g_mode == MPM_P2G_BINNED) {
+    Timed t(c, MPM_K_OTHER);
     k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)c->n, ctx_sorted_keys(c), c->slot,
                                                       c->in_cnt, c->in_lst, c->fwd);
     check_launch("k_drift");
     dl = DriftLists{c->in_cnt, c->in_lst, c->fwd};
   }
   // four waves (four bricks at a time) per workgroup, grid-stride over the device-side brick list
+  Timed t(c, MPM_K_P2G);
   int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 2048;
   blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, (std::min<int64_t>(c->n_bricks, c->n) + 3) / 4));
   k_p2g_cells<<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets, c->active,
@@ -550,37 +555,44 @@ void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   Timed t(c, MPM_K_GRID);
   if (c->p2g_mode == MPM_P2G_ATOMIC) {
     k_grid_dense<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->sc, dt, c->n_nodes, c->accum, c->vel,
-                                                                  c->live, c->collider_set(), s, inv_frame_dt);
+                                                                  c->live, c->collider_set(), s, inv_frame_dt,
+                                                                  c->n_fix);
     check_launch("k_grid_dense");
   } else {
     const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
     k_grid_bricks<<<blocks, 256, 0, c->stream>>>(c->geom, c->sc, dt, c->n_bricks, c->dirty, c->live, c->accum,
                                                  c->vel, c->collider_set(), s, inv_frame_dt, c->fixed(), c->stage,
-                                                 c->slot);
+                                                 c->slot, c->n_fix);
     check_launch("k_grid_bricks");
   }
+  c->n_fix_cleared = true;
 }
 
 void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   if (c->n == 0) return;
-  Timed t(c, MPM_K_G2P);
   const bool fix = c->sc.precision != kPrecisionFp32;
-  if (fix) {
+  if (fix && !c->n_fix_cleared) {  // a G2P without the grid update before it (a debug hook)
     k_zero_u32<<<1, 64, 0, c->stream>>>(c->n_fix, 1);
     check_launch("k_zero_u32");
   }
-  k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel,
-                                                  block_max(c), c->collider_set(), s, inv_frame_dt,
-                                                  FixList{c->fix_list, c->n_fix});
-  check_launch("k_g2p");
+  c->n_fix_cleared = false;
+  {
+    Timed t(c, MPM_K_G2P);
+    k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel,
+                                                    block_max(c), c->collider_set(), s, inv_frame_dt,
+                                                    FixList{c->fix_list, c->n_fix});
+    check_launch("k_g2p");
+  }
   if (fix) {
     // a persistent grid: the deferred particles are usually few
+    Timed t(c, MPM_K_OTHER);
     k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 1024), kBlock, 0, c->stream>>>(
         c->parts(), c->d_mats, c->fix_list, c->n_fix);
     check_launch("k_fp64_fixup");
   }
   c->block_max_count = grid_for(c->n);
   c->vmax_valid = true;
+  c->vmax_pending = false;  // a reduction enqueued before this G2P is of the old state
 }
 
 void substep(mpm_ctx* c, double dt, double s, double frame_dt) {
@@ -606,15 +618,20 @@ void reduce_speed_async(mpm_ctx* c, float* out) {
   check_launch("k_reduce_max");
 }
 
+// Enqueue the reduction of the current state's maximum speed into pinned host memory, right behind the
+// work that produced the state, so that the next max_speed only waits for it (no launch, no copy, on the
+// path between two advances).
+void request_speed(mpm_ctx* c) {
+  if (c->n == 0 || c->vmax_pending) return;
+  Timed t(c, MPM_K_OTHER);
+  reduce_speed_async(c, c->h_vmax_dev);
+  c->vmax_pending = true;
+}
+
 float max_speed(mpm_ctx* c) {
-  {
-    Timed t(c, MPM_K_OTHER);
-    reduce_speed_async(c, c->d_vmax);
-  }
-  float f = 0.0f;
-  HIP_CHECK(hipMemcpyAsync(&f, c->d_vmax, sizeof f, hipMemcpyDeviceToHost, c->stream));
+  request_speed(c);
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  return f;
+  return *static_cast<volatile float*>(c->h_vmax);
 }
 
 void load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const float* Cm, const float* F,
@@ -1141,6 +1158,7 @@ void batch_release_scene(mpm_batch* b, int64_t i) {
   c->steps_since_sort = st.since_sort;
   c->last_perm = const_cast<uint32_t*>((radix_passes(c) - 1) % 2 == 0 ? c->vals1 : c->vals2);
   c->vmax_valid = false;  // the batch keeps the speed maximum; the context's partial maxima are stale
+  c->vmax_pending = false;
   c->version += 1;
   c->busy = false;
   b->rec[i] = scene_record(b, c);
@@ -1469,8 +1487,9 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     HIP_CHECK(hipMemsetAsync(c->dirty, 0, bricks * 4, c->stream));
     HIP_CHECK(hipMemsetAsync(c->live, 0, bricks, c->stream));
     HIP_CHECK(hipMemsetAsync(c->n_active, 0, 4, c->stream));
-    c->d_vmax = dev_alloc<float>(1);
-    HIP_CHECK(hipMemsetAsync(c->d_vmax, 0, 4, c->stream));
+    HIP_CHECK(hipHostMalloc((void**)&c->h_vmax, sizeof(float), hipHostMallocDefault));
+    *c->h_vmax = 0.0f;
+    HIP_CHECK(hipHostGetDevicePointer((void**)&c->h_vmax_dev, c->h_vmax, 0));
     reserve(c, 256);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -1491,7 +1510,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
-                  (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_vmax, (void*)c->d_block_max, (void*)c->d_cols,
+                  (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_block_max, (void*)c->d_cols,
                   (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris})
     dev_free(p);
   c->scratch.release();
@@ -1502,6 +1521,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->timing.pool) (void)hipEventDestroy(e);
+  if (c->h_vmax) (void)hipHostFree(c->h_vmax);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   (void)hipSetDevice(prev);
   delete c;
@@ -1657,6 +1677,7 @@ int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
     // stay in flight and the host prepares the next call meanwhile (every read of the state waits for them)
     const int64_t removed = remove_escaped(c);
     if (removed) refresh_present(c);
+    request_speed(c);  // the next advance's CFL step
     *out = mpm_advance_stats{steps, dt, removed};
   });
 }
@@ -1866,11 +1887,11 @@ int64_t mpm_get_option(const mpm_ctx* c, int32_t option) {
   }
 }
 
-int32_t mpm_set_timing(mpm_ctx* c, int32_t enable) {
+int32_t mpm_set_timing(mpm_ctx* c, int32_t mask) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
     collect_timing(c);
-    c->timing.on = enable != 0;
+    c->timing.mask = (uint32_t)mask & (uint32_t)MPM_TIMING_ALL;
   });
 }
 

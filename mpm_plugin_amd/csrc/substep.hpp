@@ -446,8 +446,11 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
                                                      float4* __restrict__ accum, float4* __restrict__ vel,
                                                      ColliderSet cs, float s, float inv_frame_dt, FixedAccum fa,
                                                      const float4* __restrict__ stage,
-                                                     const int32_t* __restrict__ slot) {
+                                                     const int32_t* __restrict__ slot,
+                                                     uint32_t* __restrict__ n_fix) {
   __shared__ ColliderFrame frames[kMaxColliders];
+  // the FP64 work list of the G2P that follows starts empty
+  if (n_fix && blockIdx.x == 0 && threadIdx.x == 0) *n_fix = 0u;
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (brick >= n_bricks) return;
@@ -463,8 +466,9 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
 __global__ void __launch_bounds__(256) k_grid_dense(GridGeom g, StepConsts sc, float dt, int n_nodes,
                                                     float4* __restrict__ accum, float4* __restrict__ vel,
                                                     uint8_t* __restrict__ live, ColliderSet cs, float s,
-                                                    float inv_frame_dt) {
+                                                    float inv_frame_dt, uint32_t* __restrict__ n_fix) {
   __shared__ ColliderFrame frames[kMaxColliders];
+  if (n_fix && blockIdx.x == 0 && threadIdx.x == 0) *n_fix = 0u;
   if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int addr = blockIdx.x * blockDim.x + threadIdx.x;
   if (addr >= n_nodes) return;
