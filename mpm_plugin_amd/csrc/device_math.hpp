@@ -13,8 +13,9 @@
 // The SVD (Warp's wp.svd3 in the reference) is written here from the published two-stage method
 // (McAdams et al. 2011, "Computing the SVD of 3x3 matrices with minimal branching..."): a cyclic Jacobi
 // eigen-decomposition of F^T F accumulated into V, column ordering of B = F V by decreasing norm, and
-// a Givens QR of B that yields U. Exact Jacobi rotation angles are used instead of the paper's
-// approximate ones. Convention: U and V are proper rotations, |sig_0| >= |sig_1| >= |sig_2|,
+// a Givens QR of B that yields U. The Jacobi angles follow the exact formula (not the paper's
+// approximate ones), on F^T F scaled to unit trace; in FP32 they are evaluated with the hardware
+// reciprocal and square-root instructions. Convention: U and V are proper rotations, |sig_0| >= |sig_1| >= |sig_2|,
 // sig_0, sig_1 >= 0 and sig_2 < 0 exactly when det F < 0.
 #pragma once
 
@@ -30,12 +31,31 @@ template <class T> struct M3 { T a[3][3]; };
 template <class T> __device__ __forceinline__ T sqrt_t(T v);
 template <> __device__ __forceinline__ float sqrt_t<float>(float v) { return __builtin_sqrtf(v); }
 template <> __device__ __forceinline__ double sqrt_t<double>(double v) { return __builtin_sqrt(v); }
+// Logarithm and exponential of the material models. FP32: the hardware base-2 instructions (v_log_f32,
+// v_exp_f32, about 1 ulp) scaled by ln 2; their arguments are clamped strains and stretches, far from
+// denormals. FP64: the libm functions.
 template <class T> __device__ __forceinline__ T log_t(T v);
-template <> __device__ __forceinline__ float log_t<float>(float v) { return logf(v); }
+template <> __device__ __forceinline__ float log_t<float>(float v) {
+  return __builtin_amdgcn_logf(v) * 0.693147180559945309f;
+}
 template <> __device__ __forceinline__ double log_t<double>(double v) { return log(v); }
 template <class T> __device__ __forceinline__ T exp_t(T v);
-template <> __device__ __forceinline__ float exp_t<float>(float v) { return expf(v); }
+template <> __device__ __forceinline__ float exp_t<float>(float v) {
+  return __builtin_amdgcn_exp2f(v * 1.44269504088896341f);
+}
 template <> __device__ __forceinline__ double exp_t<double>(double v) { return exp(v); }
+// Reciprocal, square root and reciprocal square root for rotation angles, where any value within a few
+// ulp serves (the rotation stays orthonormal because c = 1/sqrt(1 + t^2) and s = t c are formed from the
+// same t): FP32 uses the hardware instructions (v_rcp_f32, v_sqrt_f32, v_rsq_f32); FP64 is exact.
+template <class T> __device__ __forceinline__ T rcp_a(T v);
+template <> __device__ __forceinline__ float rcp_a<float>(float v) { return __builtin_amdgcn_rcpf(v); }
+template <> __device__ __forceinline__ double rcp_a<double>(double v) { return 1.0 / v; }
+template <class T> __device__ __forceinline__ T sqrt_a(T v);
+template <> __device__ __forceinline__ float sqrt_a<float>(float v) { return __builtin_amdgcn_sqrtf(v); }
+template <> __device__ __forceinline__ double sqrt_a<double>(double v) { return __builtin_sqrt(v); }
+template <class T> __device__ __forceinline__ T rsqrt_a(T v);
+template <> __device__ __forceinline__ float rsqrt_a<float>(float v) { return __builtin_amdgcn_rsqf(v); }
+template <> __device__ __forceinline__ double rsqrt_a<double>(double v) { return 1.0 / __builtin_sqrt(v); }
 template <class T> __device__ __forceinline__ T abs_t(T v) { return v < T(0) ? -v : v; }
 template <class T> __device__ __forceinline__ T max_t(T a, T b) { return a > b ? a : b; }
 template <class T> __device__ __forceinline__ T min_t(T a, T b) { return a < b ? a : b; }
@@ -67,15 +87,19 @@ template <class T> __device__ __forceinline__ T bspline_w(T fx, int i) {
 }
 
 // Symmetric Jacobi rotation that zeroes S[p][q]; S is updated in place (symmetric entries only) and the
-// rotation is accumulated into the columns p, q of V.
+// rotation is accumulated into the columns p, q of V. S is scaled to unit trace, so an off-diagonal entry
+// below 1e-30 is zero for every purpose (and keeps the reciprocal's argument normal); then t = 0 and the
+// rotation is the identity. Branch-free: every lane of a wave runs the same instructions.
 template <class T, int p, int q>
 __device__ __forceinline__ void jacobi_rotate(T (&S)[3][3], M3<T>& V) {
   constexpr int r = 3 - p - q;
   const T apq = S[p][q];
-  if (apq == T(0)) return;
-  const T tau = (S[q][q] - S[p][p]) / (T(2) * apq);
-  const T t = (tau >= T(0) ? T(1) : T(-1)) / (abs_t(tau) + sqrt_t(T(1) + tau * tau));
-  const T c = T(1) / sqrt_t(T(1) + t * t);
+  const bool skip = abs_t(apq) < T(1.0e-30);
+  const T tau = (S[q][q] - S[p][p]) * rcp_a(T(2) * (skip ? T(1) : apq));
+  // tau^2 may overflow to +inf: then t = 1 / inf = 0, the rotation angle of an already diagonal pair
+  T t = rcp_a(abs_t(tau) + sqrt_a(T(1) + tau * tau));
+  t = skip ? T(0) : (tau >= T(0) ? t : -t);
+  const T c = rsqrt_a(T(1) + t * t);
   const T s = t * c;
   S[p][p] -= t * apq;
   S[q][q] += t * apq;
@@ -105,8 +129,14 @@ __device__ __forceinline__ void svd3(const M3<T>& F, M3<T>& U, T (&sig)[3], M3<T
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) V.a[i][j] = i == j ? T(1) : T(0);
-  const T scale = S[0][0] + S[1][1] + S[2][2];
-  const T tol = scale * (sizeof(T) == 4 ? T(1e-8) : T(1e-17));
+  // unit trace (an all-zero or denormal F^T F stays zero: no rotations)
+  const T trace = S[0][0] + S[1][1] + S[2][2];
+  const T inv_trace = trace > T(1.0e-30) ? rcp_a(trace) : T(0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) S[i][j] *= inv_trace;
+  const T tol = sizeof(T) == 4 ? T(1e-8) : T(1e-17);
   for (int sw = 0; sw < SweepCount<T>::value; ++sw) {
     jacobi_rotate<T, 0, 1>(S, V);
     jacobi_rotate<T, 0, 2>(S, V);
@@ -140,13 +170,11 @@ __device__ __forceinline__ void svd3(const M3<T>& F, M3<T>& U, T (&sig)[3], M3<T
     const int p = g == 2 ? 1 : 0;
     const int q = g == 0 ? 1 : 2;
     const T a = B.a[p][p], b = B.a[q][p];
-    const T rr = sqrt_t(a * a + b * b);
-    T c = T(1), s = T(0);
-    if (rr > T(0)) {
-      const T inv = T(1) / rr;
-      c = a * inv;
-      s = b * inv;
-    }
+    const T n2 = a * a + b * b;
+    const bool rot = n2 > T(1.0e-30);
+    const T inv = rsqrt_a(rot ? n2 : T(1));
+    const T c = rot ? a * inv : T(1);
+    const T s = rot ? b * inv : T(0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const T bp = B.a[p][k], bq = B.a[q][k];
