@@ -551,18 +551,26 @@ void run_p2g(mpm_ctx* c, float dt) {
   check_launch("k_p2g_cells");
 }
 
+// the colliders' frames at frame fraction s (the same function the batched plan runs on the device)
+ColliderFrames frames_at(const mpm_ctx* c, float s, float inv_frame_dt) {
+  ColliderFrames fr{};
+  for (size_t i = 0; i < c->col_records.size() && i < (size_t)kMaxColliders; ++i)
+    collider_frame(c->col_records[i], s, inv_frame_dt, fr.f[i]);
+  return fr;
+}
+
 void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   Timed t(c, MPM_K_GRID);
   if (c->p2g_mode == MPM_P2G_ATOMIC) {
     k_grid_dense<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->sc, dt, c->n_nodes, c->accum, c->vel,
-                                                                  c->live, c->collider_set(), s, inv_frame_dt,
-                                                                  c->n_fix);
+                                                                  c->live, c->collider_set(),
+                                                                  frames_at(c, s, inv_frame_dt), c->n_fix);
     check_launch("k_grid_dense");
   } else {
     const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
     k_grid_bricks<<<blocks, 256, 0, c->stream>>>(c->geom, c->sc, dt, c->n_bricks, c->dirty, c->live, c->accum,
-                                                 c->vel, c->collider_set(), s, inv_frame_dt, c->fixed(), c->stage,
-                                                 c->slot, c->n_fix);
+                                                 c->vel, c->collider_set(), frames_at(c, s, inv_frame_dt),
+                                                 c->fixed(), c->stage, c->slot, c->n_fix);
     check_launch("k_grid_bricks");
   }
   c->n_fix_cleared = true;
@@ -579,7 +587,7 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   {
     Timed t(c, MPM_K_G2P);
     k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel,
-                                                    block_max(c), c->collider_set(), s, inv_frame_dt,
+                                                    block_max(c), c->collider_set(), frames_at(c, s, inv_frame_dt),
                                                     FixList{c->fix_list, c->n_fix});
     check_launch("k_g2p");
   }

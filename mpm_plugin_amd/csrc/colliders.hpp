@@ -66,7 +66,13 @@ struct ColliderFrame {
   float dA[9], dt[3];      // velocity matrix
 };
 
-__device__ __forceinline__ void quat_rot(const float (&q)[4], float (&R)[9]) {
+// every collider's frame of one substep, computed once on the host and passed by value to the kernels
+// (they read it through the scalar cache; no workgroup recomputes it)
+struct ColliderFrames {
+  ColliderFrame f[kMaxColliders];
+};
+
+__host__ __device__ inline void quat_rot(const float (&q)[4], float (&R)[9]) {
   const float x = q[0], y = q[1], z = q[2], w = q[3];
   R[0] = 1.0f - 2.0f * (y * y + z * z); R[1] = 2.0f * (x * y - z * w); R[2] = 2.0f * (x * z + y * w);
   R[3] = 2.0f * (x * y + z * w); R[4] = 1.0f - 2.0f * (x * x + z * z); R[5] = 2.0f * (y * z - x * w);
@@ -74,7 +80,7 @@ __device__ __forceinline__ void quat_rot(const float (&q)[4], float (&R)[9]) {
 }
 
 // pose at s (colliders.py:30-37): slerp of the rotation, lerp of stretch and translation
-__device__ void collider_matrix(const ColliderDev& c, float s, float (&A)[9], float (&t)[3]) {
+__host__ __device__ inline void collider_matrix(const ColliderDev& c, float s, float (&A)[9], float (&t)[3]) {
   float d = c.q0[0] * c.q1[0] + c.q0[1] * c.q1[1] + c.q0[2] * c.q1[2] + c.q0[3] * c.q1[3];
   d = fminf(fmaxf(d, -1.0f), 1.0f);
   const float ang = acosf(d);
@@ -109,7 +115,7 @@ __device__ void collider_matrix(const ColliderDev& c, float s, float (&A)[9], fl
   for (int i = 0; i < 3; ++i) t[i] = c.t0[i] + s * (c.t1[i] - c.t0[i]);
 }
 
-__device__ void collider_frame(const ColliderDev& c, float s, float inv_frame_dt, ColliderFrame& f) {
+__host__ __device__ inline void collider_frame(const ColliderDev& c, float s, float inv_frame_dt, ColliderFrame& f) {
   collider_matrix(c, s, f.A, f.t);
   const float* a = f.A;
   const float c00 = a[4] * a[8] - a[5] * a[7];
@@ -253,12 +259,6 @@ __device__ __forceinline__ void collider_velocity(const ColliderFrame& f, const 
   float pl[3];
   affine_point(f.Ai, f.ti, p, pl);
   affine_point(f.dA, f.dt, pl, vc);
-}
-
-// all collider frames of this substep into LDS (thread 0 of the block), then a barrier
-__device__ __forceinline__ void load_frames(const ColliderSet& cs, float s, float inv_frame_dt, ColliderFrame* sf) {
-  if (threadIdx.x < cs.count) collider_frame(cs.cols[threadIdx.x], s, inv_frame_dt, sf[threadIdx.x]);
-  __syncthreads();
 }
 
 // grid_colliders on one node with mass (colliders.py:95-117)

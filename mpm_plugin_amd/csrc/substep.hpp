@@ -444,14 +444,13 @@ __device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& 
 __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, float dt, int n_bricks,
                                                      uint32_t* __restrict__ dirty, uint8_t* __restrict__ live,
                                                      float4* __restrict__ accum, float4* __restrict__ vel,
-                                                     ColliderSet cs, float s, float inv_frame_dt, FixedAccum fa,
+                                                     ColliderSet cs, ColliderFrames fr, FixedAccum fa,
                                                      const float4* __restrict__ stage,
                                                      const int32_t* __restrict__ slot,
                                                      uint32_t* __restrict__ n_fix) {
-  __shared__ ColliderFrame frames[kMaxColliders];
+  const ColliderFrame* frames = fr.f;
   // the FP64 work list of the G2P that follows starts empty
   if (n_fix && blockIdx.x == 0 && threadIdx.x == 0) *n_fix = 0u;
-  if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (brick >= n_bricks) return;
   const int lane = threadIdx.x & 63;
@@ -465,11 +464,10 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
 // dense variant for the reference-order P2G (which flags no bricks): every node of the grid
 __global__ void __launch_bounds__(256) k_grid_dense(GridGeom g, StepConsts sc, float dt, int n_nodes,
                                                     float4* __restrict__ accum, float4* __restrict__ vel,
-                                                    uint8_t* __restrict__ live, ColliderSet cs, float s,
-                                                    float inv_frame_dt, uint32_t* __restrict__ n_fix) {
-  __shared__ ColliderFrame frames[kMaxColliders];
+                                                    uint8_t* __restrict__ live, ColliderSet cs, ColliderFrames fr,
+                                                    uint32_t* __restrict__ n_fix) {
+  const ColliderFrame* frames = fr.f;
   if (n_fix && blockIdx.x == 0 && threadIdx.x == 0) *n_fix = 0u;
-  if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
   const int addr = blockIdx.x * blockDim.x + threadIdx.x;
   if (addr >= n_nodes) return;
   if ((addr & 63) == 0) live[addr >> 6] = 1;
@@ -663,9 +661,8 @@ __device__ __forceinline__ void block_max_speed(float speed, float* __restrict__
 __global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restrict__ mats, GridGeom g,
                                              StepConsts sc, int n, float dt, const float4* __restrict__ vel,
                                              float* __restrict__ block_max,
-                                             ColliderSet cs, float s, float inv_frame_dt, FixList fix) {
-  __shared__ ColliderFrame frames[kMaxColliders];
-  if (cs.count > 0) load_frames(cs, s, inv_frame_dt, frames);
+                                             ColliderSet cs, ColliderFrames fr, FixList fix) {
+  const ColliderFrame* frames = fr.f;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float speed = 0.0f;
   bool defer = false;
