@@ -7,7 +7,8 @@
 // Substep on the default (binned) path:
 //   [every sort_interval substeps, or after any state change]
 //     k_cell_keys -> radix passes (k_radix_hist, scan, k_radix_scatter) -> k_cell_bounds
-//     -> k_active_bricks -> k_reorder
+//     -> k_brick_cost, k_brick_order (occupied bricks, costliest first) -> k_reorder
+//   [between sorts] k_drift (particles that changed cell -> incoming lists of their current cell)
 //   k_p2g_cells    (one wave per occupied brick, one lane per cell, wave-private LDS window)
 //   k_grid_bricks  (dirty bricks: update, walls, colliders)
 //   k_g2p          (gather, advection, collider projection, return map, stress, max speed)
@@ -154,7 +155,8 @@ struct mpm_ctx {
 
   // binning
   uint32_t *keys0 = nullptr, *keys1 = nullptr, *keys2 = nullptr, *vals1 = nullptr, *vals2 = nullptr;
-  uint32_t *offsets = nullptr, *active = nullptr, *n_active = nullptr;  // offsets: per cell
+  uint32_t *offsets = nullptr, *active = nullptr;  // offsets: per cell; active: occupied bricks, costliest first
+  uint32_t* n_active = nullptr;  // their count
   uint32_t* dirty_list = nullptr;  // batched grid update work list
   int32_t* slot = nullptr;         // per brick: its position in `active`, or -1
   float4* stage = nullptr;         // P2G windows of the occupied bricks (stage_cap x 512 nodes)
@@ -468,6 +470,14 @@ void scan_u32(Scratch& scan_sums, hipStream_t stream, const uint32_t* in, uint32
 }
 
 // ---- binning ---------------------------------------------------------------------------------
+// the occupied-brick list (costliest bricks first) and the brick -> slot map, from the cell offsets
+void active_bricks(mpm_ctx* c) {
+  k_brick_cost<<<(unsigned)((c->n_bricks + 3) / 4), 256, 0, c->stream>>>(c->offsets, c->n_bricks, c->slot);
+  check_launch("k_brick_cost");
+  k_brick_order<<<1, kOrderThreads, 0, c->stream>>>(c->n_bricks, c->active, c->n_active, c->slot);
+  check_launch("k_brick_order");
+}
+
 void sort_particles(mpm_ctx* c) {
   const int64_t n = c->n;
   Timed t(c, MPM_K_BIN);
@@ -476,9 +486,7 @@ void sort_particles(mpm_ctx* c) {
   if (n == 0) {
     k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(c->keys1, 0, c->n_cells, c->offsets);
     check_launch("k_cell_bounds");
-    k_active_bricks<<<grid_for(c->n_bricks), kBlock, 0, c->stream>>>(c->offsets, c->n_bricks, c->active,
-                                                                     c->n_active, c->slot);
-    check_launch("k_active_bricks");
+    active_bricks(c);
     c->sorted = true;
     c->steps_since_sort = 0;
     return;
@@ -503,9 +511,7 @@ void sort_particles(mpm_ctx* c) {
   }
   k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(kin, (int)n, c->n_cells, c->offsets);
   check_launch("k_cell_bounds");
-  k_active_bricks<<<grid_for(c->n_bricks), kBlock, 0, c->stream>>>(c->offsets, c->n_bricks, c->active, c->n_active,
-                                                                   c->slot);
-  check_launch("k_active_bricks");
+  active_bricks(c);
   k_reorder<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.view(), c->B.view(), vin, (int)n);
   check_launch("k_reorder");
   std::swap(c->A, c->B);
@@ -542,11 +548,12 @@ void run_p2g(mpm_ctx* c, float dt) {
     check_launch("k_drift");
     dl = DriftLists{c->in_cnt, c->in_lst, c->fwd};
   }
-  // four waves (four bricks at a time) per workgroup, grid-stride over the device-side brick list
+  // one wave per workgroup, so a brick starts as soon as any wave slot frees; grid-stride over the
+  // device-side brick list (costliest first)
   Timed t(c, MPM_K_P2G);
-  int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 2048;
-  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, (std::min<int64_t>(c->n_bricks, c->n) + 3) / 4));
-  k_p2g_cells<<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets, c->active,
+  int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 16384;
+  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, std::min<int64_t>(c->n_bricks, c->n)));
+  k_p2g_cells<1><<<(unsigned)blocks, 64, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets, c->active,
                                                        c->n_active, c->accum, c->dirty, c->fixed(), c->stage, dl);
   check_launch("k_p2g_cells");
 }

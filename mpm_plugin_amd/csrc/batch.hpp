@@ -304,7 +304,7 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__
 }
 
 // offsets[c] for c in [0, n_cells] (k_cell_bounds); the thread of a brick's first cell also appends the
-// brick to the occupied list when its 64 cells hold a particle (k_active_bricks)
+// brick to the occupied list when its 64 cells hold a particle
 __global__ void __launch_bounds__(256) k_batch_cell_bounds(const SceneDev* __restrict__ sd,
                                                            const uint32_t* __restrict__ pre, int n_scenes) {
   const uint32_t* P0 = pre + kPreCellBlocks * (n_scenes + 1);

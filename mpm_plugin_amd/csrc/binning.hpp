@@ -191,19 +191,59 @@ __global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict_
   offsets[c] = (uint32_t)lo;
 }
 
-// bricks holding at least one particle are appended to `active` (their order there is irrelevant to
-// every result); slot[b] = the brick's position in that list, or -1
-__global__ void __launch_bounds__(256) k_active_bricks(const uint32_t* __restrict__ offsets, int n_bricks,
-                                                       uint32_t* __restrict__ active, uint32_t* __restrict__ n_active,
-                                                       int32_t* __restrict__ slot) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// Occupied-brick list in decreasing order of work. A P2G wave takes one brick and its lanes walk their
+// cells' particles in lockstep, so a brick costs in proportion to its fullest cell; the waves of the
+// costliest bricks start first and the cheap ones fill the machine behind them (longest-processing-time
+// order). Cost class q = 15 - min(fullest cell, 15): class 0 first.
+constexpr int kCostClasses = 16;
+
+// pass 1, one wave per brick (lane = cell): the cost class into slot[b] (-1: no particles)
+__global__ void __launch_bounds__(256) k_brick_cost(const uint32_t* __restrict__ offsets, int n_bricks,
+                                                    int32_t* __restrict__ slot) {
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (b >= n_bricks) return;
-  int32_t s = -1;
-  if (offsets[64 * (b + 1)] > offsets[64 * b]) {
-    s = (int32_t)atomicAdd(n_active, 1u);
-    active[s] = (uint32_t)b;
+  const uint32_t* o = offsets + 64 * (size_t)b;
+  uint32_t most = o[lane + 1] - o[lane];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) most = max(most, (uint32_t)__shfl_xor((int)most, d));
+  if (lane == 0)
+    slot[b] = most == 0 ? -1 : kCostClasses - 1 - (int)min(most, (uint32_t)(kCostClasses - 1));
+}
+
+// pass 2, one workgroup: bricks per class (LDS counters: the few classes would serialise global atomics),
+// class offsets, then each occupied brick's position = its class offset + its rank within the class;
+// writes active[], slot[] (brick -> position) and the count
+constexpr int kOrderThreads = 1024;
+__global__ void __launch_bounds__(kOrderThreads) k_brick_order(int n_bricks, uint32_t* __restrict__ active,
+                                                              uint32_t* __restrict__ n_active,
+                                                              int32_t* __restrict__ slot) {
+  __shared__ uint32_t count[kCostClasses], start[kCostClasses];
+  if (threadIdx.x < kCostClasses) count[threadIdx.x] = 0u;
+  __syncthreads();
+#pragma unroll 4
+  for (int b = threadIdx.x; b < n_bricks; b += kOrderThreads) {
+    const int32_t q = slot[b];
+    if (q >= 0) atomicAdd(&count[q], 1u);
   }
-  slot[b] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int q = 0; q < kCostClasses; ++q) {
+      start[q] = acc;
+      acc += count[q];
+      count[q] = 0u;
+    }
+    *n_active = acc;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int b = threadIdx.x; b < n_bricks; b += kOrderThreads) {
+    const int32_t q = slot[b];
+    if (q < 0) continue;
+    const uint32_t pos = start[q] + atomicAdd(&count[q], 1u);
+    active[pos] = (uint32_t)b;
+    slot[b] = (int32_t)pos;
+  }
 }
 
 // gather every per-slot particle array into sorted order: dst[i] = src[perm[i]]

@@ -351,15 +351,15 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
   wave_lds_fence();
 }
 
-// P2G over the device-side list of occupied bricks: four waves (four bricks at a time) per workgroup,
+// P2G over the device-side list of occupied bricks: kWaves waves (bricks at a time) per workgroup,
 // grid-stride, so the launch needs no host-side count
-__global__ void __launch_bounds__(256) k_p2g_cells(Particles P, GridGeom g, float p_vol, float dt,
+template <int kWaves>
+__global__ void __launch_bounds__(64 * kWaves) k_p2g_cells(Particles P, GridGeom g, float p_vol, float dt,
                                                    const uint32_t* __restrict__ cell_off,
                                                    const uint32_t* __restrict__ active,
                                                    const uint32_t* __restrict__ n_active_ptr,
                                                    float4* __restrict__ accum, uint32_t* __restrict__ dirty,
                                                    FixedAccum fa, float4* __restrict__ stage, DriftLists dl) {
-  constexpr int kWaves = 4;
   __shared__ float s_win[kWaves][4][kWinNodes];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t n_active = *n_active_ptr;
