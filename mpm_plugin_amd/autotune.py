@@ -38,11 +38,11 @@ KNOB_NAMES = {N.BATCH_OPT_G2P_TILE: "g2p_tile", N.BATCH_OPT_G2P_BLOCKS: "g2p_blo
               N.BATCH_OPT_SORT_INTERVAL: "sort_interval"}
 
 
-def coordinate_descent(space: dict[int, Sequence[int]], measure: Callable[[dict[int, int]], float],
+def coordinate_descent(space: dict[int, Sequence[int]], measure: Callable[..., float],
                        repeats: int = 2) -> tuple[dict[int, int], float]:
-    """Maximise measure(config) one knob at a time, starting from each knob's first value. The candidates
-    of one knob are measured in interleaved rounds (v1 v2 v3 v1 v2 v3 ...) and each keeps its best, so a
-    measure that drifts with time (scenes finishing their frames) treats them alike."""
+    """Maximise measure(config, knob) one knob at a time, starting from each knob's first value. The
+    candidates of one knob are measured in interleaved rounds (v1 v2 v3 v1 v2 v3 ...) and each keeps its
+    best, so a measure that drifts with time (scenes finishing their frames) treats them alike."""
     best = {k: v[0] for k, v in space.items()}
     best_score = -math.inf
     for knob, values in space.items():
@@ -51,7 +51,7 @@ def coordinate_descent(space: dict[int, Sequence[int]], measure: Callable[[dict[
             for v in values:
                 trial = dict(best)
                 trial[knob] = v
-                scores[v] = max(scores[v], measure(trial))
+                scores[v] = max(scores[v], measure(trial, knob))
         pick = max(values, key=lambda v: scores[v])
         best[knob] = pick
         best_score = scores[pick]
@@ -113,13 +113,16 @@ def apply(batch, config: dict[int, int]) -> None:
         batch.set_option(knob, value)
 
 
-def tune_batch(batch, frame_dt, launches: int = 12, repeats: int = 2, space: dict[int, Sequence[int]] | None = None,
+def tune_batch(batch, frame_dt, launches: int = 16, repeats: int = 2, space: dict[int, Sequence[int]] | None = None,
                use_cache: bool = True) -> dict:
     """Pick the batch's launch shapes by measurement; applies and returns them.
 
     Every member must be between frames. One frame of every member is started and run to its end;
     candidate shapes are timed on its substeps (a frame shorter than the measurements needs is followed
-    by further frames of the same length)."""
+    by further frames of the same length). A sort-cadence candidate is timed over two cadences of the
+    longest candidate, so that every candidate's window holds its sorts (a window without one would favour
+    the longest cadence); the result replaces the library's defaults only if it beats them over such a
+    window too."""
     space = dict(space or BATCH_SPACE)
     key = f"{_device_name()}|{signature(batch)}"
     if use_cache:
@@ -131,8 +134,12 @@ def tune_batch(batch, frame_dt, launches: int = 12, repeats: int = 2, space: dic
     counts = [s.particle_count for s in batch.scenes]
     in_frame = [False]
 
-    def measure(config: dict[int, int]) -> float:
-        """Particle-substeps per second over `launches` substep launches with this config."""
+    sort_values = space.get(N.BATCH_OPT_SORT_INTERVAL, [0])
+    long_window = max(launches, 2 * max(sort_values))
+
+    def measure(config: dict[int, int], knob: int | None = None) -> float:
+        """Particle-substeps per second over the substep launches of one window with this config."""
+        window = long_window if knob in (N.BATCH_OPT_SORT_INTERVAL, None) else launches
         apply(batch, config)
         if not in_frame[0]:
             batch.begin(frame_dt)
@@ -143,7 +150,7 @@ def tune_batch(batch, frame_dt, launches: int = 12, repeats: int = 2, space: dic
             batch.begin(frame_dt)
             done0, _ = batch.progress()
         t0 = time.perf_counter()
-        batch.run(launches)
+        batch.run(window)
         dt = time.perf_counter() - t0
         done1, total = batch.progress()
         work = sum(counts[i] * (int(done1[i]) - int(done0[i])) for i in range(n))
@@ -153,6 +160,15 @@ def tune_batch(batch, frame_dt, launches: int = 12, repeats: int = 2, space: dic
 
     t_start = time.perf_counter()
     config, score = coordinate_descent(space, measure, repeats)
+    default = {k: v[0] for k, v in space.items()}
+    if config != default:
+        # interleaved final comparison with the defaults over the long window
+        best_default = best_tuned = -math.inf
+        for _ in range(repeats):
+            best_default = max(best_default, measure(default))
+            best_tuned = max(best_tuned, measure(config))
+        if best_tuned < 1.02 * best_default:
+            config, score = default, best_default
     apply(batch, config)
     if in_frame[0]:
         batch.run(0)  # finish the frame in flight

@@ -113,7 +113,10 @@ struct ParticleSet {
   float4* rec() const { return reinterpret_cast<float4*>(x() + kFloatPlanes * cap); }
   int32_t* mat() const { return reinterpret_cast<int32_t*>(x() + (kFloatPlanes + kRecFloats) * cap); }
   int32_t* pid() const { return mat() + cap; }
-  Particles view() const { return Particles{x(), v(), F(), Jp(), mat(), pid(), rec(), cap}; }
+  Particles view() const {
+    return Particles{as_global(x()), as_global(v()), as_global(F()), as_global(Jp()), as_global(mat()),
+                     as_global(pid()), as_global(rec()), cap};
+  }
   static size_t bytes(int64_t cap) { return (size_t)cap * 4 * (kFloatPlanes + kRecFloats + kIntPlanes); }
 };
 
@@ -206,11 +209,11 @@ struct mpm_ctx {
   ColliderSet collider_set() const {
     ColliderSet cs{};
     cs.count = (int)colliders.size();
-    cs.cols = d_cols;
-    cs.tris = d_tris;
-    cs.normals = d_normals;
-    cs.cell_start = d_cell_start;
-    cs.cell_tris = d_cell_tris;
+    cs.cols = as_global(d_cols);
+    cs.tris = as_global(d_tris);
+    cs.normals = as_global(d_normals);
+    cs.cell_start = as_global(d_cell_start);
+    cs.cell_tris = as_global(d_cell_tris);
     cs.thickness = (float)(2.0 * dom.dx_f64);
     cs.band = (float)dom.dx_f64;
     return cs;
@@ -242,6 +245,7 @@ struct mpm_batch {
   ColliderFrame* d_frames = nullptr;
   uint32_t* d_pre = nullptr;  // work-unit prefixes (kPrefixKinds x (scenes + 1))
   float* d_speed = nullptr;
+  uint32_t* d_speed_slots = nullptr;  // kSpeedSlots per scene: a substep's G2P speed maxima
   SceneDev* h_scenes = nullptr;
   SceneStep* h_steps = nullptr;
   float* h_speed = nullptr;
@@ -922,7 +926,8 @@ void batch_drop_graphs(mpm_batch* b) {
 }
 
 void batch_free_tables(mpm_batch* b) {
-  for (void* p : {(void*)b->d_scenes, (void*)b->d_steps, (void*)b->d_frames, (void*)b->d_pre, (void*)b->d_speed})
+  for (void* p : {(void*)b->d_scenes, (void*)b->d_steps, (void*)b->d_frames, (void*)b->d_pre, (void*)b->d_speed,
+                  (void*)b->d_speed_slots})
     dev_free(p);
   for (void* p : {(void*)b->h_scenes, (void*)b->h_steps, (void*)b->h_speed})
     if (p) (void)hipHostFree(p);
@@ -931,6 +936,7 @@ void batch_free_tables(mpm_batch* b) {
   b->d_frames = nullptr;
   b->d_pre = nullptr;
   b->d_speed = nullptr;
+  b->d_speed_slots = nullptr;
   b->h_scenes = nullptr;
   b->h_steps = nullptr;
   b->h_speed = nullptr;
@@ -958,6 +964,7 @@ void batch_reserve_tables(mpm_batch* b, int64_t n) {
   b->d_frames = dev_alloc<ColliderFrame>(cap * kMaxColliders);
   b->d_pre = dev_alloc<uint32_t>((size_t)kPrefixKinds * (cap + 1));
   b->d_speed = dev_alloc<float>(cap);
+  b->d_speed_slots = dev_alloc<uint32_t>((size_t)cap * kSpeedSlots);
   b->h_scenes = host_alloc<SceneDev>(cap);
   b->h_steps = host_alloc<SceneStep>(cap);
   b->h_speed = host_alloc<float>(cap);
@@ -998,7 +1005,7 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   hipStream_t s = b->stream;
   // persistent grids: a few workgroups per CU, striding over the running scenes' work
   k_batch_plan<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, b->d_frames, n, sorting ? 1 : 0,
-                                          (int)b->g2p_tile, b->d_pre);
+                                          (int)b->g2p_tile, b->d_pre, b->d_speed_slots);
   check_launch("k_batch_plan");
   if (sorting) {
     k_batch_cell_keys<<<2048, kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
@@ -1030,7 +1037,8 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
                                                                   b->d_frames);
   check_launch("k_batch_grid");
   k_batch_g2p<<<batch_blocks(b->g2p_blocks, 2048), (unsigned)b->g2p_tile, 0, s>>>(b->d_scenes, b->d_steps,
-                                                                                  b->d_pre, n, b->d_frames);
+                                                                                  b->d_pre, n, b->d_frames,
+                                                                                  b->d_speed_slots);
   check_launch("k_batch_g2p");
   if (b->fp64_pass) {
     k_batch_fix_prefix<<<1, kPlanThreads, 0, s>>>(b->d_steps, n, b->d_pre);
@@ -1038,7 +1046,7 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
     k_batch_fixup<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
     check_launch("k_batch_fixup");
   }
-  k_batch_advance<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(b->d_steps, n);
+  k_batch_advance<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(b->d_steps, n, b->d_speed_slots);
   check_launch("k_batch_advance");
 }
 
@@ -1064,29 +1072,29 @@ SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
   SceneDev d{};
   d.set[0] = c->A.view();
   d.set[1] = c->B.view();
-  d.mats = c->d_mats;
+  d.mats = as_global(c->d_mats);
   d.g = c->geom;
   d.sc = c->sc;
   d.cs = c->collider_set();
-  d.accum = c->accum;
-  d.vel = c->vel;
-  d.dirty = c->dirty;
-  d.live = c->live;
-  d.keys0 = c->keys0;
-  d.keys1 = c->keys1;
-  d.keys2 = c->keys2;
-  d.vals1 = c->vals1;
-  d.vals2 = c->vals2;
-  d.offsets = c->offsets;
-  d.active = c->active;
-  d.n_active = c->n_active;
-  d.dirty_list = c->dirty_list;
-  d.slot = c->slot;
-  d.stage = c->stage;
-  d.in_cnt = c->in_cnt;
-  d.in_lst = c->in_lst;
-  d.fwd = c->fwd;
-  d.fix_list = c->fix_list;
+  d.accum = as_global(c->accum);
+  d.vel = as_global(c->vel);
+  d.dirty = as_global(c->dirty);
+  d.live = as_global(c->live);
+  d.keys0 = as_global(c->keys0);
+  d.keys1 = as_global(c->keys1);
+  d.keys2 = as_global(c->keys2);
+  d.vals1 = as_global(c->vals1);
+  d.vals2 = as_global(c->vals2);
+  d.offsets = as_global(c->offsets);
+  d.active = as_global(c->active);
+  d.n_active = as_global(c->n_active);
+  d.dirty_list = as_global(c->dirty_list);
+  d.slot = as_global(c->slot);
+  d.stage = as_global(c->stage);
+  d.in_cnt = as_global(c->in_cnt);
+  d.in_lst = as_global(c->in_lst);
+  d.fwd = as_global(c->fwd);
+  d.fix_list = as_global(c->fix_list);
   d.n = (int32_t)c->n;
   d.n_bricks = c->n_bricks;
   d.n_cells = c->n_cells;

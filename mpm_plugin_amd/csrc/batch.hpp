@@ -35,28 +35,31 @@ namespace mpm {
 constexpr int kTile = 256;        // particle slots per unit of a per-particle kernel
 constexpr int kGroupBricks = 64;  // bricks whose dirty flags one wave of the grid update examines
 constexpr int kPlanThreads = 1024;
+// per scene, the speed maximum of a substep's G2P is gathered in this many words (workgroup u takes word
+// u mod kSpeedSlots): thousands of workgroups of one large scene on one word would serialise in L2
+constexpr int kSpeedSlots = 64;
 
 struct SceneDev {
   Particles set[2];  // set[cur] is live, the other is the reorder target
-  const MatDev* mats;
+  MPM_GLOBAL const MatDev* mats;
   GridGeom g;
   StepConsts sc;
   ColliderSet cs;
-  float4* accum;
-  float4* vel;
-  uint32_t* dirty;
-  uint8_t* live;
-  uint32_t *keys0, *keys1, *keys2, *vals1, *vals2;
-  uint32_t* offsets;   // per cell
-  uint32_t* active;    // occupied bricks
-  uint32_t* n_active;  // their count
-  uint32_t* dirty_list;  // bricks P2G wrote this substep (grid update work list)
-  uint32_t* fix_list;    // particles whose material update is redone in FP64 this substep
-  int32_t* slot;         // per brick: position in `active`, or -1
-  float4* stage;         // per occupied brick: its P2G window (8^3 nodes)
-  uint32_t* in_cnt;      // drift routing (DriftLists): per cell of an occupied brick
-  uint32_t* in_lst;
-  uint8_t* fwd;          // per particle
+  MPM_GLOBAL float4* accum;
+  MPM_GLOBAL float4* vel;
+  MPM_GLOBAL uint32_t* dirty;
+  MPM_GLOBAL uint8_t* live;
+  MPM_GLOBAL uint32_t *keys0, *keys1, *keys2, *vals1, *vals2;
+  MPM_GLOBAL uint32_t* offsets;   // per cell
+  MPM_GLOBAL uint32_t* active;    // occupied bricks
+  MPM_GLOBAL uint32_t* n_active;  // their count
+  MPM_GLOBAL uint32_t* dirty_list;  // bricks P2G wrote this substep (grid update work list)
+  MPM_GLOBAL uint32_t* fix_list;    // particles whose material update is redone in FP64 this substep
+  MPM_GLOBAL int32_t* slot;         // per brick: position in `active`, or -1
+  MPM_GLOBAL float4* stage;         // per occupied brick: its P2G window (8^3 nodes)
+  MPM_GLOBAL uint32_t* in_cnt;      // drift routing (DriftLists): per cell of an occupied brick
+  MPM_GLOBAL uint32_t* in_lst;
+  MPM_GLOBAL uint8_t* fwd;          // per particle
   int32_t n;           // particles
   int32_t n_bricks, n_cells, passes;
   int32_t sort_interval;
@@ -121,7 +124,8 @@ __device__ __forceinline__ const uint32_t* sorted_vals(const SceneDev& S) {
 __global__ void __launch_bounds__(kPlanThreads) k_batch_plan(const SceneDev* __restrict__ sd,
                                                              SceneStep* __restrict__ st,
                                                              ColliderFrame* __restrict__ frames, int n_scenes,
-                                                             int allow_sort, int g2p_tile, uint32_t* __restrict__ pre) {
+                                                             int allow_sort, int g2p_tile, uint32_t* __restrict__ pre,
+                                                             uint32_t* __restrict__ speed_slots) {
   __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
   const int stride = n_scenes + 1;
   uint32_t carry[5] = {0, 0, 0, 0, 0};
@@ -161,6 +165,9 @@ __global__ void __launch_bounds__(kPlanThreads) k_batch_plan(const SceneDev* __r
 #pragma unroll
     for (int q = 0; q < 5; ++q) pre[q * stride + n_scenes] = carry[q];
   }
+  __syncthreads();  // every scene's `run` is set
+  for (int t = threadIdx.x; t < n_scenes * kSpeedSlots; t += kPlanThreads)
+    if (st[t / kSpeedSlots].run) speed_slots[t] = 0u;
   for (int t = threadIdx.x; t < n_scenes * kMaxColliders; t += kPlanThreads) {
     const int i = t / kMaxColliders, j = t % kMaxColliders;
     const int32_t k = st[i].k, steps = st[i].steps;
@@ -495,7 +502,8 @@ __global__ void __launch_bounds__(256) k_batch_grid(const SceneDev* __restrict__
 // 256: a launch-shape choice); each tile's speed maximum goes to the scene's record
 __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ sd, SceneStep* __restrict__ st,
                                                    const uint32_t* __restrict__ pre, int n_scenes,
-                                                   const ColliderFrame* __restrict__ frames) {
+                                                   const ColliderFrame* __restrict__ frames,
+                                                   uint32_t* __restrict__ speed_slots) {
   const uint32_t* P0 = pre + kPreTiles * (n_scenes + 1);
   const uint32_t total = P0[n_scenes];
   const int tile = (int)blockDim.x, waves = tile >> 6;
@@ -514,8 +522,9 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ 
     if (S.sc.precision != kPrecisionFp32) wave_append(FixList{S.fix_list, &st[i].n_fix}, defer, (uint32_t)p);
     if (!(speed == speed)) speed = __int_as_float(0x7f800000);
     const float wm = wave_max(speed);
+    uint32_t* slot_max = speed_slots + (size_t)i * kSpeedSlots + (u % kSpeedSlots);
     if (waves == 1) {
-      if (threadIdx.x == 0) atomicMax(&st[i].speed_bits, __float_as_uint(wm));
+      if (threadIdx.x == 0) atomicMax(slot_max, __float_as_uint(wm));
       continue;
     }
     if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
@@ -524,7 +533,7 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ 
       float b = s_max[0];
       for (int w = 1; w < waves; ++w) b = fmaxf(b, s_max[w]);
       // non-negative floats order like their bit patterns
-      atomicMax(&st[i].speed_bits, __float_as_uint(b));
+      atomicMax(slot_max, __float_as_uint(b));
     }
     __syncthreads();
   }
@@ -563,11 +572,16 @@ __global__ void __launch_bounds__(256) k_batch_fixup(const SceneDev* __restrict_
   }
 }
 
-__global__ void __launch_bounds__(256) k_batch_advance(SceneStep* __restrict__ st, int n_scenes) {
+// end of a substep: bookkeeping, and the speed maximum of the scene's G2P from its slots
+__global__ void __launch_bounds__(256) k_batch_advance(SceneStep* __restrict__ st, int n_scenes,
+                                                       const uint32_t* __restrict__ speed_slots) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_scenes || !st[i].run) return;
   st[i].since_sort += 1;
   st[i].k += 1;
+  uint32_t m = 0u;
+  for (int j = 0; j < kSpeedSlots; ++j) m = max(m, speed_slots[(size_t)i * kSpeedSlots + j]);
+  st[i].speed_bits = m;
 }
 
 }  // namespace mpm

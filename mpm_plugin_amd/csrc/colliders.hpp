@@ -51,11 +51,11 @@ struct TriNormals {
 
 struct ColliderSet {
   int count;
-  const ColliderDev* cols;
-  const TriGeom* tris;
-  const TriNormals* normals;
-  const uint32_t* cell_start;  // per cell: first candidate; cell_start[cell + 1] ends it
-  const uint32_t* cell_tris;   // candidate triangle indices (collider-local)
+  MPM_GLOBAL const ColliderDev* cols;
+  MPM_GLOBAL const TriGeom* tris;
+  MPM_GLOBAL const TriNormals* normals;
+  MPM_GLOBAL const uint32_t* cell_start;  // per cell: first candidate; cell_start[cell + 1] ends it
+  MPM_GLOBAL const uint32_t* cell_tris;   // candidate triangle indices (collider-local)
   float thickness;             // open meshes are solid this far behind the surface (2 dx)
   float band;                  // grid contact band in front of the surface (dx)
 };

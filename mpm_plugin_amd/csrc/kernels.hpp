@@ -27,14 +27,27 @@ struct MatDev {
 // holding what P2G reads besides x and v: C (row-major, 9), the Kirchhoff stress of F (xx, yy, zz, xy,
 // xz, yz) and the particle mass. One record per lane makes P2G's loads whole cache-line pieces where
 // planes would scatter them (its lanes walk different cells, not consecutive slots).
+// Device arrays in HBM. The qualifier makes device code address them as global memory even when the
+// pointer itself is read from memory (a scene table) rather than passed as a kernel argument, where the
+// compiler could not infer it and would emit flat (global-or-LDS) accesses; host code sees plain pointers
+// with the same representation.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MPM_GLOBAL __attribute__((address_space(1)))
+#else
+#define MPM_GLOBAL
+#endif
+
+// a plain pointer as a global one (a no-op cast in host code)
+template <class T> __host__ __device__ inline MPM_GLOBAL T* as_global(T* p) { return (MPM_GLOBAL T*)p; }
+
 struct Particles {
-  float* x;
-  float* v;
-  float* F;
-  float* Jp;
-  int32_t* mat;
-  int32_t* pid;
-  float4* rec;
+  MPM_GLOBAL float* x;
+  MPM_GLOBAL float* v;
+  MPM_GLOBAL float* F;
+  MPM_GLOBAL float* Jp;
+  MPM_GLOBAL int32_t* mat;
+  MPM_GLOBAL int32_t* pid;
+  MPM_GLOBAL float4* rec;
   int64_t cap;
 };
 

@@ -15,7 +15,7 @@ def test_coordinate_descent_finds_the_separable_optimum():
     space = {0: [1, 2, 3, 4], 1: [10, 20, 30]}
     calls = []
 
-    def score(c):
+    def score(c, knob=None):
         calls.append(dict(c))
         return -((c[0] - 3) ** 2) - (c[1] - 20) ** 2
 
@@ -29,7 +29,7 @@ def test_interleaving_cancels_a_drifting_measure():
     space = {0: [1, 2]}
     t = [0]
 
-    def score(c):
+    def score(c, knob=None):
         t[0] += 1
         return (1.1 if c[0] == 2 else 1.0) * 100.0 / t[0]
 
