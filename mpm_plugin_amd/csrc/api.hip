@@ -127,6 +127,7 @@ struct ColliderHost {
   float built_pad = -1.0f;
   std::vector<uint32_t> cell_start;  // relative to this collider's candidate list
   std::vector<uint32_t> cell_tris;
+  std::vector<float> cell_far;       // per cell: signed lower bound of the distance to the surface
 };
 
 }  // namespace
@@ -189,6 +190,7 @@ struct mpm_ctx {
   TriGeom* d_tris = nullptr;
   TriNormals* d_normals = nullptr;
   uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
+  float* d_cell_far = nullptr;
 
   float* h_vmax = nullptr;        // reduced maximum speed, in pinned host memory the reduction writes to
   float* h_vmax_dev = nullptr;    // its device address
@@ -214,6 +216,7 @@ struct mpm_ctx {
     cs.normals = as_global(d_normals);
     cs.cell_start = as_global(d_cell_start);
     cs.cell_tris = as_global(d_cell_tris);
+    cs.cell_far = as_global(d_cell_far);
     cs.thickness = (float)(2.0 * dom.dx_f64);
     cs.band = (float)dom.dx_f64;
     return cs;
@@ -768,17 +771,19 @@ void upload_collider_records(mpm_ctx* c) {
 void upload_colliders(mpm_ctx* c) {
   HIP_CHECK(hipStreamSynchronize(c->stream));
   for (void* p : {(void*)c->d_cols, (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start,
-                  (void*)c->d_cell_tris})
+                  (void*)c->d_cell_tris, (void*)c->d_cell_far})
     dev_free(p);
   c->d_cols = nullptr;
   c->d_tris = nullptr;
   c->d_normals = nullptr;
   c->d_cell_start = nullptr;
   c->d_cell_tris = nullptr;
+  c->d_cell_far = nullptr;
   if (c->colliders.empty()) return;
   std::vector<TriGeom> tris;
   std::vector<TriNormals> normals;
   std::vector<uint32_t> starts, cand;
+  std::vector<float> far;
   for (auto& h : c->colliders) {
     tris.insert(tris.end(), h.tris.begin(), h.tris.end());
     normals.insert(normals.end(), h.normals.begin(), h.normals.end());
@@ -786,13 +791,17 @@ void upload_colliders(mpm_ctx* c) {
     const size_t cells = h.cell_start.empty() ? 0 : h.cell_start.size() - 1;
     for (size_t i = 0; i < cells; ++i) starts.push_back(base + h.cell_start[i]);
     cand.insert(cand.end(), h.cell_tris.begin(), h.cell_tris.end());
+    far.insert(far.end(), h.cell_far.begin(), h.cell_far.end());
   }
   starts.push_back((uint32_t)cand.size());
+  far.push_back(0.0f);  // never empty
   c->d_cols = dev_alloc<ColliderDev>(c->colliders.size());
   c->d_tris = dev_alloc<TriGeom>(tris.size());
   c->d_normals = dev_alloc<TriNormals>(normals.size());
   c->d_cell_start = dev_alloc<uint32_t>(starts.size());
   c->d_cell_tris = dev_alloc<uint32_t>(cand.size());
+  c->d_cell_far = dev_alloc<float>(far.size());
+  HIP_CHECK(hipMemcpy(c->d_cell_far, far.data(), far.size() * sizeof(float), hipMemcpyHostToDevice));
   if (!tris.empty()) {
     HIP_CHECK(hipMemcpy(c->d_tris, tris.data(), tris.size() * sizeof(TriGeom), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->d_normals, normals.data(), normals.size() * sizeof(TriNormals), hipMemcpyHostToDevice));
@@ -807,6 +816,7 @@ void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
   ColliderDev& d = h.dev;
   h.cell_start.clear();
   h.cell_tris.clear();
+  h.cell_far.clear();
   if (d.n_tris == 0) {
     d.g_res[0] = d.g_res[1] = d.g_res[2] = 0;
     h.built_pad = pad;
@@ -826,27 +836,37 @@ void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
   }
   d.g_inv_h = 1.0f / hvox;
   TriGeom* dtris = dev_alloc<TriGeom>(h.tris.size());
+  TriNormals* dnorm = dev_alloc<TriNormals>(h.normals.size());
+  float* dfar = dev_alloc<float>(cells);
   uint32_t* counts = dev_alloc<uint32_t>(cells + 1);
   uint32_t* starts = dev_alloc<uint32_t>(cells + 1);
   ColliderDev local = d;
   local.tri_offset = 0;
   HIP_CHECK(hipMemcpyAsync(dtris, h.tris.data(), h.tris.size() * sizeof(TriGeom), hipMemcpyHostToDevice, c->stream));
-  k_voxel_candidates<<<grid_for(cells), kBlock, 0, c->stream>>>(local, dtris, (int)cells, 0, counts, nullptr, nullptr);
+  HIP_CHECK(hipMemcpyAsync(dnorm, h.normals.data(), h.normals.size() * sizeof(TriNormals), hipMemcpyHostToDevice,
+                           c->stream));
+  k_voxel_candidates<<<grid_for(cells), kBlock, 0, c->stream>>>(local, dtris, dnorm, (int)cells, 0, counts, dfar,
+                                                                nullptr, nullptr);
   check_launch("k_voxel_candidates");
   std::vector<uint32_t> cnt(cells);
+  h.cell_far.resize(cells);
   HIP_CHECK(hipMemcpyAsync(cnt.data(), counts, cells * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(h.cell_far.data(), dfar, cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   h.cell_start.assign(cells + 1, 0);
   for (int64_t i = 0; i < cells; ++i) h.cell_start[i + 1] = h.cell_start[i] + cnt[i];
   const uint32_t total = h.cell_start[cells];
   uint32_t* out = dev_alloc<uint32_t>(std::max<uint32_t>(total, 1));
   HIP_CHECK(hipMemcpyAsync(starts, h.cell_start.data(), (cells + 1) * 4, hipMemcpyHostToDevice, c->stream));
-  k_voxel_candidates<<<grid_for(cells), kBlock, 0, c->stream>>>(local, dtris, (int)cells, 1, nullptr, starts, out);
+  k_voxel_candidates<<<grid_for(cells), kBlock, 0, c->stream>>>(local, dtris, dnorm, (int)cells, 1, nullptr, nullptr,
+                                                                starts, out);
   check_launch("k_voxel_candidates");
   h.cell_tris.resize(total);
   if (total) HIP_CHECK(hipMemcpyAsync(h.cell_tris.data(), out, total * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   dev_free(dtris);
+  dev_free(dnorm);
+  dev_free(dfar);
   dev_free(counts);
   dev_free(starts);
   dev_free(out);
@@ -1534,7 +1554,8 @@ int32_t mpm_destroy(mpm_ctx* c) {
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
                   (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_block_max, (void*)c->d_cols,
-                  (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris})
+                  (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris,
+                  (void*)c->d_cell_far})
     dev_free(p);
   c->scratch.release();
   c->hist.release();

@@ -56,6 +56,9 @@ struct ColliderSet {
   MPM_GLOBAL const TriNormals* normals;
   MPM_GLOBAL const uint32_t* cell_start;  // per cell: first candidate; cell_start[cell + 1] ends it
   MPM_GLOBAL const uint32_t* cell_tris;   // candidate triangle indices (collider-local)
+  // per cell: a lower bound of the local distance from any point of the cell to the surface, signed by
+  // the side of the cell's centre (0: the surface may come that close)
+  MPM_GLOBAL const float* cell_far;
   float thickness;             // open meshes are solid this far behind the surface (2 dx)
   float band;                  // grid contact band in front of the surface (dx)
 };
@@ -64,6 +67,7 @@ struct ColliderFrame {
   float A[9], t[3];        // M(s)
   float Ai[9], ti[3];      // M(s)^-1
   float dA[9], dt[3];      // velocity matrix
+  float smin;              // 1 / |M(s)^-1|_F: at most the smallest singular value of A, so |A u| >= smin |u|
 };
 
 // every collider's frame of one substep, computed once on the host and passed by value to the kernels
@@ -127,6 +131,10 @@ __host__ __device__ inline void collider_frame(const ColliderDev& c, float s, fl
   f.Ai[6] = c02 * inv; f.Ai[7] = (a[1] * a[6] - a[0] * a[7]) * inv; f.Ai[8] = (a[0] * a[4] - a[1] * a[3]) * inv;
 #pragma unroll
   for (int i = 0; i < 3; ++i) f.ti[i] = -(f.Ai[3 * i] * f.t[0] + f.Ai[3 * i + 1] * f.t[1] + f.Ai[3 * i + 2] * f.t[2]);
+  float fro = 0.0f;
+  for (int i = 0; i < 9; ++i) fro += f.Ai[i] * f.Ai[i];
+  // rounded down a little: a lower bound is all the far-cell test may use
+  f.smin = fro > 0.0f ? 0.9999f / sqrtf(fro) : 0.0f;
   const float h = 1.0e-3f;
   float Ap[9], tp[3], Am[9], tm[3];
   collider_matrix(c, s + h, Ap, tp);
@@ -203,8 +211,10 @@ __device__ __forceinline__ void scan_triangle(const ColliderSet& cs, const Colli
 }
 
 // (outward normal, signed distance); phi = 1e30 when far (colliders.py:40-65)
+// `threshold`: the caller only acts on phi < threshold; a point provably farther (or, for an open mesh,
+// also beyond the solid thickness behind it) gets phi = 1e30 without the triangle scan
 __device__ void collider_sdf(const ColliderSet& cs, const ColliderDev& c, const ColliderFrame& f, const float (&p)[3],
-                             float (&n)[3], float& phi) {
+                             float (&n)[3], float& phi, float threshold) {
   n[0] = 0.0f; n[1] = 0.0f; n[2] = 1.0f;
   phi = 1.0e30f;
   float pl[3];
@@ -225,6 +235,11 @@ __device__ void collider_sdf(const ColliderSet& cs, const ColliderDev& c, const 
   }
   if (in_grid) {
     const int ci = c.cell_offset + (cell[0] * c.g_res[1] + cell[1]) * c.g_res[2] + cell[2];
+    // the world distance is at least smin x the local one: no contact possible from this cell
+    const float far = cs.cell_far[ci];
+    if (c.closed ? (far > 0.0f && far * f.smin >= threshold)
+                 : (fabsf(far) * f.smin >= fmaxf(threshold, cs.thickness)))
+      return;
     const uint32_t a0 = cs.cell_start[ci], a1 = cs.cell_start[ci + 1];
     for (uint32_t k = a0; k < a1; ++k) scan_triangle(cs, c, (int)cs.cell_tris[k], pl, best, bt, bf, q);
   } else {
@@ -269,7 +284,7 @@ __device__ void collide_node_frames(const ColliderSet& cs, const ColliderFrame* 
   for (int c = 0; c < cs.count; ++c) {
     const ColliderDev col = cs.cols[c];
     float n[3], phi;
-    collider_sdf(cs, col, frames[c], pos, n, phi);
+    collider_sdf(cs, col, frames[c], pos, n, phi, cs.band);
     if (phi < cs.band) {
       float vcol[3];
       collider_velocity(frames[c], pos, vcol);
@@ -302,7 +317,7 @@ __device__ void project_particle(const ColliderSet& cs, const ColliderFrame* fra
   for (int c = 0; c < cs.count; ++c) {
     const ColliderDev col = cs.cols[c];
     float n[3], phi;
-    collider_sdf(cs, col, frames[c], pos, n, phi);
+    collider_sdf(cs, col, frames[c], pos, n, phi, 0.0f);
     if (phi < 0.0f) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) pos[a] -= phi * n[a];
@@ -336,8 +351,12 @@ __device__ __forceinline__ void voxel_centre(const ColliderDev& c, int cell, flo
 }
 
 // pass 0: candidate count per voxel; pass 1: candidate indices, ascending
-__global__ void __launch_bounds__(256) k_voxel_candidates(ColliderDev c, const TriGeom* __restrict__ tris, int n_cells,
+// pass 0 also writes the cell's far bound: (distance of the centre - half diagonal), shrunk by the same
+// slack, signed by the pseudonormal side of the centre's closest feature; 0 when not positive
+__global__ void __launch_bounds__(256) k_voxel_candidates(ColliderDev c, const TriGeom* __restrict__ tris,
+                                                          const TriNormals* __restrict__ normals, int n_cells,
                                                           int pass, uint32_t* __restrict__ counts,
+                                                          float* __restrict__ far,
                                                           const uint32_t* __restrict__ starts,
                                                           uint32_t* __restrict__ out) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -345,8 +364,31 @@ __global__ void __launch_bounds__(256) k_voxel_candidates(ColliderDev c, const T
   float p[3], hd;
   voxel_centre(c, cell, p, hd);
   const TriGeom* T = tris + c.tri_offset;
-  float dmin2 = 3.0e38f;
-  for (int t = 0; t < c.n_tris; ++t) dmin2 = fminf(dmin2, tri_dist2(T, t, p));
+  float dmin2 = 3.0e38f, bq[3] = {0.0f, 0.0f, 0.0f};
+  int bt = 0, bf = 0;
+  for (int t = 0; t < c.n_tris; ++t) {
+    float q[3];
+    int f;
+    closest_on_triangle(p, T[t].v, q, f);
+    const float d2 = (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]);
+    if (d2 < dmin2) {
+      dmin2 = d2;
+      bt = t;
+      bf = f;
+      bq[0] = q[0]; bq[1] = q[1]; bq[2] = q[2];
+    }
+  }
+  if (pass == 0) {
+    const float lower = (sqrtf(dmin2) - hd) * 0.99999f - 1.0e-6f;
+    float signed_far = 0.0f;
+    if (lower > 0.0f) {
+      const float* nn = normals[bt].n;
+      const float* pn = bf == 0 ? nn : (bf <= 3 ? nn + 12 + 3 * (bf - 1) : nn + 3 + 3 * (bf - 4));
+      const float side = (p[0] - bq[0]) * pn[0] + (p[1] - bq[1]) * pn[1] + (p[2] - bq[2]) * pn[2];
+      signed_far = side < 0.0f ? -lower : lower;
+    }
+    far[cell] = signed_far;
+  }
   // 1e-5 relative slack absorbs FP32 rounding of the distances
   const float lim = (sqrtf(dmin2) + 2.0f * hd) * 1.00001f + 1.0e-6f;
   const float lim2 = lim * lim;
