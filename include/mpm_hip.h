@@ -143,7 +143,7 @@ int32_t mpm_debug_sort(mpm_ctx* ctx);
 /* ---- options ------------------------------------------------------------------------------ */
 #define MPM_OPT_P2G_MODE 0       /* MPM_P2G_BINNED (default), MPM_P2G_ATOMIC or MPM_P2G_FIXED */
 #define MPM_OPT_SORT_INTERVAL 1  /* substeps between brick re-sorts (default 16; >= 1) */
-#define MPM_OPT_P2G_BLOCKS 2     /* workgroups of the binned P2G launch (0: automatic) */
+#define MPM_OPT_P2G_BLOCKS 2     /* workgroups (one wave each) of the binned P2G launch (0: automatic) */
 #define MPM_OPT_PRECISION 3      /* SVD, return maps and stress: MPM_PRECISION_FP32, _FALLBACK (default:
                                     FP64 only where FP32 cannot resolve F: sigma_min/sigma_max < 1e-2 or
                                     J = Jp det F < 0.25), _FP64 */
@@ -200,7 +200,7 @@ int32_t mpm_batch_run(mpm_batch* batch, int64_t max_launches, int64_t* launched,
                       mpm_advance_stats* out);
 /* substeps done / in the frame in flight, per scene (equal when the scene is not in a frame) */
 int32_t mpm_batch_progress(const mpm_batch* batch, int32_t* substeps_done, int32_t* substeps_total);
-#define MPM_BATCH_OPT_P2G_BLOCKS 0        /* workgroups of the persistent P2G launch (0: automatic) */
+#define MPM_BATCH_OPT_P2G_BLOCKS 0        /* workgroups (one wave each) of the persistent P2G launch (0: automatic) */
 #define MPM_BATCH_OPT_GRAPH 1             /* 1: replay captured graphs (default); 0: launch directly */
 #define MPM_BATCH_OPT_SORT_INTERVAL 2     /* substeps between re-sorts for every scene (0: each scene's own) */
 #define MPM_BATCH_OPT_SUBSTEPS_LAUNCHED 3 /* read only: substep launches since creation */

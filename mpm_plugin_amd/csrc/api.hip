@@ -1047,7 +1047,7 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   check_launch("k_batch_prefix");
   k_batch_drift<<<2048, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n, (int)b->g2p_tile);
   check_launch("k_batch_drift");
-  k_batch_p2g<<<batch_blocks(b->p2g_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+  k_batch_p2g<1><<<batch_blocks(b->p2g_blocks, 8192), 64, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
   check_launch("k_batch_p2g");
   k_batch_grid_scan<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
   check_launch("k_batch_grid_scan");
@@ -2078,7 +2078,7 @@ int64_t mpm_batch_get_option(const mpm_batch* b, int32_t option) {
   if (!b) return -1;
   switch (option) {
     case MPM_BATCH_OPT_P2G_BLOCKS:
-      return batch_blocks(b->p2g_blocks, 2048);
+      return batch_blocks(b->p2g_blocks, 8192);
     case MPM_BATCH_OPT_G2P_BLOCKS:
       return batch_blocks(b->g2p_blocks, 2048);
     case MPM_BATCH_OPT_GRID_BLOCKS:

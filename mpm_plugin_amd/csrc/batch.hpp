@@ -415,9 +415,11 @@ __global__ void __launch_bounds__(256) k_batch_drift(const SceneDev* __restrict_
 }
 
 // P2G over every running scene's occupied bricks, one wave per brick (p2g_brick)
-__global__ void __launch_bounds__(256) k_batch_p2g(const SceneDev* __restrict__ sd, const SceneStep* __restrict__ st,
-                                                   const uint32_t* __restrict__ pre, int n_scenes) {
-  constexpr int kWaves = 4;
+// (one wave per workgroup, so a brick starts as soon as any wave slot frees)
+template <int kWaves>
+__global__ void __launch_bounds__(64 * kWaves) k_batch_p2g(const SceneDev* __restrict__ sd,
+                                                           const SceneStep* __restrict__ st,
+                                                           const uint32_t* __restrict__ pre, int n_scenes) {
   __shared__ float s_win[kWaves][4][kWinNodes];
   const uint32_t* P0 = pre + kPreBricks * (n_scenes + 1);
   const int wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
