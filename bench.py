@@ -51,6 +51,28 @@ G2P_BYTES_NODE = 12
 GRID_BYTES_NODE = 28 + 16  # read accum 16, write velocity 12 (+ zeroing 16 folded into the update)
 
 
+CU_COUNT = 256
+CLOCK_HZ = 2.4e9
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_multi_material_128.md")
+
+
+def measured_counters(workload: str) -> dict:
+    """HBM traffic and VALU instruction counts per launch of P2G and G2P on the 1M multi-material workload,
+    from the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
+    calibration given there); empty for any other workload or without the summary."""
+    out = {}
+    if not workload.startswith("multi_material") or not os.path.exists(PMC_SUMMARY):
+        return out
+    with open(PMC_SUMMARY) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch":
+                out[("traffic", parts[1])] = int(parts[2])
+            elif len(parts) == 3 and parts[0] == "valu_instructions_per_launch":
+                out[("valu", parts[1])] = int(parts[2])
+    return out
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -252,8 +274,7 @@ def main(argv=None) -> int:
     n_snow = int(sum((w.material_ids == i).sum() for i, m in enumerate(w.materials) if m.model == "snow"))
     log(f"warmup done ({args.warmup} steps); active nodes {n_active}")
 
-    # per-class kernel times over a sort interval's worth of steps, every class bracketed with events;
-    # they pick the dominant kernel, the only one bracketed in the timed steps
+    # per-class kernel times over two sort intervals, every class bracketed with events
     solver.reset_timing()
     solver.set_timing(True)
     for _ in range(PROFILE_STEPS):
@@ -262,10 +283,18 @@ def main(argv=None) -> int:
     times = solver.kernel_times()
     log(f"profile pass ({PROFILE_STEPS} steps), kernel ms: "
         + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in times.items()))
-    dominant = max(("p2g", "g2p"), key=lambda k: times[k][0])
 
+    # P2G's launch time with only P2G bracketed (every bracketed kernel adds its events' cost to the step)
     solver.reset_timing()
-    solver.set_timing(True, phases=[dominant])
+    solver.set_timing(True, phases=["p2g"])
+    for _ in range(PROFILE_STEPS):
+        step()
+    solver.set_timing(False)
+    timed = {"p2g": solver.kernel_times()["p2g"]}
+
+    # the timed steps bracket only G2P
+    solver.reset_timing()
+    solver.set_timing(True, phases=["g2p"])
     solver.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -273,21 +302,35 @@ def main(argv=None) -> int:
     solver.synchronize()
     elapsed = time.perf_counter() - t0
     solver.set_timing(False)
-    timed = solver.kernel_times()
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; {dominant} kernel {timed[dominant][0]:.2f} ms / "
-        f"{timed[dominant][1]} launches")
+    timed["g2p"] = solver.kernel_times()["g2p"]
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; g2p {timed['g2p'][0]:.2f} ms / {timed['g2p'][1]} launches "
+        f"(p2g {timed['p2g'][0]:.2f} ms / {timed['p2g'][1]} launches in its own pass)")
 
     n = solver.particle_count
     bytes_per_launch = {
         "p2g": P2G_BYTES_PARTICLE * n + P2G_BYTES_SNOW * n_snow + P2G_BYTES_NODE * n_active,
         "g2p": G2P_BYTES_PARTICLE * n + G2P_BYTES_SNOW * n_snow + G2P_BYTES_NODE * n_active,
     }
-    ms, launches = timed[dominant]
-    mean_s = ms / 1000.0 / max(launches, 1)
-    achieved = bytes_per_launch[dominant] / mean_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
-                "mean_launch_us": round(mean_s * 1e6, 2), "algorithmic_bytes": int(bytes_per_launch[dominant])}
+    pmc = measured_counters(w.name)
+
+    def roofline_of(kernel: str) -> dict:
+        ms, launches = timed[kernel]
+        mean_s = ms / 1000.0 / max(launches, 1)
+        achieved = bytes_per_launch[kernel] / mean_s / 1e9
+        lib_name = {"p2g": "k_p2g_cells", "g2p": "k_g2p"}[kernel]
+        out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc.get(("traffic", lib_name)),
+               "kernel": kernel, "mean_launch_us": round(mean_s * 1e6, 2),
+               "algorithmic_bytes": int(bytes_per_launch[kernel])}
+        valu = pmc.get(("valu", lib_name))
+        if valu:
+            # wave instructions over what the CUs issue in the launch (one per CU per cycle)
+            out["valu_busy_frac"] = round(valu / (mean_s * CU_COUNT * CLOCK_HZ), 3)
+        return out
+
+    dominant = max(("p2g", "g2p"), key=lambda k: timed[k][0] / max(timed[k][1], 1))
+    roofline = roofline_of(dominant)
+    roofline["other_kernel"] = roofline_of("g2p" if dominant == "p2g" else "p2g")
     kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in times.items() if v[1]}
     kernels["profile_steps"] = PROFILE_STEPS
 
