@@ -186,7 +186,8 @@ struct mpm_ctx {
   // colliders
   std::vector<ColliderHost> colliders;
   ColliderDev* d_cols = nullptr;
-  std::vector<ColliderDev> col_records;  // host copy of d_cols (kept alive for the async upload)
+  std::vector<ColliderDev> col_records;  // the records; d_cols is their copy for the batch
+  bool cols_stale = false;               // d_cols older than col_records
   TriGeom* d_tris = nullptr;
   TriNormals* d_normals = nullptr;
   uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
@@ -568,8 +569,10 @@ void run_p2g(mpm_ctx* c, float dt) {
 // the colliders' frames at frame fraction s (the same function the batched plan runs on the device)
 ColliderFrames frames_at(const mpm_ctx* c, float s, float inv_frame_dt) {
   ColliderFrames fr{};
-  for (size_t i = 0; i < c->col_records.size() && i < (size_t)kMaxColliders; ++i)
+  for (size_t i = 0; i < c->col_records.size() && i < (size_t)kMaxColliders; ++i) {
     collider_frame(c->col_records[i], s, inv_frame_dt, fr.f[i]);
+    fr.cols[i] = c->col_records[i];
+  }
   return fr;
 }
 
@@ -750,6 +753,8 @@ int64_t remove_escaped(mpm_ctx* c) {
 // ---- colliders -------------------------------------------------------------------------------
 // per-collider records (pose, margin, offsets into the shared tables); small, rewritten on every pose
 // update with an async copy on the context's stream
+// the collider records (pose endpoints, margins, offsets into the shared tables) on the host; the
+// single-scene kernels take them by value, the batch reads a device copy refreshed when stale
 void upload_collider_records(mpm_ctx* c) {
   if (c->colliders.empty()) return;
   std::vector<ColliderDev> cols;
@@ -763,8 +768,14 @@ void upload_collider_records(mpm_ctx* c) {
     cols.push_back(d);
   }
   c->col_records = cols;
-  HIP_CHECK(hipMemcpyAsync(c->d_cols, c->col_records.data(), cols.size() * sizeof(ColliderDev),
+  c->cols_stale = true;
+}
+
+void ensure_collider_records(mpm_ctx* c) {
+  if (!c->cols_stale || c->colliders.empty()) return;
+  HIP_CHECK(hipMemcpyAsync(c->d_cols, c->col_records.data(), c->col_records.size() * sizeof(ColliderDev),
                            hipMemcpyHostToDevice, c->stream));
+  c->cols_stale = false;
 }
 
 // meshes, pseudonormals and candidate lists of every collider (after set_colliders or a grid rebuild)
@@ -1088,7 +1099,8 @@ hipGraphExec_t batch_capture(mpm_batch* b, bool sorting) {
   return exec;
 }
 
-SceneDev scene_record(const mpm_batch* b, const mpm_ctx* c) {
+SceneDev scene_record(const mpm_batch* b, mpm_ctx* c) {
+  ensure_collider_records(c);
   SceneDev d{};
   d.set[0] = c->A.view();
   d.set[1] = c->B.view();

@@ -70,11 +70,19 @@ struct ColliderFrame {
   float smin;              // 1 / |M(s)^-1|_F: at most the smallest singular value of A, so |A u| >= smin |u|
 };
 
-// every collider's frame of one substep, computed once on the host and passed by value to the kernels
-// (they read it through the scalar cache; no workgroup recomputes it)
+// every collider's frame of one substep and its record, computed once on the host and passed by value to
+// the kernels (they read them through the scalar cache; no workgroup recomputes a frame, and a pose
+// change needs no upload)
 struct ColliderFrames {
   ColliderFrame f[kMaxColliders];
+  ColliderDev cols[kMaxColliders];
 };
+
+// the collider set with its records read from the kernel arguments
+__device__ __forceinline__ ColliderSet with_records(ColliderSet cs, const ColliderFrames& fr) {
+  cs.cols = (MPM_GLOBAL const ColliderDev*)fr.cols;
+  return cs;
+}
 
 __host__ __device__ inline void quat_rot(const float (&q)[4], float (&R)[9]) {
   const float x = q[0], y = q[1], z = q[2], w = q[3];

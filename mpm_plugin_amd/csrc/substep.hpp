@@ -449,6 +449,7 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
                                                      const int32_t* __restrict__ slot,
                                                      uint32_t* __restrict__ n_fix) {
   const ColliderFrame* frames = fr.f;
+  cs = with_records(cs, fr);
   // the FP64 work list of the G2P that follows starts empty
   if (n_fix && blockIdx.x == 0 && threadIdx.x == 0) *n_fix = 0u;
   const int brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -467,6 +468,7 @@ __global__ void __launch_bounds__(256) k_grid_dense(GridGeom g, StepConsts sc, f
                                                     uint8_t* __restrict__ live, ColliderSet cs, ColliderFrames fr,
                                                     uint32_t* __restrict__ n_fix) {
   const ColliderFrame* frames = fr.f;
+  cs = with_records(cs, fr);
   if (n_fix && blockIdx.x == 0 && threadIdx.x == 0) *n_fix = 0u;
   const int addr = blockIdx.x * blockDim.x + threadIdx.x;
   if (addr >= n_nodes) return;
@@ -663,6 +665,7 @@ __global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restri
                                              float* __restrict__ block_max,
                                              ColliderSet cs, ColliderFrames fr, FixList fix) {
   const ColliderFrame* frames = fr.f;
+  cs = with_records(cs, fr);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float speed = 0.0f;
   bool defer = false;
