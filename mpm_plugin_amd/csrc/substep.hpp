@@ -563,19 +563,19 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
   float fx[3];
   int base[3];
   stencil_base(g, xo, base, fx);
-  float wx[3][3], dp[3][3];
+  float wx[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      wx[a][i] = bspline_w(fx[a], i);
-      dp[a][i] = ((float)i - fx[a]) * g.dx;
-    }
+    for (int i = 0; i < 3; ++i) wx[a][i] = bspline_w(fx[a], i);
+  // v = sum w v_n and C = 4/dx^2 sum w v_n (x_n - x)^T (kernels.py:208-224). With o_b in {-1, 0, 1} the
+  // stencil offset of the node along axis b less one, (x_n - x)_b = (o_b - (fx_b - 1)) dx, so
+  // C_ab = 4/dx (M_ab - (fx_b - 1) v_a) with M_ab = sum w v_n,a o_b: a node adds into M only along the
+  // axes where its offset is not the middle one, and only by addition or subtraction.
   float nv[3] = {0.0f, 0.0f, 0.0f};
-  M3<float> nC;
+  float M[3][3];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) nC.a[e / 3][e % 3] = 0.0f;
-  const float c4 = 4.0f * g.inv_dx * g.inv_dx;
+  for (int e = 0; e < 9; ++e) M[e / 3][e % 3] = 0.0f;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
 #pragma unroll
@@ -584,19 +584,27 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
 #pragma unroll
       for (int kk = 0; kk < 3; ++kk) {
         const float w = wij * wx[2][kk];
-        const float dpos[3] = {dp[0][i], dp[1][j], dp[2][kk]};
         const float4 gv4 = vel[node_addr(g, base[0] + i, base[1] + j, base[2] + kk)];
-        const float gv[3] = {gv4.x, gv4.y, gv4.z};
-        const float wc = c4 * w;
+        const float wg[3] = {w * gv4.x, w * gv4.y, w * gv4.z};
+        const int o[3] = {i - 1, j - 1, kk - 1};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          nv[a] += w * gv[a];
+          nv[a] += wg[a];
 #pragma unroll
-          for (int b = 0; b < 3; ++b) nC.a[a][b] += wc * gv[a] * dpos[b];
+          for (int b = 0; b < 3; ++b) {
+            if (o[b] > 0) M[a][b] += wg[a];
+            if (o[b] < 0) M[a][b] -= wg[a];
+          }
         }
       }
     }
   }
+  M3<float> nC;
+  const float c4dx = 4.0f * g.inv_dx;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) nC.a[a][b] = c4dx * (M[a][b] - (fx[b] - 1.0f) * nv[a]);
   // the record's C part now (only C22 stays live: it shares a float4 with the stress written last)
   P.rec[4 * p] = make_float4(nC.a[0][0], nC.a[0][1], nC.a[0][2], nC.a[1][0]);
   P.rec[4 * p + 1] = make_float4(nC.a[1][1], nC.a[1][2], nC.a[2][0], nC.a[2][1]);
