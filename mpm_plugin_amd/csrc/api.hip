@@ -127,6 +127,7 @@ struct ColliderHost {
   float built_pad = -1.0f;
   std::vector<uint32_t> cell_start;  // relative to this collider's candidate list
   std::vector<uint32_t> cell_tris;
+  std::vector<float> cell_key;       // per candidate: distance from its cell's centre (lists sorted by it)
   std::vector<float> cell_far;       // per cell: signed lower bound of the distance to the surface
 };
 
@@ -191,11 +192,12 @@ struct mpm_ctx {
   TriGeom* d_tris = nullptr;
   TriNormals* d_normals = nullptr;
   uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
-  float* d_cell_far = nullptr;
+  float *d_cell_key = nullptr, *d_cell_far = nullptr;
 
   float* h_vmax = nullptr;        // reduced maximum speed, in pinned host memory the reduction writes to
   float* h_vmax_dev = nullptr;    // its device address
   bool vmax_pending = false;      // a reduction into h_vmax of the current state is on the stream
+  hipEvent_t vmax_done = nullptr; // recorded behind that reduction
   float* d_block_max = nullptr;   // per-workgroup maxima of the last G2P (or speed pass)
   int64_t block_max_cap = 0;
   int64_t block_max_count = 0;
@@ -217,6 +219,7 @@ struct mpm_ctx {
     cs.normals = as_global(d_normals);
     cs.cell_start = as_global(d_cell_start);
     cs.cell_tris = as_global(d_cell_tris);
+    cs.cell_key = as_global(d_cell_key);
     cs.cell_far = as_global(d_cell_far);
     cs.thickness = (float)(2.0 * dom.dx_f64);
     cs.band = (float)dom.dx_f64;
@@ -534,7 +537,26 @@ const uint32_t* ctx_sorted_keys(const mpm_ctx* c) {
 }
 
 // ---- substep phases --------------------------------------------------------------------------
-void run_p2g(mpm_ctx* c, float dt) {
+// The binning the next binned P2G reads: a sort every sort_interval substeps, else (BINNED) the drift
+// lists. It reads positions only, so it can be enqueued before the substep's dt is known.
+DriftLists bin_for_p2g(mpm_ctx* c) {
+  const bool sorting = !c->sorted || c->steps_since_sort >= c->sort_interval;
+  if (sorting) sort_particles(c);
+  // between sorts, drifted particles go to their current cell's lane (not in fixed point: the lists'
+  // order is that of atomics, and the fixed-point sums must not depend on it)
+  DriftLists dl{};
+  if (!sorting && c->n > 0 && c->p2g_mode == MPM_P2G_BINNED) {
+    Timed t(c, MPM_K_OTHER);
+    k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)c->n, ctx_sorted_keys(c), c->slot,
+                                                      c->in_cnt, c->in_lst, c->fwd);
+    check_launch("k_drift");
+    dl = DriftLists{c->in_cnt, c->in_lst, c->fwd};
+  }
+  return dl;
+}
+
+// P2G; `pre` is the binning bin_for_p2g already enqueued for it (null: bin now)
+void run_p2g(mpm_ctx* c, float dt, const DriftLists* pre = nullptr) {
   if (c->p2g_mode == MPM_P2G_ATOMIC) {
     if (c->n == 0) return;
     Timed t(c, MPM_K_P2G);
@@ -543,19 +565,8 @@ void run_p2g(mpm_ctx* c, float dt) {
     check_launch("k_p2g_atomic");
     return;
   }
-  const bool sorting = !c->sorted || c->steps_since_sort >= c->sort_interval;
-  if (sorting) sort_particles(c);
+  const DriftLists dl = pre ? *pre : bin_for_p2g(c);
   if (c->n == 0) return;
-  // between sorts, drifted particles go to their current cell's lane (not in fixed point: the lists'
-  // order is that of atomics, and the fixed-point sums must not depend on it)
-  DriftLists dl{};
-  if (!sorting && c->p2g_mode == MPM_P2G_BINNED) {
-    Timed t(c, MPM_K_OTHER);
-    k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)c->n, ctx_sorted_keys(c), c->slot,
-                                                      c->in_cnt, c->in_lst, c->fwd);
-    check_launch("k_drift");
-    dl = DriftLists{c->in_cnt, c->in_lst, c->fwd};
-  }
   // one wave per workgroup, so a brick starts as soon as any wave slot frees; grid-stride over the
   // device-side brick list (costliest first)
   Timed t(c, MPM_K_P2G);
@@ -620,9 +631,9 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   c->vmax_pending = false;  // a reduction enqueued before this G2P is of the old state
 }
 
-void substep(mpm_ctx* c, double dt, double s, double frame_dt) {
+void substep(mpm_ctx* c, double dt, double s, double frame_dt, const DriftLists* pre = nullptr) {
   const float dtf = (float)dt, sf = (float)s, inv_fdt = (float)(1.0 / frame_dt);
-  run_p2g(c, dtf);
+  run_p2g(c, dtf, pre);
   run_grid(c, dtf, sf, inv_fdt);
   run_g2p(c, dtf, sf, inv_fdt);
   c->steps_since_sort += 1;
@@ -648,14 +659,25 @@ void reduce_speed_async(mpm_ctx* c, float* out) {
 // path between two advances).
 void request_speed(mpm_ctx* c) {
   if (c->n == 0 || c->vmax_pending) return;
-  Timed t(c, MPM_K_OTHER);
-  reduce_speed_async(c, c->h_vmax_dev);
+  {
+    Timed t(c, MPM_K_OTHER);
+    reduce_speed_async(c, c->h_vmax_dev);
+  }
+  HIP_CHECK(hipEventRecord(c->vmax_done, c->stream));
   c->vmax_pending = true;
 }
 
+// Waits for the requested reduction only (work enqueued behind it keeps running) by polling its event:
+// the step between two advances is on the critical path, and a blocking wait adds its wake-up latency
+// to every one of them.
 float max_speed(mpm_ctx* c) {
   request_speed(c);
-  HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (c->n > 0) {
+    hipError_t e;
+    while ((e = hipEventQuery(c->vmax_done)) == hipErrorNotReady) {
+    }
+    HIP_CHECK(e);
+  }
   return *static_cast<volatile float*>(c->h_vmax);
 }
 
@@ -782,19 +804,20 @@ void ensure_collider_records(mpm_ctx* c) {
 void upload_colliders(mpm_ctx* c) {
   HIP_CHECK(hipStreamSynchronize(c->stream));
   for (void* p : {(void*)c->d_cols, (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start,
-                  (void*)c->d_cell_tris, (void*)c->d_cell_far})
+                  (void*)c->d_cell_tris, (void*)c->d_cell_key, (void*)c->d_cell_far})
     dev_free(p);
   c->d_cols = nullptr;
   c->d_tris = nullptr;
   c->d_normals = nullptr;
   c->d_cell_start = nullptr;
   c->d_cell_tris = nullptr;
+  c->d_cell_key = nullptr;
   c->d_cell_far = nullptr;
   if (c->colliders.empty()) return;
   std::vector<TriGeom> tris;
   std::vector<TriNormals> normals;
   std::vector<uint32_t> starts, cand;
-  std::vector<float> far;
+  std::vector<float> keys, far;
   for (auto& h : c->colliders) {
     tris.insert(tris.end(), h.tris.begin(), h.tris.end());
     normals.insert(normals.end(), h.normals.begin(), h.normals.end());
@@ -802,6 +825,7 @@ void upload_colliders(mpm_ctx* c) {
     const size_t cells = h.cell_start.empty() ? 0 : h.cell_start.size() - 1;
     for (size_t i = 0; i < cells; ++i) starts.push_back(base + h.cell_start[i]);
     cand.insert(cand.end(), h.cell_tris.begin(), h.cell_tris.end());
+    keys.insert(keys.end(), h.cell_key.begin(), h.cell_key.end());
     far.insert(far.end(), h.cell_far.begin(), h.cell_far.end());
   }
   starts.push_back((uint32_t)cand.size());
@@ -811,6 +835,7 @@ void upload_colliders(mpm_ctx* c) {
   c->d_normals = dev_alloc<TriNormals>(normals.size());
   c->d_cell_start = dev_alloc<uint32_t>(starts.size());
   c->d_cell_tris = dev_alloc<uint32_t>(cand.size());
+  c->d_cell_key = dev_alloc<float>(keys.size());
   c->d_cell_far = dev_alloc<float>(far.size());
   HIP_CHECK(hipMemcpy(c->d_cell_far, far.data(), far.size() * sizeof(float), hipMemcpyHostToDevice));
   if (!tris.empty()) {
@@ -818,7 +843,10 @@ void upload_colliders(mpm_ctx* c) {
     HIP_CHECK(hipMemcpy(c->d_normals, normals.data(), normals.size() * sizeof(TriNormals), hipMemcpyHostToDevice));
   }
   HIP_CHECK(hipMemcpy(c->d_cell_start, starts.data(), starts.size() * 4, hipMemcpyHostToDevice));
-  if (!cand.empty()) HIP_CHECK(hipMemcpy(c->d_cell_tris, cand.data(), cand.size() * 4, hipMemcpyHostToDevice));
+  if (!cand.empty()) {
+    HIP_CHECK(hipMemcpy(c->d_cell_tris, cand.data(), cand.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->d_cell_key, keys.data(), keys.size() * 4, hipMemcpyHostToDevice));
+  }
   upload_collider_records(c);
 }
 
@@ -827,6 +855,7 @@ void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
   ColliderDev& d = h.dev;
   h.cell_start.clear();
   h.cell_tris.clear();
+  h.cell_key.clear();
   h.cell_far.clear();
   if (d.n_tris == 0) {
     d.g_res[0] = d.g_res[1] = d.g_res[2] = 0;
@@ -838,7 +867,20 @@ void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
     ext[a] = (d.aabb_hi[a] - d.aabb_lo[a]) + 2.0f * pad;
     emax = std::max(emax, ext[a]);
   }
-  const float hvox = std::max(emax / 32.0f, 1e-12f);
+  // Voxel edge 0.8 sqrt(mean triangle area), within [emax/63.5, emax/16] (at most 64 cells per axis
+  // after rounding): about six triangles lie within
+  // the candidate radius (twice the half diagonal) of a surface point, so a query near the surface ends
+  // after one or two rounds whatever the mesh's resolution.
+  double area = 0.0;
+  for (const TriGeom& t : h.tris) {
+    const double e1[3] = {t.v[3] - t.v[0], t.v[4] - t.v[1], t.v[5] - t.v[2]};
+    const double e2[3] = {t.v[6] - t.v[0], t.v[7] - t.v[1], t.v[8] - t.v[2]};
+    const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                 cz = e1[0] * e2[1] - e1[1] * e2[0];
+    area += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+  }
+  const float target = (float)(0.8 * std::sqrt(area / (double)h.tris.size()));
+  const float hvox = std::max(std::min(std::max(target, emax / 63.5f), emax / 16.0f), 1e-12f);
   int64_t cells = 1;
   for (int a = 0; a < 3; ++a) {
     d.g_res[a] = std::min(64, std::max(1, (int)std::ceil(ext[a] / hvox)));
@@ -846,6 +888,9 @@ void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
     cells *= d.g_res[a];
   }
   d.g_inv_h = 1.0f / hvox;
+  // cells with a candidate list: centre within pad (the contact reach of any pose the grid is built for)
+  // plus two half diagonals of the surface
+  const float band = pad + 2.0f * 0.8660254f * hvox;
   TriGeom* dtris = dev_alloc<TriGeom>(h.tris.size());
   TriNormals* dnorm = dev_alloc<TriNormals>(h.normals.size());
   float* dfar = dev_alloc<float>(cells);
@@ -857,7 +902,7 @@ void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
   HIP_CHECK(hipMemcpyAsync(dnorm, h.normals.data(), h.normals.size() * sizeof(TriNormals), hipMemcpyHostToDevice,
                            c->stream));
   k_voxel_candidates<<<grid_for(cells), kBlock, 0, c->stream>>>(local, dtris, dnorm, (int)cells, 0, counts, dfar,
-                                                                nullptr, nullptr);
+                                                                nullptr, nullptr, nullptr, band);
   check_launch("k_voxel_candidates");
   std::vector<uint32_t> cnt(cells);
   h.cell_far.resize(cells);
@@ -868,12 +913,19 @@ void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
   for (int64_t i = 0; i < cells; ++i) h.cell_start[i + 1] = h.cell_start[i] + cnt[i];
   const uint32_t total = h.cell_start[cells];
   uint32_t* out = dev_alloc<uint32_t>(std::max<uint32_t>(total, 1));
+  float* out_key = dev_alloc<float>(std::max<uint32_t>(total, 1));
   HIP_CHECK(hipMemcpyAsync(starts, h.cell_start.data(), (cells + 1) * 4, hipMemcpyHostToDevice, c->stream));
   k_voxel_candidates<<<grid_for(cells), kBlock, 0, c->stream>>>(local, dtris, dnorm, (int)cells, 1, nullptr, nullptr,
-                                                                starts, out);
+                                                                starts, out, out_key, band);
   check_launch("k_voxel_candidates");
+  k_sort_candidates<<<(unsigned)std::min<int64_t>(cells, 4096), 256, 0, c->stream>>>((int)cells, starts, out, out_key);
+  check_launch("k_sort_candidates");
   h.cell_tris.resize(total);
-  if (total) HIP_CHECK(hipMemcpyAsync(h.cell_tris.data(), out, total * 4, hipMemcpyDeviceToHost, c->stream));
+  h.cell_key.resize(total);
+  if (total) {
+    HIP_CHECK(hipMemcpyAsync(h.cell_tris.data(), out, total * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(h.cell_key.data(), out_key, total * 4, hipMemcpyDeviceToHost, c->stream));
+  }
   HIP_CHECK(hipStreamSynchronize(c->stream));
   dev_free(dtris);
   dev_free(dnorm);
@@ -881,6 +933,7 @@ void build_candidates(mpm_ctx* c, ColliderHost& h, float pad) {
   dev_free(counts);
   dev_free(starts);
   dev_free(out);
+  dev_free(out_key);
   h.built_pad = pad;
 }
 
@@ -1545,6 +1598,7 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     HIP_CHECK(hipHostMalloc((void**)&c->h_vmax, sizeof(float), hipHostMallocDefault));
     *c->h_vmax = 0.0f;
     HIP_CHECK(hipHostGetDevicePointer((void**)&c->h_vmax_dev, c->h_vmax, 0));
+    HIP_CHECK(hipEventCreateWithFlags(&c->vmax_done, hipEventDisableTiming));
     reserve(c, 256);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -1567,7 +1621,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
                   (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_block_max, (void*)c->d_cols,
                   (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris,
-                  (void*)c->d_cell_far})
+                  (void*)c->d_cell_key, (void*)c->d_cell_far})
     dev_free(p);
   c->scratch.release();
   c->hist.release();
@@ -1578,6 +1632,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   }
   for (auto e : c->timing.pool) (void)hipEventDestroy(e);
   if (c->h_vmax) (void)hipHostFree(c->h_vmax);
+  if (c->vmax_done) (void)hipEventDestroy(c->vmax_done);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   (void)hipSetDevice(prev);
   delete c;
@@ -1719,7 +1774,14 @@ int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
       *out = mpm_advance_stats{0, frame_dt, 0};
       return;
     }
+    // the substep count divides by frame_dt (solver.py:183-185 raises on zero or non-finite values); a
+    // valid frame_dt always gives at least one substep, which consumes the binning enqueued below
+    if (frame_dt == 0.0 || !std::isfinite(frame_dt)) fail(MPM_ERR_VALUE, "frame_dt must be finite and nonzero");
     c->frame_dt = frame_dt;
+    // The first substep's binning needs the positions but not dt: it goes on the stream behind the speed
+    // reduction, so the GPU works through it while the host waits for the speed and picks dt.
+    request_speed(c);
+    const DriftLists pre = c->p2g_mode == MPM_P2G_ATOMIC ? DriftLists{} : bin_for_p2g(c);
     double vmax = (double)max_speed(c);
     if (!std::isfinite(vmax)) vmax = 0.0;  // already blown up: keep stepping so the NaN reaches x
     double wave = 0.0;
@@ -1728,7 +1790,8 @@ int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
     const double cfl_dt = std::min(0.4 * c->dom.dx_f64 / (wave + vmax), frame_dt);
     const int64_t steps = (int64_t)std::ceil(frame_dt / cfl_dt - 1e-9);
     const double dt = frame_dt / (double)steps;
-    for (int64_t k = 0; k < steps; ++k) substep(c, dt, (double)(k + 1) / (double)steps, frame_dt);
+    for (int64_t k = 0; k < steps; ++k)
+      substep(c, dt, (double)(k + 1) / (double)steps, frame_dt, k == 0 ? &pre : nullptr);
     // the removal count needs the positions (a host wait, only with open sides); otherwise the substeps
     // stay in flight and the host prepares the next call meanwhile (every read of the state waits for them)
     const int64_t removed = remove_escaped(c);

@@ -56,6 +56,9 @@ struct ColliderSet {
   MPM_GLOBAL const TriNormals* normals;
   MPM_GLOBAL const uint32_t* cell_start;  // per cell: first candidate; cell_start[cell + 1] ends it
   MPM_GLOBAL const uint32_t* cell_tris;   // candidate triangle indices (collider-local)
+  // per candidate: its distance from the cell's centre; each cell's list is in increasing order of it
+  // (ties by index), or all zero for a list too long to sort at build time
+  MPM_GLOBAL const float* cell_key;
   // per cell: a lower bound of the local distance from any point of the cell to the surface, signed by
   // the side of the cell's centre (0: the surface may come that close)
   MPM_GLOBAL const float* cell_far;
@@ -203,14 +206,14 @@ __device__ __forceinline__ void closest_on_triangle(const float (&p)[3], const f
   feature = 0;
 }
 
-__device__ __forceinline__ void scan_triangle(const ColliderSet& cs, const ColliderDev& c, int t, const float (&pl)[3],
-                                              float& best, int& best_t, int& best_f, float (&best_q)[3]) {
-  const float* v = cs.tris[c.tri_offset + t].v;
+__device__ __forceinline__ void scan_corners(const float* v, int t, const float (&pl)[3], float& best, int& best_t,
+                                             int& best_f, float (&best_q)[3]) {
   float q[3];
   int f;
   closest_on_triangle(pl, v, q, f);
   const float d2 = (pl[0] - q[0]) * (pl[0] - q[0]) + (pl[1] - q[1]) * (pl[1] - q[1]) + (pl[2] - q[2]) * (pl[2] - q[2]);
-  if (d2 < best) {
+  // the smallest (distance, index): the same triangle whatever order the candidates come in
+  if (d2 < best || (d2 == best && t < best_t)) {
     best = d2;
     best_t = t;
     best_f = f;
@@ -218,9 +221,16 @@ __device__ __forceinline__ void scan_triangle(const ColliderSet& cs, const Colli
   }
 }
 
+__device__ __forceinline__ void scan_triangle(const ColliderSet& cs, const ColliderDev& c, int t, const float (&pl)[3],
+                                              float& best, int& best_t, int& best_f, float (&best_q)[3]) {
+  scan_corners(cs.tris[c.tri_offset + t].v, t, pl, best, best_t, best_f, best_q);
+}
+
 // (outward normal, signed distance); phi = 1e30 when far (colliders.py:40-65)
 // `threshold`: the caller only acts on phi < threshold; a point provably farther (or, for an open mesh,
 // also beyond the solid thickness behind it) gets phi = 1e30 without the triangle scan
+// kRound: candidates loaded per round (more rounds in flight at once, more registers)
+template <int kRound>
 __device__ void collider_sdf(const ColliderSet& cs, const ColliderDev& c, const ColliderFrame& f, const float (&p)[3],
                              float (&n)[3], float& phi, float threshold) {
   n[0] = 0.0f; n[1] = 0.0f; n[2] = 1.0f;
@@ -241,15 +251,42 @@ __device__ void collider_sdf(const ColliderSet& cs, const ColliderDev& c, const 
     cell[a] = (int)floorf(u);
     in_grid = in_grid && u >= 0.0f && cell[a] < c.g_res[a];
   }
+  const int ci = in_grid ? c.cell_offset + (cell[0] * c.g_res[1] + cell[1]) * c.g_res[2] + cell[2] : 0;
+  const uint32_t a0 = in_grid ? cs.cell_start[ci] : 0u, a1 = in_grid ? cs.cell_start[ci + 1] : 0u;
   if (in_grid) {
-    const int ci = c.cell_offset + (cell[0] * c.g_res[1] + cell[1]) * c.g_res[2] + cell[2];
     // the world distance is at least smin x the local one: no contact possible from this cell
     const float far = cs.cell_far[ci];
     if (c.closed ? (far > 0.0f && far * f.smin >= threshold)
                  : (fabsf(far) * f.smin >= fmaxf(threshold, cs.thickness)))
       return;
-    const uint32_t a0 = cs.cell_start[ci], a1 = cs.cell_start[ci + 1];
-    for (uint32_t k = a0; k < a1; ++k) scan_triangle(cs, c, (int)cs.cell_tris[k], pl, best, bt, bf, q);
+  }
+  if (a1 > a0) {
+    // distance of the query from the cell's centre: no candidate t is closer to pl than key(t) - rp
+    float rp2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float cc = c.g_lo[a] + ((float)cell[a] + 0.5f) / c.g_inv_h;
+      rp2 += (pl[a] - cc) * (pl[a] - cc);
+    }
+    const float rp = sqrtf(rp2);
+    // kRound candidates per round, their indices and then their corners loaded together (fewer dependent
+    // round trips to memory). The list is sorted by key, so the scan ends at the first round whose key,
+    // less rp and a rounding allowance, is farther than the best distance so far: every later candidate
+    // is at least that far.
+    for (uint32_t k = a0; k < a1; k += kRound) {
+      const float key = cs.cell_key[k];
+      const float lb = key - rp - 1.0e-5f * (key + rp);
+      if (lb > 0.0f && lb * lb > best * 1.0001f) break;
+      int t[kRound];
+#pragma unroll
+      for (int j = 0; j < kRound; ++j) t[j] = (int)cs.cell_tris[k + j < a1 ? k + j : a1 - 1];
+      TriGeom tg[kRound];
+#pragma unroll
+      for (int j = 0; j < kRound; ++j) tg[j] = cs.tris[c.tri_offset + t[j]];
+#pragma unroll
+      for (int j = 0; j < kRound; ++j)
+        if (k + j < a1) scan_corners(tg[j].v, t[j], pl, best, bt, bf, q);
+    }
   } else {
     for (int t = 0; t < c.n_tris; ++t) scan_triangle(cs, c, t, pl, best, bt, bf, q);
   }
@@ -292,7 +329,7 @@ __device__ void collide_node_frames(const ColliderSet& cs, const ColliderFrame* 
   for (int c = 0; c < cs.count; ++c) {
     const ColliderDev col = cs.cols[c];
     float n[3], phi;
-    collider_sdf(cs, col, frames[c], pos, n, phi, cs.band);
+    collider_sdf<4>(cs, col, frames[c], pos, n, phi, cs.band);
     if (phi < cs.band) {
       float vcol[3];
       collider_velocity(frames[c], pos, vcol);
@@ -325,7 +362,8 @@ __device__ void project_particle(const ColliderSet& cs, const ColliderFrame* fra
   for (int c = 0; c < cs.count; ++c) {
     const ColliderDev col = cs.cols[c];
     float n[3], phi;
-    collider_sdf(cs, col, frames[c], pos, n, phi, 0.0f);
+    // one candidate per round: G2P's register budget; particle queries are rarer than node queries
+    collider_sdf<1>(cs, col, frames[c], pos, n, phi, 0.0f);
     if (phi < 0.0f) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) pos[a] -= phi * n[a];
@@ -358,7 +396,9 @@ __device__ __forceinline__ void voxel_centre(const ColliderDev& c, int cell, flo
   half_diag = 0.8660254f * h;
 }
 
-// pass 0: candidate count per voxel; pass 1: candidate indices, ascending
+// pass 0: candidate count per voxel; pass 1: candidate indices, ascending, and their distances from the
+// centre. Only cells whose centre lies within `band` of the surface get a list; a query in any other cell
+// (deep inside, or outside beyond every contact distance) scans all triangles.
 // pass 0 also writes the cell's far bound: (distance of the centre - half diagonal), shrunk by the same
 // slack, signed by the pseudonormal side of the centre's closest feature; 0 when not positive
 __global__ void __launch_bounds__(256) k_voxel_candidates(ColliderDev c, const TriGeom* __restrict__ tris,
@@ -366,7 +406,8 @@ __global__ void __launch_bounds__(256) k_voxel_candidates(ColliderDev c, const T
                                                           int pass, uint32_t* __restrict__ counts,
                                                           float* __restrict__ far,
                                                           const uint32_t* __restrict__ starts,
-                                                          uint32_t* __restrict__ out) {
+                                                          uint32_t* __restrict__ out, float* __restrict__ out_key,
+                                                          float band) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= n_cells) return;
   float p[3], hd;
@@ -400,14 +441,71 @@ __global__ void __launch_bounds__(256) k_voxel_candidates(ColliderDev c, const T
   // 1e-5 relative slack absorbs FP32 rounding of the distances
   const float lim = (sqrtf(dmin2) + 2.0f * hd) * 1.00001f + 1.0e-6f;
   const float lim2 = lim * lim;
+  if (sqrtf(dmin2) > band) {
+    if (pass == 0) counts[cell] = 0u;
+    return;
+  }
   if (pass == 0) {
     uint32_t cnt = 0;
     for (int t = 0; t < c.n_tris; ++t) cnt += tri_dist2(T, t, p) <= lim2 ? 1u : 0u;
     counts[cell] = cnt;
   } else {
     uint32_t w = starts[cell];
-    for (int t = 0; t < c.n_tris; ++t)
-      if (tri_dist2(T, t, p) <= lim2) out[w++] = (uint32_t)t;
+    for (int t = 0; t < c.n_tris; ++t) {
+      const float d2 = tri_dist2(T, t, p);
+      if (d2 <= lim2) {
+        out_key[w] = sqrtf(d2);
+        out[w++] = (uint32_t)t;
+      }
+    }
+  }
+}
+
+// Sorts each cell's candidates by (key, index), one workgroup per cell (grid-stride): a bitonic sort in
+// LDS of up to kSortCand entries. A longer list keeps index order with every key zero (a valid lower
+// bound that never ends a scan early).
+constexpr int kSortCand = 2048;
+__global__ void __launch_bounds__(256) k_sort_candidates(int n_cells, const uint32_t* __restrict__ starts,
+                                                         uint32_t* __restrict__ idx, float* __restrict__ key) {
+  __shared__ float s_key[kSortCand];
+  __shared__ uint32_t s_idx[kSortCand];
+  for (int cell = blockIdx.x; cell < n_cells; cell += gridDim.x) {
+    const uint32_t a0 = starts[cell], n = starts[cell + 1] - a0;
+    if (n <= 1) continue;
+    if (n > (uint32_t)kSortCand) {
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) key[a0 + i] = 0.0f;
+      continue;
+    }
+    uint32_t P = 2;
+    while (P < n) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+      s_key[i] = i < n ? key[a0 + i] : 3.0e38f;
+      s_idx[i] = i < n ? idx[a0 + i] : 0xffffffffu;
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+          const uint32_t j = i ^ stride;
+          if (j > i) {
+            const bool up = (i & size) == 0;
+            const float ki = s_key[i], kj = s_key[j];
+            const uint32_t ii = s_idx[i], ij = s_idx[j];
+            const bool greater = ki > kj || (ki == kj && ii > ij);
+            if (greater == up) {
+              s_key[i] = kj; s_key[j] = ki;
+              s_idx[i] = ij; s_idx[j] = ii;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      key[a0 + i] = s_key[i];
+      idx[a0 + i] = s_idx[i];
+    }
+    __syncthreads();
   }
 }
 
