@@ -4,7 +4,9 @@
 
 A step is one ``Solver.advance(dt)`` of the workload, which is exactly one substep (dt below the CFL
 step): collider pose upload, CFL speed reduction, P2G, grid update, collider contact, G2P, particle
-projection. Inputs are resident in HBM before the timed region; the timed region is bracketed by
+projection. Unless --steps is given, the timed steps run to the end of the workload's configured
+substeps (2000 for the multi-material mix, whose collider reaches the material after about 300), after
+the warm-up and the two profile passes. Inputs are resident in HBM before the timed region; the timed region is bracketed by
 device synchronisation. Per-kernel device time comes from HIP events recorded by the library on its
 own stream around every launch.
 
@@ -80,7 +82,9 @@ def log(msg: str) -> None:
 def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the rest of the workload's configured run, so every phase of "
+                         "it, collider contact included, is in the measurement)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="multi_material", choices=["multi_material", "elastic_block"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
@@ -267,6 +271,8 @@ def main(argv=None) -> int:
             raise RuntimeError(f"advance({w.dt}) took {st.substeps} substeps; the workload assumes one")
         t_sim += w.dt
 
+    if args.steps is None:
+        args.steps = max(1, w.substeps - args.warmup - 2 * PROFILE_STEPS)
     for _ in range(args.warmup):
         step()
     solver.synchronize()
@@ -364,7 +370,9 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": w.name, "particles": n, "grid": list(w.domain.grid_shape), "dt": w.dt,
                    "materials": [m.model for m in w.materials], "colliders": len(w.colliders),
-                   "substeps_per_step": 1, "svd_precision": args.precision},
+                   "substeps_per_step": 1, "svd_precision": args.precision,
+                   "timed_substeps": [args.warmup + 2 * PROFILE_STEPS,
+                                      args.warmup + 2 * PROFILE_STEPS + args.steps]},
         "roofline": roofline,
         "cpu_baseline": baseline,
         "kernels": kernels,

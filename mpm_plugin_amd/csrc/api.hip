@@ -6,8 +6,9 @@
 //
 // Substep on the default (binned) path:
 //   [every sort_interval substeps, or after any state change]
-//     k_cell_keys -> radix passes (k_radix_hist, scan, k_radix_scatter) -> k_cell_bounds
-//     -> k_brick_cost, k_brick_order (occupied bricks, costliest first) -> k_reorder
+//     k_cell_keys -> radix passes (k_radix_hist, scan, k_radix_scatter) -> k_brick_bounds
+//     -> k_brick_cost (cell offsets, cost class), k_brick_order (occupied bricks, costliest first)
+//     -> k_reorder
 //   [between sorts] k_drift (particles that changed cell -> incoming lists of their current cell)
 //   k_p2g_cells    (one wave per occupied brick, one lane per cell, wave-private LDS window)
 //   k_grid_bricks  (dirty bricks: update, walls, colliders)
@@ -482,10 +483,34 @@ void scan_u32(Scratch& scan_sums, hipStream_t stream, const uint32_t* in, uint32
 
 // ---- binning ---------------------------------------------------------------------------------
 // the occupied-brick list (costliest bricks first) and the brick -> slot map, from the cell offsets
-void active_bricks(mpm_ctx* c) {
-  k_brick_cost<<<(unsigned)((c->n_bricks + 3) / 4), 256, 0, c->stream>>>(c->offsets, c->n_bricks, c->slot);
+// cell offsets from the sorted keys (brick boundaries, then each brick's cells within its own range), the
+// bricks' cost classes and the occupied-brick list
+// the radix sort's digit histograms, then the brick classes' per-workgroup counts (one buffer, sized once
+// for both so that no reallocation waits on the stream between them)
+uint32_t* sort_hist(mpm_ctx* c, int64_t tiles) {
+  const size_t words = std::max<size_t>((size_t)256 * (size_t)tiles, (size_t)kCostClasses * kBrickGroups);
+  return static_cast<uint32_t*>(c->hist.get(words * sizeof(uint32_t)));
+}
+
+void bin_offsets_and_bricks(mpm_ctx* c, const uint32_t* sorted_keys, int64_t n) {
+  const int64_t groups = (c->n_bricks + kBrickGroup - 1) / kBrickGroup;
+  if (groups > kBrickGroups) fail(MPM_ERR_VALUE, "grid too large for the brick ordering (at most 2^18 bricks)");
+  uint32_t* hist = sort_hist(c, (n + kSortTile - 1) / kSortTile);
+  k_brick_bounds<<<grid_for(c->n_bricks + 1), kBlock, 0, c->stream>>>(sorted_keys, (int)n, (int)c->n_bricks,
+                                                                      c->offsets);
+  check_launch("k_brick_bounds");
+  if (groups == 0) {  // no bricks: nothing occupied
+    k_zero_u32<<<1, 64, 0, c->stream>>>(c->n_active, 1);
+    check_launch("k_zero_u32");
+    return;
+  }
+  k_brick_cost<<<(unsigned)((c->n_bricks + 3) / 4), 256, 0, c->stream>>>(sorted_keys, (int)c->n_bricks, c->offsets,
+                                                                         c->slot);
   check_launch("k_brick_cost");
-  k_brick_order<<<1, kOrderThreads, 0, c->stream>>>(c->n_bricks, c->active, c->n_active, c->slot);
+  k_brick_rank<<<(unsigned)groups, kBrickGroup, 0, c->stream>>>((int)c->n_bricks, c->slot, hist);
+  check_launch("k_brick_rank");
+  k_brick_order<<<(unsigned)groups, kBrickGroup, 0, c->stream>>>((int)c->n_bricks, hist, c->active, c->n_active,
+                                                                 c->slot);
   check_launch("k_brick_order");
 }
 
@@ -495,9 +520,7 @@ void sort_particles(mpm_ctx* c) {
   k_zero_u32<<<1, 64, 0, c->stream>>>(c->n_active, 1);
   check_launch("k_zero_u32");
   if (n == 0) {
-    k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(c->keys1, 0, c->n_cells, c->offsets);
-    check_launch("k_cell_bounds");
-    active_bricks(c);
+    bin_offsets_and_bricks(c, c->keys1, 0);
     c->sorted = true;
     c->steps_since_sort = 0;
     return;
@@ -506,7 +529,7 @@ void sort_particles(mpm_ctx* c) {
   check_launch("k_cell_keys");
   const int passes = (c->key_bits + 7) / 8;
   const int tiles = (int)((n + kSortTile - 1) / kSortTile);
-  uint32_t* hist = static_cast<uint32_t*>(c->hist.get((size_t)256 * tiles * sizeof(uint32_t)));
+  uint32_t* hist = sort_hist(c, tiles);
   const uint32_t* kin = c->keys0;
   const uint32_t* vin = c->vals2;
   for (int pass = 0; pass < passes; ++pass) {
@@ -520,9 +543,7 @@ void sort_particles(mpm_ctx* c) {
     kin = kout;
     vin = vout;
   }
-  k_cell_bounds<<<grid_for(c->n_cells + 1), kBlock, 0, c->stream>>>(kin, (int)n, c->n_cells, c->offsets);
-  check_launch("k_cell_bounds");
-  active_bricks(c);
+  bin_offsets_and_bricks(c, kin, n);
   k_reorder<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.view(), c->B.view(), vin, (int)n);
   check_launch("k_reorder");
   std::swap(c->A, c->B);
@@ -604,7 +625,9 @@ void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   c->n_fix_cleared = true;
 }
 
-void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
+// reduce_speed: the caller wants this G2P's speed maximum next (the last substep of an advance); with the
+// FP64 pass on, that pass reduces it into h_vmax on the way and the request is marked as enqueued
+void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_speed = false) {
   if (c->n == 0) return;
   const bool fix = c->sc.precision != kPrecisionFp32;
   if (fix && !c->n_fix_cleared) {  // a G2P without the grid update before it (a debug hook)
@@ -619,23 +642,30 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
                                                     FixList{c->fix_list, c->n_fix});
     check_launch("k_g2p");
   }
+  const bool fused = fix && reduce_speed;
   if (fix) {
     // a persistent grid: the deferred particles are usually few
     Timed t(c, MPM_K_OTHER);
     k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 1024), kBlock, 0, c->stream>>>(
-        c->parts(), c->d_mats, c->fix_list, c->n_fix);
+        c->parts(), c->d_mats, c->fix_list, c->n_fix, block_max(c), (int)grid_for(c->n),
+        fused ? c->h_vmax_dev : nullptr);
     check_launch("k_fp64_fixup");
   }
   c->block_max_count = grid_for(c->n);
   c->vmax_valid = true;
   c->vmax_pending = false;  // a reduction enqueued before this G2P is of the old state
+  if (fused) {
+    HIP_CHECK(hipEventRecord(c->vmax_done, c->stream));
+    c->vmax_pending = true;
+  }
 }
 
-void substep(mpm_ctx* c, double dt, double s, double frame_dt, const DriftLists* pre = nullptr) {
+void substep(mpm_ctx* c, double dt, double s, double frame_dt, const DriftLists* pre = nullptr,
+             bool reduce_speed = false) {
   const float dtf = (float)dt, sf = (float)s, inv_fdt = (float)(1.0 / frame_dt);
   run_p2g(c, dtf, pre);
   run_grid(c, dtf, sf, inv_fdt);
-  run_g2p(c, dtf, sf, inv_fdt);
+  run_g2p(c, dtf, sf, inv_fdt, reduce_speed);
   c->steps_since_sort += 1;
   c->version += 1;
 }
@@ -1791,7 +1821,7 @@ int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
     const int64_t steps = (int64_t)std::ceil(frame_dt / cfl_dt - 1e-9);
     const double dt = frame_dt / (double)steps;
     for (int64_t k = 0; k < steps; ++k)
-      substep(c, dt, (double)(k + 1) / (double)steps, frame_dt, k == 0 ? &pre : nullptr);
+      substep(c, dt, (double)(k + 1) / (double)steps, frame_dt, k == 0 ? &pre : nullptr, k == steps - 1);
     // the removal count needs the positions (a host wait, only with open sides); otherwise the substeps
     // stay in flight and the host prepares the next call meanwhile (every read of the state waits for them)
     const int64_t removed = remove_escaped(c);

@@ -310,8 +310,8 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__
   return (uint32_t)lo;
 }
 
-// offsets[c] for c in [0, n_cells] (k_cell_bounds); the thread of a brick's first cell also appends the
-// brick to the occupied list when its 64 cells hold a particle
+// offsets[c] for c in [0, n_cells] (first sorted position with key >= c); the thread of a brick's first
+// cell also appends the brick to the occupied list when its 64 cells hold a particle
 __global__ void __launch_bounds__(256) k_batch_cell_bounds(const SceneDev* __restrict__ sd,
                                                            const uint32_t* __restrict__ pre, int n_scenes) {
   const uint32_t* P0 = pre + kPreCellBlocks * (n_scenes + 1);

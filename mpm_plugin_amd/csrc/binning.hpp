@@ -177,73 +177,118 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* __restrict_
     if (base + k < n) out[base + k] += add;
 }
 
-// offsets[c] = first sorted position with key >= c, for c in [0, n_cells] (binary search per cell)
-__global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict__ sorted_keys, int n, int n_cells,
-                                                     uint32_t* __restrict__ offsets) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > n_cells) return;
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if ((int)sorted_keys[mid] < c) lo = mid + 1;
-    else hi = mid;
-  }
-  offsets[c] = (uint32_t)lo;
-}
-
 // Occupied-brick list in decreasing order of work. A P2G wave takes one brick and its lanes walk their
 // cells' particles in lockstep, so a brick costs in proportion to its fullest cell; the waves of the
 // costliest bricks start first and the cheap ones fill the machine behind them (longest-processing-time
 // order). Cost class q = 15 - min(fullest cell, 15): class 0 first.
 constexpr int kCostClasses = 16;
 
-// pass 1, one wave per brick (lane = cell): the cost class into slot[b] (-1: no particles)
-__global__ void __launch_bounds__(256) k_brick_cost(const uint32_t* __restrict__ offsets, int n_bricks,
-                                                    int32_t* __restrict__ slot) {
+// first sorted position with key >= k (binary search of [lo, hi))
+__device__ __forceinline__ uint32_t lower_bound_key(const uint32_t* __restrict__ keys, uint32_t lo, uint32_t hi,
+                                                    uint32_t k) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// offsets[64 b] for every brick boundary b in [0, n_bricks] (the last one is n)
+__global__ void __launch_bounds__(256) k_brick_bounds(const uint32_t* __restrict__ sorted_keys, int n, int n_bricks,
+                                                      uint32_t* __restrict__ offsets) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > n_bricks) return;
+  offsets[64 * (size_t)b] = lower_bound_key(sorted_keys, 0u, (uint32_t)n, 64u * (uint32_t)b);
+}
+
+// Occupied bricks are ordered by a counting sort on the cost class, stable in brick index, with no atomics:
+//   k_brick_cost   one wave per brick (lane = cell): the brick's cell offsets and class into slot[b]
+//   k_brick_rank   workgroup g (256 bricks): each brick's rank within its class in the workgroup, by
+//                  ballots (slot[b] = class | rank << 4), and the class counts hist[q][g]
+//   k_brick_order  every workgroup scans hist class-major (it is small) and places its own bricks
+constexpr int kBrickGroup = 256;    // bricks per k_brick_rank / k_brick_order workgroup
+constexpr int kBrickGroups = 1024;  // most workgroups (2^18 bricks): hist fits k_brick_order's LDS
+
+// After k_brick_bounds. Each cell's offset is searched within its brick's own range (nothing to search in
+// an empty brick); the brick's cost is its fullest cell.
+__global__ void __launch_bounds__(256) k_brick_cost(const uint32_t* __restrict__ sorted_keys, int n_bricks,
+                                                    uint32_t* __restrict__ offsets, int32_t* __restrict__ slot) {
   const int b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (b >= n_bricks) return;
-  const uint32_t* o = offsets + 64 * (size_t)b;
-  uint32_t most = o[lane + 1] - o[lane];
+  uint32_t* o = offsets + 64 * (size_t)b;
+  const uint32_t lo = o[0], hi = o[64];
+  const uint32_t mine = lane == 0 ? lo : lower_bound_key(sorted_keys, lo, hi, 64u * (uint32_t)b + (uint32_t)lane);
+  if (lane > 0) o[lane] = mine;
+  const uint32_t next = (uint32_t)__shfl_down((int)mine, 1);
+  uint32_t most = (lane == 63 ? hi : next) - mine;
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) most = max(most, (uint32_t)__shfl_xor((int)most, d));
   if (lane == 0)
     slot[b] = most == 0 ? -1 : kCostClasses - 1 - (int)min(most, (uint32_t)(kCostClasses - 1));
 }
 
-// pass 2, one workgroup: bricks per class (LDS counters: the few classes would serialise global atomics),
-// class offsets, then each occupied brick's position = its class offset + its rank within the class;
-// writes active[], slot[] (brick -> position) and the count
-constexpr int kOrderThreads = 1024;
-__global__ void __launch_bounds__(kOrderThreads) k_brick_order(int n_bricks, uint32_t* __restrict__ active,
-                                                              uint32_t* __restrict__ n_active,
-                                                              int32_t* __restrict__ slot) {
-  __shared__ uint32_t count[kCostClasses], start[kCostClasses];
-  if (threadIdx.x < kCostClasses) count[threadIdx.x] = 0u;
-  __syncthreads();
-#pragma unroll 4
-  for (int b = threadIdx.x; b < n_bricks; b += kOrderThreads) {
-    const int32_t q = slot[b];
-    if (q >= 0) atomicAdd(&count[q], 1u);
+__global__ void __launch_bounds__(kBrickGroup) k_brick_rank(int n_bricks, int32_t* __restrict__ slot,
+                                                           uint32_t* __restrict__ hist) {
+  constexpr int kWaves = kBrickGroup / 64;
+  __shared__ uint32_t s_cnt[kCostClasses][kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int b = blockIdx.x * kBrickGroup + threadIdx.x;
+  const int32_t cls = b < n_bricks ? slot[b] : -1;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  uint32_t in_wave = 0;
+#pragma unroll
+  for (int q = 0; q < kCostClasses; ++q) {
+    const uint64_t m = __ballot(cls == q);
+    if (cls == q) in_wave = (uint32_t)__popcll(m & lt);
+    if (lane == 0) s_cnt[q][wave] = (uint32_t)__popcll(m);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (int q = 0; q < kCostClasses; ++q) {
-      start[q] = acc;
-      acc += count[q];
-      count[q] = 0u;
+  if (cls >= 0) {
+    uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += s_cnt[cls][w];
+    slot[b] = cls | (int32_t)((before + in_wave) << 4);
+  }
+  if (threadIdx.x < kCostClasses) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += s_cnt[threadIdx.x][w];
+    hist[threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Writes active[] (bricks by class, then brick index), slot[] (brick -> position) and the count
+__global__ void __launch_bounds__(kBrickGroup) k_brick_order(int n_bricks, const uint32_t* __restrict__ hist,
+                                                            uint32_t* __restrict__ active,
+                                                            uint32_t* __restrict__ n_active,
+                                                            int32_t* __restrict__ slot) {
+  __shared__ uint32_t s_off[kCostClasses * kBrickGroups];
+  __shared__ uint32_t s_warp[kBrickGroup / 64 + 1];
+  const int groups = gridDim.x, entries = kCostClasses * groups;
+  const int per_thread = (entries + kBrickGroup - 1) / kBrickGroup;
+  uint32_t sum = 0;
+  for (int k = 0; k < per_thread; ++k) {
+    const int e = threadIdx.x * per_thread + k;
+    sum += e < entries ? hist[e] : 0u;
+  }
+  uint32_t total;
+  uint32_t acc = block_exclusive_scan(sum, s_warp, total);
+  for (int k = 0; k < per_thread; ++k) {
+    const int e = threadIdx.x * per_thread + k;
+    if (e < entries) {
+      s_off[e] = acc;
+      acc += hist[e];
     }
-    *n_active = acc;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *n_active = total;
   __syncthreads();
-#pragma unroll 4
-  for (int b = threadIdx.x; b < n_bricks; b += kOrderThreads) {
-    const int32_t q = slot[b];
-    if (q < 0) continue;
-    const uint32_t pos = start[q] + atomicAdd(&count[q], 1u);
-    active[pos] = (uint32_t)b;
-    slot[b] = (int32_t)pos;
-  }
+  const int b = blockIdx.x * kBrickGroup + threadIdx.x;
+  if (b >= n_bricks) return;
+  const int32_t packed = slot[b];
+  if (packed < 0) return;
+  const uint32_t pos = s_off[(packed & (kCostClasses - 1)) * groups + blockIdx.x] + ((uint32_t)packed >> 4);
+  active[pos] = (uint32_t)b;
+  slot[b] = (int32_t)pos;
 }
 
 // gather every per-slot particle array into sorted order: dst[i] = src[perm[i]]

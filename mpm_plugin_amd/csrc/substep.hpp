@@ -543,10 +543,30 @@ __device__ __forceinline__ void material_update_fp64(const MatDev& m, Particles&
   store_tau_pm(P, p, t, m.p_mass);
 }
 
-// FP64 pass over the deferred particles (grid-stride over the work list)
+// one workgroup of 256: the largest of count per-workgroup maxima into *out (NaN counts as +inf)
+__device__ void reduce_block_maxima(const float* __restrict__ block_max, int count, float* __restrict__ out) {
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    const float v = block_max[i];
+    m = (v == v) ? fmaxf(m, v) : __int_as_float(0x7f800000);
+  }
+  __shared__ float s_max[4];
+  const float wm = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+  __syncthreads();
+}
+
+// FP64 pass over the deferred particles (grid-stride over the work list). With vmax_out, workgroup 0
+// first reduces G2P's per-workgroup speed maxima into it (the pass changes no velocity), which saves the
+// reduction its own launch.
 __global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* __restrict__ mats,
                                                     const uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ count) {
+                                                    const uint32_t* __restrict__ count,
+                                                    const float* __restrict__ block_max, int block_count,
+                                                    float* __restrict__ vmax_out) {
+  if (vmax_out && blockIdx.x == 0) reduce_block_maxima(block_max, block_count, vmax_out);
   const uint32_t n = *count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t p = list[i];
@@ -696,16 +716,7 @@ __global__ void __launch_bounds__(256) k_speed_partials(Particles P, int n, floa
 // one workgroup: the maximum of the per-workgroup maxima
 __global__ void __launch_bounds__(256) k_reduce_max(const float* __restrict__ block_max, int count,
                                                     float* __restrict__ out) {
-  float m = 0.0f;
-  for (int i = threadIdx.x; i < count; i += blockDim.x) {
-    const float v = block_max[i];
-    m = (v == v) ? fmaxf(m, v) : __int_as_float(0x7f800000);
-  }
-  __shared__ float s_max[4];
-  const float wm = wave_max(m);
-  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
-  __syncthreads();
-  if (threadIdx.x == 0) *out = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+  reduce_block_maxima(block_max, count, out);
 }
 
 }  // namespace mpm
