@@ -6,14 +6,18 @@ A step is one ``Solver.advance(dt)`` of the workload, which is exactly one subst
 step): collider pose upload, CFL speed reduction, P2G, grid update, collider contact, G2P, particle
 projection. Unless --steps is given, the timed steps run to the end of the workload's configured
 substeps (2000 for the multi-material mix, whose collider reaches the material after about 300), after
-the warm-up and the two profile passes. Inputs are resident in HBM before the timed region; the timed region is bracketed by
+the warm-up and the profile pass. P2G's mean launch time comes from a second run of the same substeps with
+only P2G bracketed. Inputs are resident in HBM before the timed region; the timed region is bracketed by
 device synchronisation. Per-kernel device time comes from HIP events recorded by the library on its
 own stream around every launch.
 
 A second measurement (skipped with --no-batch) runs the batched engine on the 256-scene mix
 (mpm_plugin_amd.workloads.scene_mix: grid edges 32-128, 5k-400k particles per scene, four material models,
-0-2 colliders each): the same frames advanced once one scene at a time (``Solver.advance`` per scene, in
-turn) and once through one ``SceneBatch``; both counts are sums of particles x substeps actually run.
+0-2 colliders each): the same frames advanced one scene at a time (``Solver.advance`` per scene, in turn,
+each scene's frame finished before the next starts: ``plain``), with the same calls not waiting for each
+other (each scene's stream may overlap the next scenes' on the GPU: ``plain_overlapped``), and through one
+``SceneBatch``; every count is a sum of particles x substeps actually run. ``speedup`` is batched over
+plain.
 
 A third measurement (skipped with --no-serve) runs the bake server (mpm_core.serve.BakeServer) on
 requests arriving as a Poisson process at 20 per second, each drawn like the scene mix with a
@@ -175,6 +179,16 @@ def batch_bench(n_scenes: int, frames: int, tune: bool = True) -> dict:
               "frames": frames, "frame_dt": mix[0].frame_dt}
 
     def plain(pairs):
+        # one scene at a time: each scene's frame ends before the next scene starts
+        out = []
+        for w, s in pairs:
+            out.append(s.advance(w.frame_dt))
+            s.synchronize()
+        return out
+
+    def overlapped(pairs):
+        # the same calls in turn without waiting: each scene's substeps run on its own stream and the
+        # GPU may overlap them with the next scenes'
         return [s.advance(w.frame_dt) for w, s in pairs]
 
     pairs = [(w, w.make_solver(device="cuda:0")) for w in mix]
@@ -182,6 +196,11 @@ def batch_bench(n_scenes: int, frames: int, tune: bool = True) -> dict:
     secs, work = run_frames(pairs, frames, 1, plain)
     result["plain"] = {"value": work / secs, "seconds": round(secs, 3), "particle_substeps": int(work)}
     log(f"plain: {work / secs:.3e} particle-substeps/s ({secs:.2f} s)")
+    secs, work = run_frames(pairs, frames, 1 + frames, overlapped)
+    for _, s in pairs:
+        s.synchronize()
+    result["plain_overlapped"] = {"value": work / secs, "seconds": round(secs, 3), "particle_substeps": int(work)}
+    log(f"plain, scenes overlapping on their streams: {work / secs:.3e} particle-substeps/s ({secs:.2f} s)")
     for _, s in pairs:
         s.close()
     del pairs
@@ -208,6 +227,7 @@ def batch_bench(n_scenes: int, frames: int, tune: bool = True) -> dict:
                          "substep_launches": int(batch.substeps_launched - launched)}
     log(f"batched: {work / secs:.3e} particle-substeps/s ({secs:.2f} s)")
     result["speedup"] = result["batched"]["value"] / result["plain"]["value"]
+    result["speedup_vs_overlapped"] = result["batched"]["value"] / result["plain_overlapped"]["value"]
     batch.close()
     for _, s in pairs:
         s.close()
@@ -255,24 +275,29 @@ def main(argv=None) -> int:
         return 2
     w = build_workload(args.workload, with_collider=not args.no_collider)
     log(f"workload {w.name}: {w.n} particles, grid {w.domain.grid_shape}, dt {w.dt}")
-    solver = w.make_solver(device="cuda:0")
     from mpm_plugin_amd import _native as N
 
-    solver.set_option(N.OPT_PRECISION, {"fp32": N.PRECISION_FP32, "fallback": N.PRECISION_FALLBACK,
-                                        "fp64": N.PRECISION_FP64}[args.precision])
-    t_sim = 0.0
+    def make_run():
+        """A fresh solver of the workload and its step function (one advance of dt with the pose update)."""
+        solver = w.make_solver(device="cuda:0")
+        solver.set_option(N.OPT_PRECISION, {"fp32": N.PRECISION_FP32, "fallback": N.PRECISION_FALLBACK,
+                                            "fp64": N.PRECISION_FP64}[args.precision])
+        t_sim = 0.0
 
-    def step():
-        nonlocal t_sim
-        if w.colliders:
-            solver.set_collider_transforms(*w.collider_transforms(t_sim, t_sim + w.dt))
-        st = solver.advance(w.dt)
-        if st.substeps != 1:
-            raise RuntimeError(f"advance({w.dt}) took {st.substeps} substeps; the workload assumes one")
-        t_sim += w.dt
+        def step():
+            nonlocal t_sim
+            if w.colliders:
+                solver.set_collider_transforms(*w.collider_transforms(t_sim, t_sim + w.dt))
+            st = solver.advance(w.dt)
+            if st.substeps != 1:
+                raise RuntimeError(f"advance({w.dt}) took {st.substeps} substeps; the workload assumes one")
+            t_sim += w.dt
 
+        return solver, step
+
+    solver, step = make_run()
     if args.steps is None:
-        args.steps = max(1, w.substeps - args.warmup - 2 * PROFILE_STEPS)
+        args.steps = max(1, w.substeps - args.warmup - PROFILE_STEPS)
     for _ in range(args.warmup):
         step()
     solver.synchronize()
@@ -290,15 +315,8 @@ def main(argv=None) -> int:
     log(f"profile pass ({PROFILE_STEPS} steps), kernel ms: "
         + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in times.items()))
 
-    # P2G's launch time with only P2G bracketed (every bracketed kernel adds its events' cost to the step)
-    solver.reset_timing()
-    solver.set_timing(True, phases=["p2g"])
-    for _ in range(PROFILE_STEPS):
-        step()
-    solver.set_timing(False)
-    timed = {"p2g": solver.kernel_times()["p2g"]}
-
-    # the timed steps bracket only G2P
+    # the timed steps bracket only G2P (every bracketed kernel adds its events' cost to the step)
+    timed = {}
     solver.reset_timing()
     solver.set_timing(True, phases=["g2p"])
     solver.synchronize()
@@ -309,10 +327,21 @@ def main(argv=None) -> int:
     elapsed = time.perf_counter() - t0
     solver.set_timing(False)
     timed["g2p"] = solver.kernel_times()["g2p"]
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; g2p {timed['g2p'][0]:.2f} ms / {timed['g2p'][1]} launches "
-        f"(p2g {timed['p2g'][0]:.2f} ms / {timed['p2g'][1]} launches in its own pass)")
-
     n = solver.particle_count
+    solver.close()
+
+    # P2G's launch time over the same substeps, from a second run of the workload with only P2G bracketed
+    solver, step = make_run()
+    for _ in range(args.warmup + PROFILE_STEPS):
+        step()
+    solver.reset_timing()
+    solver.set_timing(True, phases=["p2g"])
+    for _ in range(args.steps):
+        step()
+    solver.set_timing(False)
+    timed["p2g"] = solver.kernel_times()["p2g"]
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; g2p {timed['g2p'][0]:.2f} ms / {timed['g2p'][1]} launches "
+        f"(p2g {timed['p2g'][0]:.2f} ms / {timed['p2g'][1]} launches over the same substeps of a second run)")
     bytes_per_launch = {
         "p2g": P2G_BYTES_PARTICLE * n + P2G_BYTES_SNOW * n_snow + P2G_BYTES_NODE * n_active,
         "g2p": G2P_BYTES_PARTICLE * n + G2P_BYTES_SNOW * n_snow + G2P_BYTES_NODE * n_active,
@@ -371,8 +400,7 @@ def main(argv=None) -> int:
         "config": {"workload": w.name, "particles": n, "grid": list(w.domain.grid_shape), "dt": w.dt,
                    "materials": [m.model for m in w.materials], "colliders": len(w.colliders),
                    "substeps_per_step": 1, "svd_precision": args.precision,
-                   "timed_substeps": [args.warmup + 2 * PROFILE_STEPS,
-                                      args.warmup + 2 * PROFILE_STEPS + args.steps]},
+                   "timed_substeps": [args.warmup + PROFILE_STEPS, args.warmup + PROFILE_STEPS + args.steps]},
         "roofline": roofline,
         "cpu_baseline": baseline,
         "kernels": kernels,
