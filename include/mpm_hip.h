@@ -143,10 +143,13 @@ int32_t mpm_debug_sort(mpm_ctx* ctx);
 /* ---- options ------------------------------------------------------------------------------ */
 #define MPM_OPT_P2G_MODE 0       /* MPM_P2G_BINNED (default), MPM_P2G_ATOMIC or MPM_P2G_FIXED */
 #define MPM_OPT_SORT_INTERVAL 1  /* substeps between brick re-sorts (default 16; >= 1) */
-#define MPM_OPT_P2G_BLOCKS 2     /* workgroups (one wave each) of the binned P2G launch (0: automatic) */
+#define MPM_OPT_P2G_BLOCKS 2     /* workgroups of the binned P2G launch (0: automatic) */
 #define MPM_OPT_PRECISION 3      /* SVD, return maps and stress: MPM_PRECISION_FP32, _FALLBACK (default:
                                     FP64 only where FP32 cannot resolve F: sigma_min/sigma_max < 1e-2 or
                                     J = Jp det F < 0.25), _FP64 */
+#define MPM_OPT_P2G_WAVES 4      /* waves per occupied brick in the binned P2G: 1, 2 (default) or 4; wave h of
+                                    n takes the particles h, h + n, ... of each cell (more waves in flight
+                                    for scenes with few bricks) */
 #define MPM_P2G_BINNED 0         /* per-brick LDS window, global FP32 atomics for the flush */
 #define MPM_P2G_ATOMIC 1         /* one thread per particle, 27 x 4 FP32 global atomics (reference flow) */
 #define MPM_P2G_FIXED 2          /* as BINNED, accumulated in 64-bit fixed point: run-to-run bit-exact grids */

@@ -184,6 +184,7 @@ struct mpm_ctx {
   double fx_scale = 0.0;
   int64_t sort_interval = 16;
   int64_t p2g_blocks = 0;
+  int p2g_waves = 2;             // waves per occupied brick in the binned P2G (1 or 2)
 
   // colliders
   std::vector<ColliderHost> colliders;
@@ -593,9 +594,22 @@ void run_p2g(mpm_ctx* c, float dt, const DriftLists* pre = nullptr) {
   Timed t(c, MPM_K_P2G);
   int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 16384;
   blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, std::min<int64_t>(c->n_bricks, c->n)));
-  k_p2g_cells<1><<<(unsigned)blocks, 64, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets, c->active,
-                                                       c->n_active, c->accum, c->dirty, c->fixed(), c->stage, dl);
-  check_launch("k_p2g_cells");
+  if (c->p2g_waves == 4) {
+    k_p2g_split<4><<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
+                                                            c->active, c->n_active, c->accum, c->dirty, c->fixed(),
+                                                            c->stage, dl);
+    check_launch("k_p2g_split");
+  } else if (c->p2g_waves == 2) {
+    k_p2g_split<2><<<(unsigned)blocks, 128, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
+                                                            c->active, c->n_active, c->accum, c->dirty, c->fixed(),
+                                                            c->stage, dl);
+    check_launch("k_p2g_split");
+  } else {
+    k_p2g_cells<1><<<(unsigned)blocks, 64, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
+                                                         c->active, c->n_active, c->accum, c->dirty, c->fixed(),
+                                                         c->stage, dl);
+    check_launch("k_p2g_cells");
+  }
 }
 
 // the colliders' frames at frame fraction s (the same function the batched plan runs on the device)
@@ -2019,6 +2033,10 @@ int32_t mpm_set_option(mpm_ctx* c, int32_t option, int64_t value) {
         if (value < 0) fail(MPM_ERR_VALUE, "block count must be >= 0");
         c->p2g_blocks = value;
         break;
+      case MPM_OPT_P2G_WAVES:
+        if (value != 1 && value != 2 && value != 4) fail(MPM_ERR_VALUE, "waves per brick must be 1, 2 or 4");
+        c->p2g_waves = (int)value;
+        break;
       default:
         fail(MPM_ERR_VALUE, "unknown option");
     }
@@ -2032,6 +2050,7 @@ int64_t mpm_get_option(const mpm_ctx* c, int32_t option) {
     case MPM_OPT_PRECISION: return c->sc.precision;
     case MPM_OPT_SORT_INTERVAL: return c->sort_interval;
     case MPM_OPT_P2G_BLOCKS: return c->p2g_blocks;
+    case MPM_OPT_P2G_WAVES: return c->p2g_waves;
     default: return -1;
   }
 }

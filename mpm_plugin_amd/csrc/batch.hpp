@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_batch_p2g(const SceneDev* __res
     const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
     // a scene that sorted this substep has no drifted particles (and k_batch_drift skipped it)
     const DriftLists dl = st[i].sort_now ? DriftLists{} : DriftLists{S.in_cnt, S.in_lst, S.fwd};
-    p2g_brick(S.set[cur], S.g, k, S.offsets, S.active[w - P0[i]], s_win[wave], S.accum, S.dirty, lane,
+    p2g_brick<1>(S.set[cur], S.g, k, S.offsets, S.active[w - P0[i]], &s_win[wave], S.accum, S.dirty, lane,
               FixedAccum{nullptr, 0.0, 0.0}, S.stage, w - P0[i], dl);
   }
 }

@@ -203,11 +203,18 @@ __device__ __forceinline__ void wave_lds_fence() {
 // of a wave own 64 different cells, so when they then add their sums into the wave's private LDS window
 // every store of one instruction hits a different node: plain read-add-write, no LDS atomics. The window
 // is flushed to the bricked HBM grid with global atomics (neighbouring windows overlap).
+// kSplit waves per brick (1, 2 or 4; above 1 the caller is a workgroup of kSplit waves on one brick, and
+// `half` is the wave): wave h takes the particles h, h + kSplit, ... of every cell's list into its own
+// window wins[h], and the workgroup stages the sum of the windows, added in wave order.
+template <int kSplit>
 __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g, float k,
-                                          const uint32_t* __restrict__ cell_off, uint32_t b, float (*win)[kWinNodes],
-                                          float4* __restrict__ accum, uint32_t* __restrict__ dirty, int lane,
-                                          const FixedAccum& fa, float4* __restrict__ stage, uint32_t slot,
-                                          const DriftLists& dl) {
+                                          const uint32_t* __restrict__ cell_off, uint32_t b,
+                                          float (*wins)[4][kWinNodes], float4* __restrict__ accum,
+                                          uint32_t* __restrict__ dirty, int lane, const FixedAccum& fa,
+                                          float4* __restrict__ stage, uint32_t slot, const DriftLists& dl,
+                                          int half = 0) {
+  static_assert(kSplit == 1 || kSplit == 2 || kSplit == 4, "1, 2 or 4 waves per brick");
+  float (*const win)[kWinNodes] = wins[half];
   const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
   const int bz = (int)(b % (uint32_t)g.bdims[2]);
   const int by = (int)((b / (uint32_t)g.bdims[2]) % (uint32_t)g.bdims[1]);
@@ -231,8 +238,8 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
     const uint32_t m = dl.cnt[c];
     n_in = m < (uint32_t)kInPerCell ? m : (uint32_t)kInPerCell;
     in_list = dl.lst + (size_t)c * kInPerCell;
-    dl.cnt[c] = 0u;
-    if (n_in > 1) {
+    if (kSplit == 1) dl.cnt[c] = 0u;  // two waves: cleared once both have read it (below)
+    if (n_in > 1 && half == 0) {
       // in particle order, so the lane's sums do not depend on the order of the routing atomics
       static_assert(kInPerCell == 4, "sorting network for 4 entries");
       uint32_t e[4];
@@ -253,6 +260,11 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
         if ((uint32_t)j < n_in) in_list[j] = e[j];
     }
   }
+  if (kSplit > 1) {
+    __threadfence_block();
+    __syncthreads();  // wave 0's sorted lists are visible to the others; all have read the counts
+    if (dl.cnt && half == 0) dl.cnt[slot * 64u + (uint32_t)lane] = 0u;
+  }
   const uint32_t total = own_n + n_in;
   float acc[3][3][3][4];
 #pragma unroll
@@ -266,18 +278,20 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
   // the next particle's inputs load while the current one is scattered
   P2GRaw next;
   bool next_fwd = false;
-  if (total > 0) {
-    const uint32_t pn = own_n > 0 ? p0 : in_list[0];
+  const uint32_t t0 = (uint32_t)half;
+  if (t0 < total) {
+    const uint32_t pn = t0 < own_n ? p0 + t0 : in_list[t0 - own_n];
     load_p2g_raw(P, (int64_t)pn, next);
-    if (dl.cnt && own_n > 0) next_fwd = dl.fwd[pn] != 0;
+    if (dl.cnt && t0 < own_n) next_fwd = dl.fwd[pn] != 0;
   }
-  for (uint32_t t = 0; t < total; ++t) {
+  for (uint32_t t = t0; t < total; t += kSplit) {
     const P2GRaw cur = next;
     const bool cur_skip = t < own_n && next_fwd;  // routed to its current cell's lane
-    if (t + 1 < total) {
-      const uint32_t pn = t + 1 < own_n ? p0 + t + 1 : in_list[t + 1 - own_n];
+    const uint32_t tn = t + kSplit;
+    if (tn < total) {
+      const uint32_t pn = tn < own_n ? p0 + tn : in_list[tn - own_n];
       load_p2g_raw(P, (int64_t)pn, next);
-      next_fwd = dl.cnt && t + 1 < own_n && dl.fwd[pn] != 0;
+      next_fwd = dl.cnt && tn < own_n && dl.fwd[pn] != 0;
     }
     P2GParticle q;
     make_p2g_particle(cur, g, k, q);
@@ -309,7 +323,7 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
   // offset, so the 27 read-add-writes must stay in program order across lanes: the fence after each
   // offset keeps the compiler from batching the reads of later offsets ahead of earlier writes (a
   // wave's LDS operations then complete in issue order). Lanes without particles add nothing.
-  const bool any = total > 0;
+  const bool any = total > t0;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -323,15 +337,26 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
         }
         wave_lds_fence();
       }
+  // the brick's window: this wave's, or the sum of the workgroup's (each thread then takes every
+  // (64 kSplit)-th node)
+  if (kSplit > 1) __syncthreads();
+  const int tid = half * 64 + lane;
+  const auto node_value = [&](int i, int c) {
+    float v = wins[0][c][i];
+#pragma unroll
+    for (int h = 1; h < kSplit; ++h) v += wins[h][c][i];
+    return v;
+  };
   if (stage && !fa.fx) {
     // the window goes to the brick's staging slot with plain stores; the grid update sums the (up to 8)
     // windows that cover each node, in a fixed order
     float4* out = stage + (int64_t)slot * kWinNodes;
-    for (int i = lane; i < kWinNodes; i += 64) out[i] = make_float4(win[0][i], win[1][i], win[2][i], win[3][i]);
-  } else for (int i = lane; i < kWinNodes; i += 64) {
+    for (int i = tid; i < kWinNodes; i += 64 * kSplit)
+      out[i] = make_float4(node_value(i, 0), node_value(i, 1), node_value(i, 2), node_value(i, 3));
+  } else for (int i = tid; i < kWinNodes; i += 64 * kSplit) {
     const int gi = wo[0] + (i >> 6), gj = wo[1] + ((i >> 3) & 7), gk = wo[2] + (i & 7);
     if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
-    const float vx = win[0][i], vy = win[1][i], vz = win[2][i], vm = win[3][i];
+    const float vx = node_value(i, 0), vy = node_value(i, 1), vz = node_value(i, 2), vm = node_value(i, 3);
     if (vx == 0.0f && vy == 0.0f && vz == 0.0f && vm == 0.0f) continue;
     if (fa.fx) {
       const int nd = node_addr(g, gi, gj, gk);
@@ -347,8 +372,9 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
     if (vz != 0.0f) atomicAdd(&node->z, vz);
     if (vm != 0.0f) atomicAdd(&node->w, vm);
   }
-  if (lane < 27) mark_brick(g, dirty, bx - 1 + lane / 9, by - 1 + (lane / 3) % 3, bz - 1 + lane % 3);
-  wave_lds_fence();
+  if (half == 0 && lane < 27) mark_brick(g, dirty, bx - 1 + lane / 9, by - 1 + (lane / 3) % 3, bz - 1 + lane % 3);
+  if (kSplit > 1) __syncthreads();  // every window read before the next brick clears them
+  else wave_lds_fence();
 }
 
 // P2G over the device-side list of occupied bricks: kWaves waves (bricks at a time) per workgroup,
@@ -365,7 +391,25 @@ __global__ void __launch_bounds__(64 * kWaves) k_p2g_cells(Particles P, GridGeom
   const uint32_t n_active = *n_active_ptr;
   const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
   for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves)
-    p2g_brick(P, g, k, cell_off, active[w], s_win[wave], accum, dirty, lane, fa, stage, w, dl);
+    p2g_brick<1>(P, g, k, cell_off, active[w], &s_win[wave], accum, dirty, lane, fa, stage, w, dl);
+}
+
+// kSplit waves per brick (one workgroup of 64 kSplit per brick at a time, grid-stride over the brick
+// list): each brick's chain of particle visits is 1/kSplit as long and kSplit times as many waves are in
+// flight, for scenes whose occupied bricks are too few to cover the chains' latency
+template <int kSplit>
+__global__ void __launch_bounds__(64 * kSplit) k_p2g_split(Particles P, GridGeom g, float p_vol, float dt,
+                                                           const uint32_t* __restrict__ cell_off,
+                                                           const uint32_t* __restrict__ active,
+                                                           const uint32_t* __restrict__ n_active_ptr,
+                                                           float4* __restrict__ accum, uint32_t* __restrict__ dirty,
+                                                           FixedAccum fa, float4* __restrict__ stage, DriftLists dl) {
+  __shared__ float s_win[kSplit][4][kWinNodes];
+  const int half = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t n_active = *n_active_ptr;
+  const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
+  for (uint32_t w = blockIdx.x; w < n_active; w += gridDim.x)
+    p2g_brick<kSplit>(P, g, k, cell_off, active[w], s_win, accum, dirty, lane, fa, stage, w, dl, half);
 }
 
 // grid update of one dirty brick by one wave (lane = node): momentum -> velocity, gravity, walls,
