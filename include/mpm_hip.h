@@ -203,13 +203,14 @@ int32_t mpm_batch_run(mpm_batch* batch, int64_t max_launches, int64_t* launched,
                       mpm_advance_stats* out);
 /* substeps done / in the frame in flight, per scene (equal when the scene is not in a frame) */
 int32_t mpm_batch_progress(const mpm_batch* batch, int32_t* substeps_done, int32_t* substeps_total);
-#define MPM_BATCH_OPT_P2G_BLOCKS 0        /* workgroups (one wave each) of the persistent P2G launch (0: automatic) */
+#define MPM_BATCH_OPT_P2G_BLOCKS 0        /* workgroups of the persistent P2G launch (0: automatic) */
 #define MPM_BATCH_OPT_GRAPH 1             /* 1: replay captured graphs (default); 0: launch directly */
 #define MPM_BATCH_OPT_SORT_INTERVAL 2     /* substeps between re-sorts for every scene (0: each scene's own) */
 #define MPM_BATCH_OPT_SUBSTEPS_LAUNCHED 3 /* read only: substep launches since creation */
 #define MPM_BATCH_OPT_G2P_BLOCKS 4        /* workgroups of the persistent G2P launch (0: automatic) */
 #define MPM_BATCH_OPT_GRID_BLOCKS 5       /* workgroups of the persistent grid-update launch (0: automatic) */
 #define MPM_BATCH_OPT_G2P_TILE 6          /* particles per G2P workgroup: 64, 128 or 256 (default) */
+#define MPM_BATCH_OPT_P2G_WAVES 7         /* waves per occupied brick in P2G: 1 (default), 2 or 4 */
 int32_t mpm_batch_set_option(mpm_batch* batch, int32_t option, int64_t value);
 int64_t mpm_batch_get_option(const mpm_batch* batch, int32_t option);
 

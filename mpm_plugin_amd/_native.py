@@ -19,7 +19,7 @@ OPT_P2G_MODE, OPT_SORT_INTERVAL, OPT_P2G_BLOCKS, OPT_PRECISION, OPT_P2G_WAVES = 
 P2G_BINNED, P2G_ATOMIC, P2G_FIXED = 0, 1, 2
 PRECISION_FP32, PRECISION_FALLBACK, PRECISION_FP64 = 0, 1, 2
 (BATCH_OPT_P2G_BLOCKS, BATCH_OPT_GRAPH, BATCH_OPT_SORT_INTERVAL, BATCH_OPT_SUBSTEPS_LAUNCHED, BATCH_OPT_G2P_BLOCKS,
- BATCH_OPT_GRID_BLOCKS, BATCH_OPT_G2P_TILE) = range(7)
+ BATCH_OPT_GRID_BLOCKS, BATCH_OPT_G2P_TILE, BATCH_OPT_P2G_WAVES) = range(8)
 KERNEL_NAMES = ("p2g", "grid", "g2p", "bin", "colliders", "other")
 
 

@@ -28,6 +28,7 @@ from mpm_plugin_amd import _native as N
 # knob -> candidate values (the first value of each list is the library default)
 BATCH_SPACE: dict[int, list[int]] = {
     N.BATCH_OPT_G2P_TILE: [256, 128, 64],
+    N.BATCH_OPT_P2G_WAVES: [1, 2],
     N.BATCH_OPT_G2P_BLOCKS: [2048, 1024, 4096, 8192],
     N.BATCH_OPT_P2G_BLOCKS: [8192, 4096, 16384],
     N.BATCH_OPT_GRID_BLOCKS: [2048, 512, 1024, 4096],
@@ -35,7 +36,7 @@ BATCH_SPACE: dict[int, list[int]] = {
 }
 KNOB_NAMES = {N.BATCH_OPT_G2P_TILE: "g2p_tile", N.BATCH_OPT_G2P_BLOCKS: "g2p_blocks",
               N.BATCH_OPT_P2G_BLOCKS: "p2g_blocks", N.BATCH_OPT_GRID_BLOCKS: "grid_blocks",
-              N.BATCH_OPT_SORT_INTERVAL: "sort_interval"}
+              N.BATCH_OPT_SORT_INTERVAL: "sort_interval", N.BATCH_OPT_P2G_WAVES: "p2g_waves"}
 
 
 def coordinate_descent(space: dict[int, Sequence[int]], measure: Callable[..., float],

@@ -272,6 +272,7 @@ struct mpm_batch {
   int64_t g2p_blocks = 0;
   int64_t grid_blocks = 0;
   int64_t g2p_tile = 256;
+  int64_t p2g_waves = 1;  // waves per occupied brick in P2G (the mix's many bricks fill the machine)
   int64_t use_graph = 1;
   bool fp64_pass = false;  // some scene defers particles to the FP64 pass (part of the captured sequence)
   int64_t sort_interval = 0;  // 0: each context's own
@@ -1155,7 +1156,13 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
   check_launch("k_batch_prefix");
   k_batch_drift<<<2048, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n, (int)b->g2p_tile);
   check_launch("k_batch_drift");
-  k_batch_p2g<1><<<batch_blocks(b->p2g_blocks, 8192), 64, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+  if (b->p2g_waves == 1) {
+    k_batch_p2g<1><<<batch_blocks(b->p2g_blocks, 8192), 64, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+  } else if (b->p2g_waves == 2) {
+    k_batch_p2g_split<2><<<batch_blocks(b->p2g_blocks, 8192), 128, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+  } else {
+    k_batch_p2g_split<4><<<batch_blocks(b->p2g_blocks, 8192), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
+  }
   check_launch("k_batch_p2g");
   k_batch_grid_scan<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
   check_launch("k_batch_grid_scan");
@@ -2186,6 +2193,10 @@ int32_t mpm_batch_set_option(mpm_batch* b, int32_t option, int64_t value) {
         if (value != 64 && value != 128 && value != 256) fail(MPM_ERR_VALUE, "G2P tile must be 64, 128 or 256");
         slot = &b->g2p_tile;
         break;
+      case MPM_BATCH_OPT_P2G_WAVES:
+        if (value != 1 && value != 2 && value != 4) fail(MPM_ERR_VALUE, "waves per brick must be 1, 2 or 4");
+        slot = &b->p2g_waves;
+        break;
       default:
         fail(MPM_ERR_VALUE, "unknown batch option");
     }
@@ -2209,6 +2220,8 @@ int64_t mpm_batch_get_option(const mpm_batch* b, int32_t option) {
       return batch_blocks(b->grid_blocks, 2048);
     case MPM_BATCH_OPT_G2P_TILE:
       return b->g2p_tile;
+    case MPM_BATCH_OPT_P2G_WAVES:
+      return b->p2g_waves;
     case MPM_BATCH_OPT_GRAPH:
       return b->use_graph;
     case MPM_BATCH_OPT_SORT_INTERVAL:

@@ -437,6 +437,28 @@ __global__ void __launch_bounds__(64 * kWaves) k_batch_p2g(const SceneDev* __res
   }
 }
 
+// The same with kSplit waves per brick (one workgroup per brick at a time; wave h takes every kSplit-th
+// particle of each cell): shorter chains per brick for the launches that carry only a few scenes
+template <int kSplit>
+__global__ void __launch_bounds__(64 * kSplit) k_batch_p2g_split(const SceneDev* __restrict__ sd,
+                                                                 const SceneStep* __restrict__ st,
+                                                                 const uint32_t* __restrict__ pre, int n_scenes) {
+  __shared__ float s_win[kSplit][4][kWinNodes];
+  const uint32_t* P0 = pre + kPreBricks * (n_scenes + 1);
+  const int half = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const uint32_t total = P0[n_scenes];
+  int i = 0;
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    i = uniform(scene_of(P0, n_scenes, w, i));
+    const SceneDev& S = sd[i];
+    const int cur = st[i].cur;
+    const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
+    const DriftLists dl = st[i].sort_now ? DriftLists{} : DriftLists{S.in_cnt, S.in_lst, S.fwd};
+    p2g_brick<kSplit>(S.set[cur], S.g, k, S.offsets, S.active[w - P0[i]], s_win, S.accum, S.dirty, lane,
+                      FixedAccum{nullptr, 0.0, 0.0}, S.stage, w - P0[i], dl, half);
+  }
+}
+
 // Grid update, first pass: one wave per group of 64 bricks of a running scene; each lane reads one
 // brick's dirty flag, records whether the brick holds a fresh velocity, clears the flag, and the dirty
 // bricks are appended to the scene's work list (one atomic per wave).
