@@ -142,7 +142,7 @@ int32_t mpm_debug_sort(mpm_ctx* ctx);
 
 /* ---- options ------------------------------------------------------------------------------ */
 #define MPM_OPT_P2G_MODE 0       /* MPM_P2G_BINNED (default), MPM_P2G_ATOMIC or MPM_P2G_FIXED */
-#define MPM_OPT_SORT_INTERVAL 1  /* substeps between brick re-sorts (default 16; >= 1) */
+#define MPM_OPT_SORT_INTERVAL 1  /* substeps between brick re-sorts (default 24; >= 1) */
 #define MPM_OPT_P2G_BLOCKS 2     /* workgroups of the binned P2G launch (0: automatic) */
 #define MPM_OPT_PRECISION 3      /* SVD, return maps and stress: MPM_PRECISION_FP32, _FALLBACK (default:
                                     FP64 only where FP32 cannot resolve F: sigma_min/sigma_max < 1e-2 or

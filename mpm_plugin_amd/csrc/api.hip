@@ -182,7 +182,7 @@ struct mpm_ctx {
   int p2g_mode = 0;  // MPM_P2G_BINNED, MPM_P2G_ATOMIC or MPM_P2G_FIXED
   unsigned long long* fx = nullptr;  // fixed-point accumulator (MPM_P2G_FIXED only)
   double fx_scale = 0.0;
-  int64_t sort_interval = 16;
+  int64_t sort_interval = 24;
   int64_t p2g_blocks = 0;
   int p2g_waves = 2;             // waves per occupied brick in the binned P2G (1 or 2)
 
