@@ -269,23 +269,37 @@ __device__ void collider_sdf(const ColliderSet& cs, const ColliderDev& c, const 
       rp2 += (pl[a] - cc) * (pl[a] - cc);
     }
     const float rp = sqrtf(rp2);
-    // kRound candidates per round, their indices and then their corners loaded together (fewer dependent
-    // round trips to memory). The list is sorted by key, so the scan ends at the first round whose key,
-    // less rp and a rounding allowance, is farther than the best distance so far: every later candidate
-    // is at least that far.
+    // kRound candidates per round, their corners loaded together; the next round's indices and key load
+    // while this round's triangles are tested, so a round costs one round trip to memory. The list is
+    // sorted by key, so the scan ends at the first round whose key, less rp and a rounding allowance, is
+    // farther than the best distance so far: every later candidate is at least that far.
+    int t[kRound];
+#pragma unroll
+    for (int j = 0; j < kRound; ++j) t[j] = (int)cs.cell_tris[a0 + j < a1 ? a0 + j : a1 - 1];
+    float key = cs.cell_key[a0];
     for (uint32_t k = a0; k < a1; k += kRound) {
-      const float key = cs.cell_key[k];
       const float lb = key - rp - 1.0e-5f * (key + rp);
       if (lb > 0.0f && lb * lb > best * 1.0001f) break;
-      int t[kRound];
-#pragma unroll
-      for (int j = 0; j < kRound; ++j) t[j] = (int)cs.cell_tris[k + j < a1 ? k + j : a1 - 1];
       TriGeom tg[kRound];
 #pragma unroll
       for (int j = 0; j < kRound; ++j) tg[j] = cs.tris[c.tri_offset + t[j]];
+      const uint32_t kn = k + kRound;
+      int tn[kRound];
+      float key_n = 0.0f;
+      if (kn < a1) {
+#pragma unroll
+        for (int j = 0; j < kRound; ++j) tn[j] = (int)cs.cell_tris[kn + j < a1 ? kn + j : a1 - 1];
+        key_n = cs.cell_key[kn];
+      } else {
+#pragma unroll
+        for (int j = 0; j < kRound; ++j) tn[j] = t[j];
+      }
 #pragma unroll
       for (int j = 0; j < kRound; ++j)
         if (k + j < a1) scan_corners(tg[j].v, t[j], pl, best, bt, bf, q);
+#pragma unroll
+      for (int j = 0; j < kRound; ++j) t[j] = tn[j];
+      key = key_n;
     }
   } else {
     for (int t = 0; t < c.n_tris; ++t) scan_triangle(cs, c, t, pl, best, bt, bf, q);
