@@ -661,7 +661,7 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_spee
   if (fix) {
     // a persistent grid: the deferred particles are usually few
     Timed t(c, MPM_K_OTHER);
-    k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 1024), kBlock, 0, c->stream>>>(
+    k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 256), kBlock, 0, c->stream>>>(
         c->parts(), c->d_mats, c->fix_list, c->n_fix, block_max(c), (int)grid_for(c->n),
         fused ? c->h_vmax_dev : nullptr);
     check_launch("k_fp64_fixup");

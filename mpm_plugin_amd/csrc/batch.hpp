@@ -310,29 +310,48 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__
   return (uint32_t)lo;
 }
 
-// offsets[c] for c in [0, n_cells] (first sorted position with key >= c); the thread of a brick's first
-// cell also appends the brick to the occupied list when its 64 cells hold a particle
+// offsets[c] for c in [0, n_cells] (first sorted position with key >= c), one wave per brick (a 256-cell
+// block = four bricks): two lanes find the brick's first and past-the-end positions over all keys, the
+// other cells are searched within that range only (nothing to search in an empty brick), and lane 0
+// appends the brick to the occupied list when it holds a particle
 __global__ void __launch_bounds__(256) k_batch_cell_bounds(const SceneDev* __restrict__ sd,
                                                            const uint32_t* __restrict__ pre, int n_scenes) {
   const uint32_t* P0 = pre + kPreCellBlocks * (n_scenes + 1);
   const uint32_t total = P0[n_scenes];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int i = 0;
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     i = uniform(scene_of(P0, n_scenes, u, i));
     const SceneDev& S = sd[i];
-    const int c = (int)(u - P0[i]) * 256 + threadIdx.x;
-    if (c > S.n_cells) continue;
+    const int b = (int)(u - P0[i]) * 4 + wave;
+    const int n_bricks = S.n_cells >> 6;
+    if (b > n_bricks) continue;
     const uint32_t* keys = sorted_keys(S);
-    const uint32_t lo = lower_bound_u32(keys, S.n, (uint32_t)c);
-    S.offsets[c] = lo;
-    if ((c & 63) == 0 && c < S.n_cells) {
-      const uint32_t hi = lower_bound_u32(keys, S.n, (uint32_t)(c + 64));
+    if (b == n_bricks) {  // the terminal offset
+      if (lane == 0) S.offsets[S.n_cells] = lower_bound_u32(keys, S.n, (uint32_t)S.n_cells);
+      continue;
+    }
+    const uint32_t c0 = 64u * (uint32_t)b;
+    const uint32_t edge = lane < 2 ? lower_bound_u32(keys, S.n, c0 + 64u * (uint32_t)lane) : 0u;
+    const uint32_t lo = (uint32_t)__shfl((int)edge, 0), hi = (uint32_t)__shfl((int)edge, 1);
+    uint32_t mine = lo;
+    if (lane > 0) {
+      uint32_t l = lo, h = hi;
+      while (l < h) {
+        const uint32_t mid = (l + h) >> 1;
+        if (keys[mid] < c0 + (uint32_t)lane) l = mid + 1;
+        else h = mid;
+      }
+      mine = l;
+    }
+    S.offsets[c0 + lane] = mine;
+    if (lane == 0) {
       int32_t sl = -1;
       if (hi > lo) {
         sl = (int32_t)atomicAdd(S.n_active, 1u);
-        S.active[sl] = (uint32_t)(c >> 6);
+        S.active[sl] = (uint32_t)b;
       }
-      S.slot[c >> 6] = sl;
+      S.slot[b] = sl;
     }
   }
 }
