@@ -769,12 +769,47 @@ void refresh_present(mpm_ctx* c) {
   for (int32_t id : m) c->present[id] = 1;
 }
 
-// drop particles more than one cell past an open side; keeps insertion order of the survivors
+int64_t remove_escaped_host(mpm_ctx* c);
+
+// drop particles more than one cell past an open side (solver.py:222-240); keeps insertion order of the
+// survivors. On the device: flags by insertion index, their exclusive scan (the survivors' new indices),
+// one compaction into the other particle buffer; only the survivor count comes back to the host.
 int64_t remove_escaped(mpm_ctx* c) {
   const mpm_domain_desc& d = c->dom;
   bool any_open = false;
   for (int a = 0; a < 3; ++a) any_open = any_open || !d.lo_on[a] || !d.hi_on[a];
   if (!any_open || c->n == 0) return 0;
+  const int64_t n = c->n;
+  if (n > (int64_t)kScanTile * kScanTile) return remove_escaped_host(c);
+  char* base = static_cast<char*>(c->scratch.get(3 * (size_t)n * sizeof(int32_t)));
+  int32_t* keep = reinterpret_cast<int32_t*>(base);
+  uint32_t* new_pid = reinterpret_cast<uint32_t*>(base + (size_t)n * 4);
+  int32_t* em = reinterpret_cast<int32_t*>(base + 2 * (size_t)n * 4);
+  k_escape_flags<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), (int)n,
+                                                        make_float3(d.remove_lo[0], d.remove_lo[1], d.remove_lo[2]),
+                                                        make_float3(d.remove_hi[0], d.remove_hi[1], d.remove_hi[2]),
+                                                        keep);
+  check_launch("k_escape_flags");
+  scan_u32(c->scan_sums, c->stream, reinterpret_cast<const uint32_t*>(keep), new_pid, n);
+  uint32_t tail[2];
+  HIP_CHECK(hipMemcpyAsync(&tail[0], new_pid + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(&tail[1], keep + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  const int64_t kept = (int64_t)tail[0] + (int64_t)tail[1];
+  const int64_t removed = n - kept;
+  if (removed == 0) return 0;
+  k_compact<<<grid_for(n), kBlock, 0, c->stream>>>(c->A.view(), c->B.view(), keep, new_pid, (int)n, c->emitter, em);
+  check_launch("k_compact");
+  if (kept) HIP_CHECK(hipMemcpyAsync(c->emitter, em, kept * 4, hipMemcpyDeviceToDevice, c->stream));
+  std::swap(c->A, c->B);
+  c->n = kept;
+  state_changed(c);
+  return removed;
+}
+
+// the same through a host copy of the state (for counts beyond the device scan's reach)
+int64_t remove_escaped_host(mpm_ctx* c) {
+  const mpm_domain_desc& d = c->dom;
   const int64_t n = c->n;
   int32_t* keep = static_cast<int32_t*>(c->scratch.get(n * sizeof(int32_t)));
   k_escape_flags<<<grid_for(n), kBlock, 0, c->stream>>>(c->parts(), (int)n,

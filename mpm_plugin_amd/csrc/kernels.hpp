@@ -473,4 +473,31 @@ __global__ void k_escape_flags(Particles P, int n, float3 lo, float3 hi, int32_t
   keep_by_pid[P.pid[p]] = out ? 0 : 1;
 }
 
+// the survivors of an escape pass, stable in insertion order: the particle with insertion index pid is kept
+// when keep_by_pid[pid] and goes to slot new_pid[pid] (the exclusive scan of the flags) of dst with that as
+// its new insertion index; its emitter id moves the same way
+__global__ void k_compact(Particles src, Particles dst, const int32_t* __restrict__ keep_by_pid,
+                          const uint32_t* __restrict__ new_pid, int n, const int32_t* __restrict__ emitter_in,
+                          int32_t* __restrict__ emitter_out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int pid = src.pid[p];
+  if (!keep_by_pid[pid]) return;
+  const int64_t q = new_pid[pid];
+  const int64_t cs = src.cap, cd = dst.cap;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    dst.x[a * cd + q] = src.x[a * cs + p];
+    dst.v[a * cd + q] = src.v[a * cs + p];
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) dst.F[e * cd + q] = src.F[e * cs + p];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) dst.rec[4 * q + e] = src.rec[4 * (int64_t)p + e];
+  dst.Jp[q] = src.Jp[p];
+  dst.mat[q] = src.mat[p];
+  dst.pid[q] = (int32_t)q;
+  emitter_out[q] = emitter_in[pid];
+}
+
 }  // namespace mpm

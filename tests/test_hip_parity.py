@@ -164,3 +164,46 @@ def test_sorted_substeps_match_oracle_over_many_steps():
     for k in ("x", "v", "F"):
         assert rel_err(a[k], b[k]) <= 1e-4, k
     assert np.array_equal(a["emitter"], b["emitter"]) and np.array_equal(a["mat"], b["mat"])
+
+
+def test_open_side_removal_matches_oracle():
+    """Particles more than one cell past an open side are dropped after the frame (solver.py:222-240), on the
+    device: the survivors keep their insertion order, with their state, material and emitter ids."""
+    rng = np.random.default_rng(7)
+    res = 64
+    dx = 1.0 / res
+    n = 20000
+    x = np.empty((n, 3), np.float32)
+    x[:, :2] = rng.uniform(0.3, 0.7, (n, 2))
+    below = rng.random(n) < 0.4
+    # 1.5 cells below the open floor (removed) or just above it (kept): far from the threshold at -1 cell
+    x[:, 2] = np.where(below, -1.5 * dx + rng.uniform(-0.2, 0.2, n) * dx, rng.uniform(0.2, 3.0, n) * dx)
+    state = {
+        "x": x,
+        "v": rng.normal(scale=0.05, size=(n, 3)).astype(np.float32),
+        "C": np.zeros((n, 3, 3), np.float32),
+        "F": np.tile(np.eye(3, dtype=np.float32), (n, 1, 1)),
+        "Jp": np.ones(n, np.float32),
+        "mat": rng.integers(0, len(MATERIALS), n).astype(np.int32),
+        "emitter": rng.integers(0, 5, n).astype(np.int32),
+    }
+    walls = {f"{a}{s}": True for a in "xyz" for s in "-+"}
+    walls["z-"] = False
+    hip, ora = pair(state, res=res, walls=walls, gravity=(0.0, 0.0, 0.0))
+    st = hip.advance(1.0e-4)
+    steps, _, removed = ora.advance(1.0e-4)
+    assert st.substeps == steps == 1
+    assert st.removed == removed == int(below.sum())
+    assert hip.particle_count == n - removed
+    a, b = hip.save_state(), ora.save_state()
+    np.testing.assert_array_equal(a["mat"], b["mat"])
+    np.testing.assert_array_equal(a["emitter"], b["emitter"])
+    np.testing.assert_array_equal(a["emitter"], state["emitter"][~below])
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert rel_err(a[k], b[k]) <= 1e-4, k
+    # the next frame steps the compacted state (re-sorted) like the oracle
+    hip.advance(1.0e-4)
+    ora.advance(1.0e-4)
+    a, b = hip.save_state(), ora.save_state()
+    for k in ("x", "v", "F"):
+        assert rel_err(a[k], b[k]) <= 1e-4, k
