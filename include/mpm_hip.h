@@ -120,6 +120,13 @@ int32_t mpm_substep(mpm_ctx* ctx, double dt, double s, double frame_dt);
 /* ---- queries ------------------------------------------------------------------------------ */
 /* replaces WarpSolver.is_occupied (solver.py:244-255; kernels.py:272-298) */
 int32_t mpm_is_occupied(mpm_ctx* ctx, int64_t m, const float* points, uint8_t* out);
+/* the lattice part of sample_mesh_volume (src/mpm_core/geometry.py:89-155; the caller adds the jitter):
+   points (i + 0.5) spacing inside the closed mesh (verts: n_verts x 3 float64, tris: n_tris x 3 indices)
+   by z-ray crossing parity, as n x 3 float64 rows in the reference's order (column i * ny + j, then z).
+   *count is set to n; the rows are written only when out is not null and capacity >= n (call with
+   capacity 0 to size the buffer). Same FP64 arithmetic as the reference, so the points are identical. */
+int32_t mpm_lattice_inside(mpm_ctx* ctx, const double* verts, int64_t n_verts, const int32_t* tris, int64_t n_tris,
+                           double spacing, int64_t capacity, double* out, int64_t* count);
 
 /* ---- phase hooks for parity tests (one reference kernel each) ------------------------------ */
 /* grid zero + p2g (kernels.py:92-140) */

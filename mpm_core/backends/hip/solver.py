@@ -14,7 +14,7 @@ from collections.abc import Sequence
 
 import numpy as np
 
-from mpm_core.geometry import decompose_transform
+from mpm_core.geometry import decompose_transform, mesh_is_closed
 from mpm_core.materials import MaterialParams
 from mpm_core.scene import Domain
 from mpm_core.solver import AdvanceStats, ColliderGeometry, ParticleData, Solver
@@ -256,6 +256,28 @@ class HipSolver(Solver):
         out = np.zeros(len(pts), np.uint8)
         N.check(self._lib.mpm_is_occupied(self._ctx, len(pts), N.fptr(pts), N.u8ptr(out)))
         return out.astype(bool)
+
+    def sample_mesh_volume(self, verts: np.ndarray, tris: np.ndarray, spacing: float, seed) -> np.ndarray:
+        """``mpm_core.geometry.sample_mesh_volume`` with the lattice test on the GPU (geometry.py:89-155).
+
+        The inside lattice points come from the library in the reference's order with its FP64 arithmetic;
+        the jitter is the reference's single ``default_rng(seed).uniform`` draw over them, so the result is
+        the same array. Not part of the Solver interface: the runner uses it when a backend offers it.
+        """
+        if not mesh_is_closed(tris):
+            raise ValueError("emitter mesh is not closed (every edge must be shared by exactly two triangles)")
+        v = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        count = C.c_int64()
+        N.check(self._lib.mpm_lattice_inside(self._ctx, N.dptr(v), len(v), N.iptr(t), len(t), float(spacing), 0,
+                                             None, C.byref(count)))
+        if count.value == 0:
+            return np.zeros((0, 3), np.float32)
+        pts = np.empty((count.value, 3), np.float64)
+        N.check(self._lib.mpm_lattice_inside(self._ctx, N.dptr(v), len(v), N.iptr(t), len(t), float(spacing),
+                                             count.value, N.dptr(pts), C.byref(count)))
+        pts += np.random.default_rng(seed).uniform(-0.25, 0.25, pts.shape) * spacing
+        return pts.astype(np.float32)
 
     # -- phase hooks and measurement (parity tests, benchmarks) ----------------------------------
     def debug_p2g(self, dt: float) -> tuple[np.ndarray, np.ndarray]:

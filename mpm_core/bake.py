@@ -65,14 +65,17 @@ def emission_seed(scene: SceneDescription, emitter_index: int, frame: int, s: Fr
 
 
 def emitter_points(scene: SceneDescription, emitter_index: int, frame: int, s: Fraction,
-                   progress: _Progress | None = None) -> np.ndarray:
-    """Jittered lattice points of one emission, clipped to the domain (bake.py:56-74)."""
+                   progress: _Progress | None = None, sampler=sample_mesh_volume) -> np.ndarray:
+    """Jittered lattice points of one emission, clipped to the domain (bake.py:56-74).
+
+    ``sampler`` has the signature and result of ``mpm_core.geometry.sample_mesh_volume`` (a backend may
+    offer its own, e.g. the HIP solver's, which computes the same points on the GPU)."""
     e: Emitter = scene.emitters[emitter_index]
     k = frame - scene.frame_start
     xf = interpolate_transform(e.transforms[k - 1], e.transforms[k], float(s)) if k > 0 else e.transforms[k]
     spacing = scene.domain.dx / round(scene.particles_per_cell ** (1.0 / 3.0))
-    pts = sample_mesh_volume(transform_points(xf, e.mesh.verts), e.mesh.tris, spacing,
-                             emission_seed(scene, emitter_index, frame, s))
+    pts = sampler(transform_points(xf, e.mesh.verts), e.mesh.tris, spacing,
+                  emission_seed(scene, emitter_index, frame, s))
     if len(pts) == 0:
         if progress is not None:
             progress.warn(f"{e.name}: no particles, the mesh is smaller than the particle spacing ({spacing:.4g} m)")
@@ -90,7 +93,8 @@ def _emit(scene: SceneDescription, solver: Solver, emitter_index: int, frame: in
           progress: _Progress) -> None:
     """Emit at time ``frame - 1 + s`` (at ``frame`` itself on the first frame)."""
     e = scene.emitters[emitter_index]
-    pts = emitter_points(scene, emitter_index, frame, s, progress)
+    pts = emitter_points(scene, emitter_index, frame, s, progress,
+                         getattr(solver, "sample_mesh_volume", sample_mesh_volume))
     if e.kind == "inflow" and len(pts) and solver.particle_count:
         pts = pts[~solver.is_occupied(pts)]
     if len(pts) == 0:

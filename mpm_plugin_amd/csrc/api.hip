@@ -25,6 +25,7 @@
 
 #include "mpm_hip.h"
 #include "batch.hpp"
+#include "emission.hpp"
 
 using namespace mpm;
 
@@ -205,6 +206,7 @@ struct mpm_ctx {
   int64_t block_max_count = 0;
   bool vmax_valid = false;
   Scratch scratch;
+  Scratch emit_scratch;  // lattice emission (mpm_lattice_inside): mesh, column tables, crossings, points
   Timing timing;
   double frame_dt = 1.0;
   bool capturing = false;  // inside a graph capture: no events, no allocations, no host waits
@@ -1710,6 +1712,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
                   (void*)c->d_cell_key, (void*)c->d_cell_far})
     dev_free(p);
   c->scratch.release();
+  c->emit_scratch.release();
   c->hist.release();
   c->scan_sums.release();
   for (auto& p : c->timing.pending) {
@@ -1893,6 +1896,107 @@ int32_t mpm_substep(mpm_ctx* c, double dt, double s, double frame_dt) {
     require_idle(c);
     c->frame_dt = frame_dt;
     substep(c, dt, s, frame_dt);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+// Lattice points inside a closed mesh (geometry.py:89-155 before the jitter), in the reference's order.
+int32_t mpm_lattice_inside(mpm_ctx* c, const double* verts, int64_t n_verts, const int32_t* tris, int64_t n_tris,
+                           double spacing, int64_t capacity, double* out, int64_t* count) {
+  if (!c || !count) return MPM_ERR_VALUE;
+  return run(c, [&] {
+    *count = 0;
+    if (!(spacing > 0.0) || !std::isfinite(spacing)) fail(MPM_ERR_VALUE, "spacing must be positive and finite");
+    if (n_verts < 0 || n_tris < 0 || capacity < 0) fail(MPM_ERR_VALUE, "negative size");
+    if (n_verts == 0 || n_tris == 0) return;
+    if (!verts || !tris) fail(MPM_ERR_VALUE, "null mesh array");
+    if (n_tris > (int64_t)kScanTile * kScanTile || n_verts > (int64_t)INT32_MAX)
+      fail(MPM_ERR_VALUE, "mesh too large for the emission kernels");
+    for (int64_t i = 0; i < 3 * n_tris; ++i)
+      if (tris[i] < 0 || tris[i] >= n_verts) fail(MPM_ERR_VALUE, "triangle index out of range");
+    // the lattice as numpy computes it: first = ceil(min / spacing - 0.5), last = floor(max / spacing - 0.5)
+    EmitLattice L{};
+    for (int a = 0; a < 3; ++a) {
+      double lo = verts[a], hi = verts[a];
+      for (int64_t v = 1; v < n_verts; ++v) {
+        lo = std::min(lo, verts[3 * v + a]);
+        hi = std::max(hi, verts[3 * v + a]);
+      }
+      L.first[a] = (int64_t)std::ceil(lo / spacing - 0.5);
+      const int64_t last = (int64_t)std::floor(hi / spacing - 0.5);
+      if (last < L.first[a]) return;
+      if (last - L.first[a] + 1 > (int64_t)INT32_MAX) fail(MPM_ERR_VALUE, "lattice too large");
+      L.n[a] = (int)(last - L.first[a] + 1);
+    }
+    L.spacing = spacing;
+    L.nudge[0] = 1.234567e-4 * spacing;  // the reference's ray nudge (geometry.py:106)
+    L.nudge[1] = 2.345678e-4 * spacing;
+    const int64_t n_cols = (int64_t)L.n[0] * L.n[1];
+    if (n_cols > (int64_t)kScanTile * kScanTile) fail(MPM_ERR_VALUE, "lattice too large for the emission kernels");
+    // device tables, carved from one buffer (8-byte aligned pieces)
+    const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_verts = al((size_t)n_verts * 24), b_tris = al((size_t)n_tris * 12), b_tri = al((size_t)n_tris * 4),
+                 b_col = al((size_t)n_cols * 4);
+    char* base = static_cast<char*>(c->emit_scratch.get(b_verts + b_tris + 2 * b_tri + 5 * b_col));
+    double* d_verts = reinterpret_cast<double*>(base);
+    int32_t* d_tris = reinterpret_cast<int32_t*>(base + b_verts);
+    uint32_t* pairs = reinterpret_cast<uint32_t*>(base + b_verts + b_tris);
+    uint32_t* pair_off = reinterpret_cast<uint32_t*>(base + b_verts + b_tris + b_tri);
+    uint32_t* col_count = reinterpret_cast<uint32_t*>(base + b_verts + b_tris + 2 * b_tri);
+    uint32_t* col_off = col_count + b_col / 4;
+    uint32_t* col_fill = col_off + b_col / 4;
+    uint32_t* pts_count = col_fill + b_col / 4;
+    uint32_t* pts_off = pts_count + b_col / 4;
+    HIP_CHECK(hipMemcpyAsync(d_verts, verts, (size_t)n_verts * 24, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(d_tris, tris, (size_t)n_tris * 12, hipMemcpyHostToDevice, c->stream));
+    k_emit_tri_columns<<<grid_for(n_tris), kBlock, 0, c->stream>>>(L, d_verts, d_tris, (int)n_tris, pairs);
+    check_launch("k_emit_tri_columns");
+    // (triangle, column) pairs: their total must fit the 32-bit scan
+    std::vector<uint32_t> h_pairs(n_tris);
+    HIP_CHECK(hipMemcpyAsync(h_pairs.data(), pairs, (size_t)n_tris * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    uint64_t n_pairs = 0;
+    for (uint32_t p : h_pairs) n_pairs += p;
+    if (n_pairs == 0) return;
+    if (n_pairs >= (1ull << 32)) fail(MPM_ERR_VALUE, "mesh covers too many lattice columns");
+    scan_u32(c->scan_sums, c->stream, pairs, pair_off, n_tris);
+    HIP_CHECK(hipMemsetAsync(col_count, 0, (size_t)n_cols * 4, c->stream));
+    const unsigned pair_blocks = (unsigned)((n_pairs + kBlock - 1) / kBlock);
+    k_emit_pair_hits<<<pair_blocks, kBlock, 0, c->stream>>>(L, d_verts, d_tris, (int)n_tris, pair_off,
+                                                            (uint32_t)n_pairs, 0, col_count, nullptr, nullptr, nullptr);
+    check_launch("k_emit_pair_hits");
+    scan_u32(c->scan_sums, c->stream, col_count, col_off, n_cols);
+    uint32_t tail[2];
+    HIP_CHECK(hipMemcpyAsync(&tail[0], col_off + n_cols - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(&tail[1], col_count + n_cols - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const uint64_t n_hits = (uint64_t)tail[0] + tail[1];
+    if (n_hits == 0) return;
+    double* hit_z = dev_alloc<double>(n_hits);
+    struct Free {
+      void* p;
+      ~Free() { dev_free(p); }
+    } free_hits{hit_z};
+    HIP_CHECK(hipMemsetAsync(col_fill, 0, (size_t)n_cols * 4, c->stream));
+    k_emit_pair_hits<<<pair_blocks, kBlock, 0, c->stream>>>(L, d_verts, d_tris, (int)n_tris, pair_off,
+                                                            (uint32_t)n_pairs, 1, nullptr, col_off, col_fill, hit_z);
+    check_launch("k_emit_pair_hits");
+    k_emit_column_points<<<grid_for(n_cols), kBlock, 0, c->stream>>>(L, n_cols, col_count, col_off, hit_z, 0,
+                                                                     pts_count, nullptr, nullptr);
+    check_launch("k_emit_column_points");
+    scan_u32(c->scan_sums, c->stream, pts_count, pts_off, n_cols);
+    HIP_CHECK(hipMemcpyAsync(&tail[0], pts_off + n_cols - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(&tail[1], pts_count + n_cols - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const int64_t n_pts = (int64_t)tail[0] + tail[1];
+    *count = n_pts;
+    if (n_pts == 0 || n_pts > capacity || !out) return;
+    double* d_out = dev_alloc<double>(3 * (size_t)n_pts);
+    Free free_out{d_out};
+    k_emit_column_points<<<grid_for(n_cols), kBlock, 0, c->stream>>>(L, n_cols, col_count, col_off, hit_z, 1,
+                                                                     nullptr, pts_off, d_out);
+    check_launch("k_emit_column_points");
+    HIP_CHECK(hipMemcpyAsync(out, d_out, 3 * (size_t)n_pts * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
