@@ -1914,6 +1914,8 @@ int32_t mpm_lattice_inside(mpm_ctx* c, const double* verts, int64_t n_verts, con
       fail(MPM_ERR_VALUE, "mesh too large for the emission kernels");
     for (int64_t i = 0; i < 3 * n_tris; ++i)
       if (tris[i] < 0 || tris[i] >= n_verts) fail(MPM_ERR_VALUE, "triangle index out of range");
+    for (int64_t i = 0; i < 3 * n_verts; ++i)
+      if (!std::isfinite(verts[i])) fail(MPM_ERR_VALUE, "non-finite vertex coordinate");
     // the lattice as numpy computes it: first = ceil(min / spacing - 0.5), last = floor(max / spacing - 0.5)
     EmitLattice L{};
     for (int a = 0; a < 3; ++a) {

@@ -59,3 +59,13 @@ def test_open_mesh_is_rejected(solver):
     verts, tris = unit_cube_mesh()
     with pytest.raises(ValueError):
         solver.sample_mesh_volume(verts, tris[:-1], 0.05, 0)
+
+
+def test_bad_meshes_are_errors(solver):
+    verts, tris = icosphere(0.1, 2)
+    bad = verts.copy()
+    bad[3, 1] = np.nan
+    with pytest.raises(ValueError):
+        solver.sample_mesh_volume(bad + 0.5, tris, 0.01, 0)
+    with pytest.raises(ValueError):
+        solver.sample_mesh_volume(verts + 0.5, tris, 0.0, 0)
