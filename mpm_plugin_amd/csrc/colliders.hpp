@@ -70,7 +70,7 @@ struct ColliderFrame {
   float A[9], t[3];        // M(s)
   float Ai[9], ti[3];      // M(s)^-1
   float dA[9], dt[3];      // velocity matrix
-  float smin;              // 1 / |M(s)^-1|_F: at most the smallest singular value of A, so |A u| >= smin |u|
+  float smin;              // at most the smallest singular value of A, so |A u| >= smin |u|
 };
 
 // every collider's frame of one substep and its record, computed once on the host and passed by value to
@@ -142,10 +142,23 @@ __host__ __device__ inline void collider_frame(const ColliderDev& c, float s, fl
   f.Ai[6] = c02 * inv; f.Ai[7] = (a[1] * a[6] - a[0] * a[7]) * inv; f.Ai[8] = (a[0] * a[4] - a[1] * a[3]) * inv;
 #pragma unroll
   for (int i = 0; i < 3; ++i) f.ti[i] = -(f.Ai[3 * i] * f.t[0] + f.Ai[3 * i + 1] * f.t[1] + f.Ai[3 * i + 2] * f.t[2]);
+  // A lower bound of the smallest singular value of A = R S (rotation times the interpolated stretch):
+  // 1 / |A^-1|_F always; and, since |S u| >= u^T S u >= lambda_min((S + S^T) / 2) for a unit u, the
+  // Gershgorin bound of the stretch's symmetric part, exact for uniform and axis-aligned scales. Rounded
+  // down a little (R's and A's rounding): a lower bound is all the far-cell test may use.
   float fro = 0.0f;
   for (int i = 0; i < 9; ++i) fro += f.Ai[i] * f.Ai[i];
-  // rounded down a little: a lower bound is all the far-cell test may use
-  f.smin = fro > 0.0f ? 0.9999f / sqrtf(fro) : 0.0f;
+  float S[9];
+  for (int i = 0; i < 9; ++i) S[i] = c.S0[i] + s * (c.S1[i] - c.S0[i]);
+  float gersh = 3.0e38f;
+  for (int i = 0; i < 3; ++i) {
+    float r = 0.0f;
+    for (int j = 0; j < 3; ++j)
+      if (j != i) r += fabsf(0.5f * (S[3 * i + j] + S[3 * j + i]));
+    gersh = fminf(gersh, S[4 * i] - r);
+  }
+  const float from_inverse = fro > 0.0f ? 1.0f / sqrtf(fro) : 0.0f;
+  f.smin = 0.9999f * fmaxf(from_inverse, gersh);
   const float h = 1.0e-3f;
   float Ap[9], tp[3], Am[9], tm[3];
   collider_matrix(c, s + h, Ap, tp);
