@@ -352,7 +352,7 @@ def main(argv=None) -> int:
         ms, launches = timed[kernel]
         mean_s = ms / 1000.0 / max(launches, 1)
         achieved = bytes_per_launch[kernel] / mean_s / 1e9
-        lib_name = {"p2g": "k_p2g_cells", "g2p": "k_g2p"}[kernel]
+        lib_name = {"p2g": "k_p2g_split", "g2p": "k_g2p"}[kernel]
         out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc.get(("traffic", lib_name)),
                "kernel": kernel, "mean_launch_us": round(mean_s * 1e6, 2),

@@ -212,7 +212,7 @@ int32_t mpm_batch_run(mpm_batch* batch, int64_t max_launches, int64_t* launched,
 int32_t mpm_batch_progress(const mpm_batch* batch, int32_t* substeps_done, int32_t* substeps_total);
 #define MPM_BATCH_OPT_P2G_BLOCKS 0        /* workgroups of the persistent P2G launch (0: automatic) */
 #define MPM_BATCH_OPT_GRAPH 1             /* 1: replay captured graphs (default); 0: launch directly */
-#define MPM_BATCH_OPT_SORT_INTERVAL 2     /* substeps between re-sorts for every scene (0: each scene's own) */
+#define MPM_BATCH_OPT_SORT_INTERVAL 2     /* substeps between re-sorts for every scene (default 16; 0: each scene's own) */
 #define MPM_BATCH_OPT_SUBSTEPS_LAUNCHED 3 /* read only: substep launches since creation */
 #define MPM_BATCH_OPT_G2P_BLOCKS 4        /* workgroups of the persistent G2P launch (0: automatic) */
 #define MPM_BATCH_OPT_GRID_BLOCKS 5       /* workgroups of the persistent grid-update launch (0: automatic) */

@@ -32,7 +32,7 @@ BATCH_SPACE: dict[int, list[int]] = {
     N.BATCH_OPT_G2P_BLOCKS: [2048, 1024, 4096, 8192],
     N.BATCH_OPT_P2G_BLOCKS: [8192, 4096, 16384],
     N.BATCH_OPT_GRID_BLOCKS: [2048, 512, 1024, 4096],
-    N.BATCH_OPT_SORT_INTERVAL: [24, 16, 32],
+    N.BATCH_OPT_SORT_INTERVAL: [16, 24, 8],
 }
 KNOB_NAMES = {N.BATCH_OPT_G2P_TILE: "g2p_tile", N.BATCH_OPT_G2P_BLOCKS: "g2p_blocks",
               N.BATCH_OPT_P2G_BLOCKS: "p2g_blocks", N.BATCH_OPT_GRID_BLOCKS: "grid_blocks",

@@ -277,7 +277,7 @@ struct mpm_batch {
   int64_t p2g_waves = 1;  // waves per occupied brick in P2G (the mix's many bricks fill the machine)
   int64_t use_graph = 1;
   bool fp64_pass = false;  // some scene defers particles to the FP64 pass (part of the captured sequence)
-  int64_t sort_interval = 0;  // 0: each context's own
+  int64_t sort_interval = 16;  // 0: each context's own (the mix's fast scenes drift more per substep)
   int64_t substeps_launched = 0;
 };
 
