@@ -26,6 +26,12 @@ once continuously batched and once as serial plain runs (one bake at a time, Sol
 same arrival times. Frames are downloaded to the host every frame as a bake streams them (no files are
 written in the measurement). Reported: per-frame latency p50/p99 and aggregate particle-substeps/s.
 
+A fourth measurement (skipped with --no-surface) meshes the final frame of the timed run with the
+surfacer (mpm_surface.field.surface_particles, default settings: field cell = particle spacing dx/2,
+kernel radius 4 cells, iso 0.35), one mesh per emitter as the surface runner does: wall time per frame
+with host arrays in and host meshes out, and the device time of the upload, splat, normalisation and
+marching cubes.
+
 The printed JSON line carries:
   roofline      for the kernel with the largest device time: algorithmic bytes per launch (bytes per
                 particle x particles + bytes per active grid node x active nodes) over its mean launch
@@ -101,6 +107,7 @@ def parse_args(argv=None):
     ap.add_argument("--batch-frames", type=int, default=2, help="timed frames of the scene mix")
     ap.add_argument("--no-tune", action="store_true", help="batched scene mix with default launch shapes")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-requests measurement")
+    ap.add_argument("--no-surface", action="store_true", help="skip the surfacing measurement")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second")
     return ap.parse_args(argv)
@@ -266,6 +273,42 @@ def serve_bench(n_requests: int, rate: float) -> dict:
     return out
 
 
+def surface_bench(particles, spacing: float, reps: int = 5) -> dict:
+    """The surfacer (mpm_surface.field.surface_particles, default settings) on every emitter of a frame of
+    particles, as the surface runner meshes one frame: host arrays in, host meshes out. Reported: wall time
+    per frame (best of ``reps``), and the device time of its phases from the surfacer's own HIP events."""
+    from mpm_surface.field import field_grid, surface_particles, surfacer
+    from mpm_surface.settings import SurfaceSettings
+
+    settings = SurfaceSettings()
+    sets = [(particles.positions[particles.emitter_ids == e], particles.velocities[particles.emitter_ids == e])
+            for e in np.unique(particles.emitter_ids)]
+    for x, v in sets:  # warm-up: buffers grown to the largest emitter
+        surface_particles(x, v, spacing, settings, "cuda:0")
+    walls, device = [], None
+    for _ in range(reps):
+        phases = dict.fromkeys(("upload", "splat", "normalize", "marching_cubes"), 0.0)
+        verts = tris = 0
+        t0 = time.perf_counter()
+        for x, v in sets:
+            m = surface_particles(x, v, spacing, settings, "cuda:0")
+            verts, tris = verts + len(m.verts), tris + len(m.tris)
+            for k, ms in surfacer("cuda:0").last_times.items():
+                phases[k] += ms
+        walls.append(time.perf_counter() - t0)
+        if device is None or sum(phases.values()) < sum(device.values()):
+            device = phases
+    cells = sum(int(np.prod(field_grid(x, spacing, settings)[1])) for x, _ in sets)
+    wall = min(walls)
+    out = {"particles": int(len(particles.positions)), "emitters": len(sets), "field_nodes": cells,
+           "verts": verts, "tris": tris, "settings": settings.to_dict(), "spacing": spacing,
+           "ms_per_frame": round(wall * 1000, 2), "particles_per_s": len(particles.positions) / wall,
+           "device_ms": {k: round(v, 3) for k, v in device.items()}}
+    log(f"surfacing: {out['particles']} particles in {len(sets)} emitters -> {verts} verts, {tris} tris; "
+        f"{out['ms_per_frame']} ms per frame (device: {out['device_ms']})")
+    return out
+
+
 def main(argv=None) -> int:
     args = parse_args(argv)
     if args.gpus != 1:
@@ -340,6 +383,7 @@ def main(argv=None) -> int:
         step()
     solver.set_timing(False)
     timed["p2g"] = solver.kernel_times()["p2g"]
+    final = solver.get_particles() if not args.no_surface else None
     log(f"timed {args.steps} steps in {elapsed:.3f} s; g2p {timed['g2p'][0]:.2f} ms / {timed['g2p'][1]} launches "
         f"(p2g {timed['p2g'][0]:.2f} ms / {timed['p2g'][1]} launches over the same substeps of a second run)")
     bytes_per_launch = {
@@ -379,6 +423,9 @@ def main(argv=None) -> int:
     solver.close()
     if not args.no_batch:
         batched = batch_bench(args.batch_scenes, args.batch_frames, tune=not args.no_tune)
+    surfaced = None
+    if final is not None:
+        surfaced = surface_bench(final, w.domain.dx / 2)
     served = None
     if not args.no_serve:
         served = serve_bench(args.serve_requests, args.serve_rate)
@@ -406,6 +453,7 @@ def main(argv=None) -> int:
         "kernels": kernels,
         "batched": batched,
         "served": served,
+        "surface": surfaced,
     }
     print(json.dumps(line), flush=True)
     return 0

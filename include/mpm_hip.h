@@ -221,6 +221,47 @@ int32_t mpm_batch_progress(const mpm_batch* batch, int32_t* substeps_done, int32
 int32_t mpm_batch_set_option(mpm_batch* batch, int32_t option, int64_t value);
 int64_t mpm_batch_get_option(const mpm_batch* batch, int32_t option);
 
+/* ---- surfacing -----------------------------------------------------------------------------
+ * Replaces the device work of surface_particles (src/mpm_surface/field.py:100-142): the density splat
+ * (field.py:35-55), its normalisation (:58-63), marching cubes (Warp's MarchingCubes, :130-131) and the
+ * vertex velocity sample (:66-85). The host part (grid placement, rest density, memory budget, world
+ * coordinates) stays in mpm_surface.field. A surfacer keeps its device buffers between calls, so meshing
+ * frame after frame reallocates only when a frame needs more. */
+typedef struct mpm_surface mpm_surface;
+typedef struct {
+  int32_t dims[3];  /* field nodes per axis (>= 2 each), node (i, j, k) at origin + (i, j, k) cell */
+  float origin[3];
+  float cell;       /* node spacing */
+  float h;          /* kernel support radius */
+  float inv_rest;   /* 1 / rest density: the normalised density of a uniform particle lattice reads 1 */
+  float iso;        /* the surface's level of the normalised density */
+} mpm_surface_desc;
+int32_t mpm_surface_create(int32_t device, mpm_surface** out);
+int32_t mpm_surface_destroy(mpm_surface* surf);
+/* splat n particles (positions x, velocities v: (n,3) float32 host rows), normalise, and extract the
+ * desc->iso surface of the density; the mesh stays on the device until the next build. n_verts and n_tris
+ * receive its sizes (0 and 0 when nothing crosses iso). */
+int32_t mpm_surface_build(mpm_surface* surf, const mpm_surface_desc* desc, int64_t n, const float* x,
+                          const float* v, int64_t* n_verts, int64_t* n_tris);
+/* the last build's mesh: vertices (n_verts,3) float32 in field-grid units (world = origin + cell * g, as
+ * Warp's marching cubes returns them), triangles (n_tris,3) int32 wound with normals pointing out of the
+ * material (the reference flips Warp's inward winding, field.py:138), vertex velocities (n_verts,3)
+ * float32. A null output is skipped. */
+int32_t mpm_surface_fetch(mpm_surface* surf, float* verts, int32_t* tris, float* vel);
+/* the last build's field (no counterpart: for tests): normalised density (D0,D1,D2) float32 and velocity
+ * (D0,D1,D2,3) float32 (momentum / density where density > 0, else 0), C order; a null output is skipped */
+int32_t mpm_surface_get_field(mpm_surface* surf, float* density, float* velocity);
+/* device time of the last build in ms: [0] upload of the particles, [1] splat, [2] normalisation,
+ * [3] marching cubes with the vertex velocities (both passes, their scans and the size read-back) */
+int32_t mpm_surface_times(const mpm_surface* surf, float* ms4);
+/* the marching cubes tables (host only, no device call): count[256] triangles per corner configuration
+ * (bit c set: corner c = (c & 1, c >> 1 & 1, c >> 2 & 1) above iso) and edges[256 * 36], three cell edges
+ * per triangle; edge e joins corners MC_EDGE_ENDS[e] (x edges 0-3 from corners 0, 2, 4, 6; y edges 4-7
+ * from corners 0, 1, 4, 5; z edges 8-11 from corners 0, 1, 2, 3) */
+int32_t mpm_surface_tables(int32_t* count, int8_t* edges);
+/* free and total device memory in bytes (the reference's default surfacing budget is 70% of free) */
+int32_t mpm_device_memory(int32_t device, int64_t* free_bytes, int64_t* total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,11 @@ class AdvanceStats(C.Structure):
     _fields_ = [("substeps", C.c_int64), ("dt", C.c_double), ("removed", C.c_int64)]
 
 
+class SurfaceDesc(C.Structure):
+    _fields_ = [("dims", C.c_int32 * 3), ("origin", C.c_float * 3), ("cell", C.c_float), ("h", C.c_float),
+                ("inv_rest", C.c_float), ("iso", C.c_float)]
+
+
 class NativeError(RuntimeError):
     pass
 
@@ -97,6 +102,14 @@ def _declare(L: C.CDLL) -> None:
         "mpm_batch_progress": ([ctx, ip, ip], i32),
         "mpm_batch_set_option": ([ctx, i32, i64], i32),
         "mpm_batch_get_option": ([ctx, i32], i64),
+        "mpm_surface_create": ([i32, C.POINTER(ctx)], i32),
+        "mpm_surface_destroy": ([ctx], i32),
+        "mpm_surface_build": ([ctx, C.POINTER(SurfaceDesc), i64, fp, fp, i64p, i64p], i32),
+        "mpm_surface_fetch": ([ctx, fp, ip, fp], i32),
+        "mpm_surface_get_field": ([ctx, fp, fp], i32),
+        "mpm_surface_times": ([ctx, fp], i32),
+        "mpm_surface_tables": ([ip, C.POINTER(C.c_int8)], i32),
+        "mpm_device_memory": ([i32, i64p, i64p], i32),
     }
     for name, (args, res) in table.items():
         fn = getattr(L, name)
