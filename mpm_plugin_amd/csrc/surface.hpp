@@ -156,11 +156,19 @@ __device__ __forceinline__ float surf_cubic(float q) {
   return 0.0f;
 }
 
+// surf_cubic in straight-line form: both pieces computed, the outer one clamped to zero past q = 1
+__device__ __forceinline__ float surf_cubic_select(float q) {
+  const float t = fmaxf(1.0f - q, 0.0f);
+  const float inner = 1.0f - 6.0f * q * q + 6.0f * q * q * q;
+  const float outer = 2.0f * t * t * t;
+  return q < 0.5f ? inner : outer;
+}
+
 // The splat as a gather: particles are counting-sorted by the field cell that holds them (k_surf_cell_keys,
 // a scan, k_surf_sort_particles), then one wave per 4x4x4 brick of nodes (k_surf_gather) walks the cells
 // within the kernel radius of the brick, row by row (a row of cells is one contiguous run of sorted
-// particles), loads 64 particles at a time and broadcasts them lane by lane; each lane sums the weights of
-// its own node in registers and stores density and momentum once. No atomics on the field, no clearing of
+// particles), reading each particle once per wave (the index is wave-uniform: scalar loads); each lane sums
+// the weights of its own node in registers and stores density and momentum once. No atomics on the field, no clearing of
 // it, and a node's sum is independent of how many particles crowd it (a scatter with atomics slows down
 // where the material is compressed).
 
@@ -198,17 +206,16 @@ __global__ void __launch_bounds__(256) k_surf_sort_particles(SurfGrid G, const f
   vs[q] = make_float4(v[3 * p], v[3 * p + 1], v[3 * p + 2], 0.0f);
 }
 
-__device__ __forceinline__ float lane_value(float x, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
-}
-
 // wave b: nodes (4 bi + lane / 16, 4 bj + lane / 4 % 4, 4 bk + lane % 4) of brick b; reach = ceil(h / cell)
 __global__ void __launch_bounds__(256) k_surf_gather(SurfGrid G, int reach, const float4* __restrict__ gs,
                                                      const float4* __restrict__ vs, const uint32_t* __restrict__ start,
                                                      int nb1, int nb2, int64_t n_bricks, float* __restrict__ density,
                                                      float* __restrict__ momentum) {
   const int lane = threadIdx.x & 63;
-  const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // the wave's index, read from the first lane so that the compiler knows it (and every particle index
+  // derived from it) to be uniform: the particles then come through scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
   if (b >= n_bricks) return;  // whole waves
   const int bk = (int)(b % nb2) * 4, bj = (int)((b / nb2) % nb1) * 4, bi = (int)(b / ((int64_t)nb2 * nb1)) * 4;
   const int i = bi + (lane >> 4), j = bj + ((lane >> 2) & 3), k = bk + (lane & 3);
@@ -221,21 +228,21 @@ __global__ void __launch_bounds__(256) k_surf_gather(SurfGrid G, int reach, cons
   for (int ci = i0; ci <= i1; ++ci)
     for (int cj = j0; cj <= j1; ++cj) {
       const uint32_t first = start[surf_node(G, ci, cj, k0)], last = start[surf_node(G, ci, cj, k1) + 1];
-      for (uint32_t base = first; base < last; base += 64) {
-        float4 pg = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pv = pg;
-        if (base + lane < last) {
-          pg = gs[base + lane];
-          pv = vs[base + lane];
-        }
-        const int m = (int)min(64u, last - base);
-        for (int t = 0; t < m; ++t) {
-          const float dx = fi - lane_value(pg.x, t), dy = fj - lane_value(pg.y, t), dz = fk - lane_value(pg.z, t);
-          const float w = surf_cubic(sqrtf(dx * dx + dy * dy + dz * dz) * q_per_cell);
-          sd += w;
-          sx += w * lane_value(pv.x, t);
-          sy += w * lane_value(pv.y, t);
-          sz += w * lane_value(pv.z, t);
-        }
+      if (first >= last) continue;
+      // wave-uniform indices: scalar loads, the next particle's issued before the current one is used
+      float4 pg = gs[first], pv = vs[first];
+      for (uint32_t q = first; q < last; ++q) {
+        const uint32_t nq = min(q + 1, last - 1);
+        const float4 ng = gs[nq], nv = vs[nq];
+        const float dx = fi - pg.x, dy = fj - pg.y, dz = fk - pg.z;
+        // the hardware square root (1 ulp) and a select form of the cubic: no branches in the loop
+        const float w = surf_cubic_select(__builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) * q_per_cell);
+        sd += w;
+        sx += w * pv.x;
+        sy += w * pv.y;
+        sz += w * pv.z;
+        pg = ng;
+        pv = nv;
       }
     }
   if (i < G.dims[0] && j < G.dims[1] && k < G.dims[2]) {
