@@ -223,10 +223,17 @@ __global__ void __launch_bounds__(256) k_surf_gather(SurfGrid G, int reach, cons
   const float q_per_cell = G.cell / G.h;
   const int i0 = max(bi - reach, 0), i1 = min(bi + 3 + reach, G.dims[0] - 1);
   const int j0 = max(bj - reach, 0), j1 = min(bj + 3 + reach, G.dims[1] - 1);
-  const int k0 = max(bk - reach, 0), k1 = min(bk + 3 + reach, G.dims[2] - 1);
+  const float r = G.h * G.inv_cell, r2 = r * r;
   float sd = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
   for (int ci = i0; ci <= i1; ++ci)
     for (int cj = j0; cj <= j1; ++cj) {
+      // a particle of cell (ci, cj, ck) lies at least di, dj (and dk) cells from the brick's nodes along each
+      // axis: rows out of reach are skipped, and each row keeps the cells within the radius left for dk
+      const int di = max(0, max(bi - ci - 1, ci - bi - 3)), dj = max(0, max(bj - cj - 1, cj - bj - 3));
+      const float lat2 = (float)(di * di + dj * dj);
+      if (lat2 >= r2) continue;
+      const int kc = (int)ceilf(sqrtf(r2 - lat2));
+      const int k0 = max(bk - kc, 0), k1 = min(bk + 3 + kc, G.dims[2] - 1);
       const uint32_t first = start[surf_node(G, ci, cj, k0)], last = start[surf_node(G, ci, cj, k1) + 1];
       if (first >= last) continue;
       // wave-uniform indices: scalar loads, the next particle's issued before the current one is used
