@@ -243,6 +243,13 @@ int32_t mpm_surface_destroy(mpm_surface* surf);
  * receive its sizes (0 and 0 when nothing crosses iso). */
 int32_t mpm_surface_build(mpm_surface* surf, const mpm_surface_desc* desc, int64_t n, const float* x,
                           const float* v, int64_t* n_verts, int64_t* n_tris);
+/* the particles' upload on its own, for a caller that places the field from their bounds (as
+ * surface_particles does, field.py:112-118): copies x and v (n > 0) to the device and returns the minimum
+ * and maximum of each coordinate, computed there (lo[3], hi[3]; all NaN when any coordinate is NaN, as
+ * numpy's min and max give for the column). mpm_surface_build_uploaded then meshes them. */
+int32_t mpm_surface_upload(mpm_surface* surf, int64_t n, const float* x, const float* v, float* lo, float* hi);
+int32_t mpm_surface_build_uploaded(mpm_surface* surf, const mpm_surface_desc* desc, int64_t* n_verts,
+                                   int64_t* n_tris);
 /* the last build's mesh: vertices (n_verts,3) float32 in field-grid units (world = origin + cell * g, as
  * Warp's marching cubes returns them), triangles (n_tris,3) int32 wound with normals pointing out of the
  * material (the reference flips Warp's inward winding, field.py:138), vertex velocities (n_verts,3)
@@ -251,8 +258,9 @@ int32_t mpm_surface_fetch(mpm_surface* surf, float* verts, int32_t* tris, float*
 /* the last build's field (no counterpart: for tests): normalised density (D0,D1,D2) float32 and velocity
  * (D0,D1,D2,3) float32 (momentum / density where density > 0, else 0), C order; a null output is skipped */
 int32_t mpm_surface_get_field(mpm_surface* surf, float* density, float* velocity);
-/* device time of the last build in ms: [0] upload of the particles, [1] splat, [2] normalisation,
- * [3] marching cubes with the vertex velocities (both passes, their scans and the size read-back) */
+/* device time of the last build in ms: [0] upload of the particles (with the bounds, when they came by
+ * mpm_surface_upload), [1] splat (cell sort and gather), [2] normalisation, [3] marching cubes with the
+ * vertex velocities (both passes, their scans and the size read-back) */
 int32_t mpm_surface_times(const mpm_surface* surf, float* ms4);
 /* the marching cubes tables (host only, no device call): count[256] triangles per corner configuration
  * (bit c set: corner c = (c & 1, c >> 1 & 1, c >> 2 & 1) above iso) and edges[256 * 36], three cell edges

@@ -105,6 +105,8 @@ def _declare(L: C.CDLL) -> None:
         "mpm_surface_create": ([i32, C.POINTER(ctx)], i32),
         "mpm_surface_destroy": ([ctx], i32),
         "mpm_surface_build": ([ctx, C.POINTER(SurfaceDesc), i64, fp, fp, i64p, i64p], i32),
+        "mpm_surface_upload": ([ctx, i64, fp, fp, fp, fp], i32),
+        "mpm_surface_build_uploaded": ([ctx, C.POINTER(SurfaceDesc), i64p, i64p], i32),
         "mpm_surface_fetch": ([ctx, fp, ip, fp], i32),
         "mpm_surface_get_field": ([ctx, fp, fp], i32),
         "mpm_surface_times": ([ctx, fp], i32),

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 #include <vector>
@@ -2382,9 +2383,13 @@ struct mpm_surface {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {};
+  hipEvent_t ev_upload[2] = {};  // around mpm_surface_upload's copies and bounds
+  bool from_upload = false;      // the last build meshed particles left by mpm_surface_upload
   Scratch tables, parts, density, momentum, crossed, vid, ntri, tri_off, verts, vvel, tris, scan_sums;
+  Scratch bounds;
   SurfGrid grid{};
   int64_t n_nodes = 0, n_verts = 0, n_tris = 0;
+  int64_t uploaded = -1;  // particles on the device (mpm_surface_upload or a build), -1: none
   bool built = false;
 };
 
@@ -2392,6 +2397,7 @@ namespace {
 
 // nodes per field at most: the edge ids (3 per node) and the launch sizes stay within 32 bits
 constexpr int64_t kMaxSurfNodes = int64_t(1) << 30;
+constexpr int kSurfBoundsBlocks = 1024;  // partial bounds per upload, reduced on the host
 
 // exclusive scan of n 32-bit counts (out != in) of any length: chunks of what one scan_u32 pass takes,
 // each after the first carried by the total of the chunk before
@@ -2413,9 +2419,11 @@ void destroy_surface(mpm_surface* s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (Scratch* b : {&s->tables, &s->parts, &s->density, &s->momentum, &s->crossed, &s->vid, &s->ntri,
-                     &s->tri_off, &s->verts, &s->vvel, &s->tris, &s->scan_sums})
+                     &s->tri_off, &s->verts, &s->vvel, &s->tris, &s->scan_sums, &s->bounds})
     b->release();
   for (hipEvent_t e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->ev_upload)
     if (e) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   (void)hipSetDevice(prev);
@@ -2423,6 +2431,121 @@ void destroy_surface(mpm_surface* s) {
 }
 
 bool finite_positive(float x) { return std::isfinite(x) && x > 0.0f; }
+
+// the build: particles uploaded from x, v (upload) or left by mpm_surface_upload
+void surface_build(mpm_surface* s, const mpm_surface_desc* d, int64_t n, const float* x, const float* v, bool upload,
+                   int64_t* n_verts, int64_t* n_tris) {
+  *n_verts = *n_tris = 0;
+  s->built = false;
+  s->n_verts = s->n_tris = 0;
+  if (n < 0) fail(MPM_ERR_VALUE, "negative particle count");
+  if (n > INT32_MAX) fail(MPM_ERR_VALUE, "too many particles for one surfacing call");
+  if (upload && n > 0 && (!x || !v)) fail(MPM_ERR_VALUE, "null particle array");
+  int64_t nodes = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (d->dims[a] < 2) fail(MPM_ERR_VALUE, "the field needs at least 2 nodes per axis");
+    if (!std::isfinite(d->origin[a])) fail(MPM_ERR_VALUE, "non-finite field origin");
+    nodes *= d->dims[a];
+    if (nodes > kMaxSurfNodes) fail(MPM_ERR_VALUE, "field too large for one surfacing call");
+  }
+  if (!finite_positive(d->cell) || !finite_positive(d->h) || !finite_positive(d->inv_rest) || !std::isfinite(d->iso))
+    fail(MPM_ERR_VALUE, "cell, h and inv_rest must be positive and finite, iso finite");
+  SurfGrid G{};
+  for (int a = 0; a < 3; ++a) {
+    G.dims[a] = d->dims[a];
+    G.origin[a] = d->origin[a];
+  }
+  G.cell = d->cell;
+  G.inv_cell = 1.0f / d->cell;
+  G.h = d->h;
+  s->grid = G;
+  s->n_nodes = nodes;
+  const int64_t cells = (int64_t)(G.dims[0] - 1) * (G.dims[1] - 1) * (G.dims[2] - 1);
+  float* density = static_cast<float*>(s->density.get((size_t)nodes * 4));
+  float* momentum = static_cast<float*>(s->momentum.get((size_t)nodes * 12));
+  uint32_t* crossed = static_cast<uint32_t*>(s->crossed.get((size_t)nodes * 12));
+  uint32_t* vid = static_cast<uint32_t*>(s->vid.get((size_t)nodes * 12));
+  uint32_t* ntri = static_cast<uint32_t*>(s->ntri.get((size_t)cells * 4));
+  uint32_t* tri_off = static_cast<uint32_t*>(s->tri_off.get((size_t)cells * 4));
+  // particles: sorted grid-unit positions and velocities (float4 each), then the host rows, keys, ranks
+  char* pbase = static_cast<char*>(s->parts.get((size_t)std::max<int64_t>(n, 1) * 64));
+  float4* gs = reinterpret_cast<float4*>(pbase);
+  float4* vs = gs + n;
+  float* px = reinterpret_cast<float*>(pbase + (size_t)n * 32);
+  float* pv = px + 3 * n;
+  uint32_t* key = reinterpret_cast<uint32_t*>(pbase + (size_t)n * 56);
+  uint32_t* rank = key + n;
+  // the cell counts and starts borrow the marching cubes' edge arrays (3 nodes each, free until then)
+  uint32_t* cell_count = crossed;
+  uint32_t* cell_start = vid;
+  const McTables* T = static_cast<const McTables*>(s->tables.ptr);
+  HIP_CHECK(hipEventRecord(s->ev[0], s->stream));
+  if (upload && n > 0) {
+    HIP_CHECK(hipMemcpyAsync(px, x, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(pv, v, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
+  }
+  s->uploaded = n;
+  HIP_CHECK(hipMemsetAsync(cell_count, 0, (size_t)(nodes + 1) * 4, s->stream));
+  HIP_CHECK(hipEventRecord(s->ev[1], s->stream));
+  if (n > 0) {
+    k_surf_cell_keys<<<grid_for(n), kBlock, 0, s->stream>>>(G, px, (int)n, key, rank, cell_count);
+    check_launch("k_surf_cell_keys");
+  }
+  scan_u32_long(s->scan_sums, s->stream, cell_count, cell_start, nodes + 1);
+  if (n > 0) {
+    k_surf_sort_particles<<<grid_for(n), kBlock, 0, s->stream>>>(G, px, pv, (int)n, key, rank, cell_start, gs, vs);
+    check_launch("k_surf_sort_particles");
+  }
+  // every node is written (zero where no particle reaches it)
+  const int reach = (int)std::min(std::ceil((double)(G.h * G.inv_cell)), (double)(1 << 20));
+  const int nb[3] = {(G.dims[0] + 3) / 4, (G.dims[1] + 3) / 4, (G.dims[2] + 3) / 4};
+  const int64_t n_bricks = (int64_t)nb[0] * nb[1] * nb[2];
+  k_surf_gather<<<(unsigned)((n_bricks + 3) / 4), 256, 0, s->stream>>>(G, reach, gs, vs, cell_start, nb[1], nb[2],
+                                                                      n_bricks, density, momentum);
+  check_launch("k_surf_gather");
+  HIP_CHECK(hipEventRecord(s->ev[2], s->stream));
+  k_surf_normalize<<<grid_for(nodes), kBlock, 0, s->stream>>>(nodes, density, momentum, d->inv_rest);
+  check_launch("k_surf_normalize");
+  HIP_CHECK(hipEventRecord(s->ev[3], s->stream));
+  // pass 0: crossed edges and triangles per cell, then their offsets
+  k_surf_edges<<<grid_for(3 * nodes), kBlock, 0, s->stream>>>(G, density, momentum, d->iso, 0, crossed, nullptr,
+                                                               nullptr, nullptr);
+  check_launch("k_surf_edges");
+  k_surf_cubes<<<grid_for(cells), kBlock, 0, s->stream>>>(G, density, d->iso, T, 0, ntri, nullptr, nullptr,
+                                                           nullptr);
+  check_launch("k_surf_cubes");
+  scan_u32_long(s->scan_sums, s->stream, crossed, vid, 3 * nodes);
+  scan_u32_long(s->scan_sums, s->stream, ntri, tri_off, cells);
+  uint32_t tail[4];
+  HIP_CHECK(hipMemcpyAsync(&tail[0], vid + 3 * nodes - 1, 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_CHECK(hipMemcpyAsync(&tail[1], crossed + 3 * nodes - 1, 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_CHECK(hipMemcpyAsync(&tail[2], tri_off + cells - 1, 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_CHECK(hipMemcpyAsync(&tail[3], ntri + cells - 1, 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_CHECK(hipStreamSynchronize(s->stream));
+  const int64_t nv = (int64_t)tail[0] + tail[1], nt = (int64_t)tail[2] + tail[3];
+  if (nt > INT32_MAX) fail(MPM_ERR_VALUE, "surface has too many triangles for one call");
+  if (nv > 0) {
+    float* verts = static_cast<float*>(s->verts.get((size_t)nv * 12));
+    float* vvel = static_cast<float*>(s->vvel.get((size_t)nv * 12));
+    k_surf_edges<<<grid_for(3 * nodes), kBlock, 0, s->stream>>>(G, density, momentum, d->iso, 1, nullptr, vid,
+                                                                 verts, vvel);
+    check_launch("k_surf_edges");
+  }
+  if (nt > 0) {
+    int32_t* tris = static_cast<int32_t*>(s->tris.get((size_t)nt * 12));
+    k_surf_cubes<<<grid_for(cells), kBlock, 0, s->stream>>>(G, density, d->iso, T, 1, nullptr, tri_off, vid,
+                                                             tris);
+    check_launch("k_surf_cubes");
+  }
+  HIP_CHECK(hipEventRecord(s->ev[4], s->stream));
+  HIP_CHECK(hipStreamSynchronize(s->stream));
+  s->n_verts = nv;
+  s->n_tris = nt;
+  s->built = true;
+  *n_verts = nv;
+  *n_tris = nt;
+}
+
 
 }  // namespace
 
@@ -2440,6 +2563,7 @@ int32_t mpm_surface_create(int32_t device, mpm_surface** out) {
   const int32_t rc = run_on(device, [&] {
     HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (hipEvent_t& e : s->ev) HIP_CHECK(hipEventCreate(&e));
+    for (hipEvent_t& e : s->ev_upload) HIP_CHECK(hipEventCreate(&e));
     const McTables t = build_mc_tables();
     HIP_CHECK(hipMemcpyAsync(s->tables.get(sizeof(McTables)), &t, sizeof(McTables), hipMemcpyHostToDevice,
                              s->stream));
@@ -2462,114 +2586,57 @@ int32_t mpm_surface_build(mpm_surface* s, const mpm_surface_desc* d, int64_t n, 
                           int64_t* n_verts, int64_t* n_tris) {
   if (!s || !d || !n_verts || !n_tris) return MPM_ERR_VALUE;
   return run_on(s->device, [&] {
-    *n_verts = *n_tris = 0;
+    s->from_upload = false;
+    surface_build(s, d, n, x, v, true, n_verts, n_tris);
+  });
+}
+
+int32_t mpm_surface_upload(mpm_surface* s, int64_t n, const float* x, const float* v, float* lo, float* hi) {
+  if (!s || !lo || !hi) return MPM_ERR_VALUE;
+  return run_on(s->device, [&] {
+    s->uploaded = -1;
     s->built = false;
-    s->n_verts = s->n_tris = 0;
-    if (n < 0) fail(MPM_ERR_VALUE, "negative particle count");
+    if (n <= 0) fail(MPM_ERR_VALUE, "no particles to upload");
     if (n > INT32_MAX) fail(MPM_ERR_VALUE, "too many particles for one surfacing call");
-    if (n > 0 && (!x || !v)) fail(MPM_ERR_VALUE, "null particle array");
-    int64_t nodes = 1;
-    for (int a = 0; a < 3; ++a) {
-      if (d->dims[a] < 2) fail(MPM_ERR_VALUE, "the field needs at least 2 nodes per axis");
-      if (!std::isfinite(d->origin[a])) fail(MPM_ERR_VALUE, "non-finite field origin");
-      nodes *= d->dims[a];
-      if (nodes > kMaxSurfNodes) fail(MPM_ERR_VALUE, "field too large for one surfacing call");
-    }
-    if (!finite_positive(d->cell) || !finite_positive(d->h) || !finite_positive(d->inv_rest) || !std::isfinite(d->iso))
-      fail(MPM_ERR_VALUE, "cell, h and inv_rest must be positive and finite, iso finite");
-    SurfGrid G{};
-    for (int a = 0; a < 3; ++a) {
-      G.dims[a] = d->dims[a];
-      G.origin[a] = d->origin[a];
-    }
-    G.cell = d->cell;
-    G.inv_cell = 1.0f / d->cell;
-    G.h = d->h;
-    s->grid = G;
-    s->n_nodes = nodes;
-    const int64_t cells = (int64_t)(G.dims[0] - 1) * (G.dims[1] - 1) * (G.dims[2] - 1);
-    float* density = static_cast<float*>(s->density.get((size_t)nodes * 4));
-    float* momentum = static_cast<float*>(s->momentum.get((size_t)nodes * 12));
-    uint32_t* crossed = static_cast<uint32_t*>(s->crossed.get((size_t)nodes * 12));
-    uint32_t* vid = static_cast<uint32_t*>(s->vid.get((size_t)nodes * 12));
-    uint32_t* ntri = static_cast<uint32_t*>(s->ntri.get((size_t)cells * 4));
-    uint32_t* tri_off = static_cast<uint32_t*>(s->tri_off.get((size_t)cells * 4));
-    // particles: sorted grid-unit positions and velocities (float4 each), then the host rows, keys, ranks
-    char* pbase = static_cast<char*>(s->parts.get((size_t)std::max<int64_t>(n, 1) * 64));
-    float4* gs = reinterpret_cast<float4*>(pbase);
-    float4* vs = gs + n;
+    if (!x || !v) fail(MPM_ERR_VALUE, "null particle array");
+    char* pbase = static_cast<char*>(s->parts.get((size_t)n * 64));  // the build's particle layout
     float* px = reinterpret_cast<float*>(pbase + (size_t)n * 32);
     float* pv = px + 3 * n;
-    uint32_t* key = reinterpret_cast<uint32_t*>(pbase + (size_t)n * 56);
-    uint32_t* rank = key + n;
-    // the cell counts and starts borrow the marching cubes' edge arrays (3 nodes each, free until then)
-    uint32_t* cell_count = crossed;
-    uint32_t* cell_start = vid;
-    const McTables* T = static_cast<const McTables*>(s->tables.ptr);
-    HIP_CHECK(hipEventRecord(s->ev[0], s->stream));
-    if (n > 0) {
-      HIP_CHECK(hipMemcpyAsync(px, x, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
-      HIP_CHECK(hipMemcpyAsync(pv, v, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
-    }
-    HIP_CHECK(hipMemsetAsync(cell_count, 0, (size_t)(nodes + 1) * 4, s->stream));
-    HIP_CHECK(hipEventRecord(s->ev[1], s->stream));
-    if (n > 0) {
-      k_surf_cell_keys<<<grid_for(n), kBlock, 0, s->stream>>>(G, px, (int)n, key, rank, cell_count);
-      check_launch("k_surf_cell_keys");
-    }
-    scan_u32_long(s->scan_sums, s->stream, cell_count, cell_start, nodes + 1);
-    if (n > 0) {
-      k_surf_sort_particles<<<grid_for(n), kBlock, 0, s->stream>>>(G, px, pv, (int)n, key, rank, cell_start, gs, vs);
-      check_launch("k_surf_sort_particles");
-    }
-    // every node is written (zero where no particle reaches it)
-    const int reach = (int)std::min(std::ceil((double)(G.h * G.inv_cell)), (double)(1 << 20));
-    const int nb[3] = {(G.dims[0] + 3) / 4, (G.dims[1] + 3) / 4, (G.dims[2] + 3) / 4};
-    const int64_t n_bricks = (int64_t)nb[0] * nb[1] * nb[2];
-    k_surf_gather<<<(unsigned)((n_bricks + 3) / 4), 256, 0, s->stream>>>(G, reach, gs, vs, cell_start, nb[1], nb[2],
-                                                                        n_bricks, density, momentum);
-    check_launch("k_surf_gather");
-    HIP_CHECK(hipEventRecord(s->ev[2], s->stream));
-    k_surf_normalize<<<grid_for(nodes), kBlock, 0, s->stream>>>(nodes, density, momentum, d->inv_rest);
-    check_launch("k_surf_normalize");
-    HIP_CHECK(hipEventRecord(s->ev[3], s->stream));
-    // pass 0: crossed edges and triangles per cell, then their offsets
-    k_surf_edges<<<grid_for(3 * nodes), kBlock, 0, s->stream>>>(G, density, momentum, d->iso, 0, crossed, nullptr,
-                                                                 nullptr, nullptr);
-    check_launch("k_surf_edges");
-    k_surf_cubes<<<grid_for(cells), kBlock, 0, s->stream>>>(G, density, d->iso, T, 0, ntri, nullptr, nullptr,
-                                                             nullptr);
-    check_launch("k_surf_cubes");
-    scan_u32_long(s->scan_sums, s->stream, crossed, vid, 3 * nodes);
-    scan_u32_long(s->scan_sums, s->stream, ntri, tri_off, cells);
-    uint32_t tail[4];
-    HIP_CHECK(hipMemcpyAsync(&tail[0], vid + 3 * nodes - 1, 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_CHECK(hipMemcpyAsync(&tail[1], crossed + 3 * nodes - 1, 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_CHECK(hipMemcpyAsync(&tail[2], tri_off + cells - 1, 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_CHECK(hipMemcpyAsync(&tail[3], ntri + cells - 1, 4, hipMemcpyDeviceToHost, s->stream));
+    float* partial = static_cast<float*>(s->bounds.get((size_t)kSurfBoundsBlocks * 8 * sizeof(float)));
+    HIP_CHECK(hipEventRecord(s->ev_upload[0], s->stream));
+    HIP_CHECK(hipMemcpyAsync(px, x, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(pv, v, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, kSurfBoundsBlocks);
+    k_surf_bounds<<<blocks, 256, 0, s->stream>>>(px, (int)n, partial);
+    check_launch("k_surf_bounds");
+    HIP_CHECK(hipEventRecord(s->ev_upload[1], s->stream));
+    std::vector<float> h((size_t)blocks * 8);
+    HIP_CHECK(hipMemcpyAsync(h.data(), partial, h.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     HIP_CHECK(hipStreamSynchronize(s->stream));
-    const int64_t nv = (int64_t)tail[0] + tail[1], nt = (int64_t)tail[2] + tail[3];
-    if (nt > INT32_MAX) fail(MPM_ERR_VALUE, "surface has too many triangles for one call");
-    if (nv > 0) {
-      float* verts = static_cast<float*>(s->verts.get((size_t)nv * 12));
-      float* vvel = static_cast<float*>(s->vvel.get((size_t)nv * 12));
-      k_surf_edges<<<grid_for(3 * nodes), kBlock, 0, s->stream>>>(G, density, momentum, d->iso, 1, nullptr, vid,
-                                                                   verts, vvel);
-      check_launch("k_surf_edges");
+    bool nan = false;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = h[a];
+      hi[a] = h[3 + a];
     }
-    if (nt > 0) {
-      int32_t* tris = static_cast<int32_t*>(s->tris.get((size_t)nt * 12));
-      k_surf_cubes<<<grid_for(cells), kBlock, 0, s->stream>>>(G, density, d->iso, T, 1, nullptr, tri_off, vid,
-                                                               tris);
-      check_launch("k_surf_cubes");
+    for (int b = 0; b < blocks; ++b) {
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], h[8 * b + a]);
+        hi[a] = std::max(hi[a], h[8 * b + 3 + a]);
+      }
+      nan = nan || h[8 * b + 6] != 0.0f;
     }
-    HIP_CHECK(hipEventRecord(s->ev[4], s->stream));
-    HIP_CHECK(hipStreamSynchronize(s->stream));
-    s->n_verts = nv;
-    s->n_tris = nt;
-    s->built = true;
-    *n_verts = nv;
-    *n_tris = nt;
+    if (nan)  // as numpy's min and max of a column holding NaN
+      for (int a = 0; a < 3; ++a) lo[a] = hi[a] = std::numeric_limits<float>::quiet_NaN();
+    s->uploaded = n;
+  });
+}
+
+int32_t mpm_surface_build_uploaded(mpm_surface* s, const mpm_surface_desc* d, int64_t* n_verts, int64_t* n_tris) {
+  if (!s || !d || !n_verts || !n_tris) return MPM_ERR_VALUE;
+  return run_on(s->device, [&] {
+    if (s->uploaded < 0) fail(MPM_ERR_VALUE, "no particles uploaded");
+    surface_build(s, d, s->uploaded, nullptr, nullptr, false, n_verts, n_tris);
+    s->from_upload = true;
   });
 }
 
@@ -2606,6 +2673,11 @@ int32_t mpm_surface_times(const mpm_surface* s, float* ms4) {
     if (!s->built) fail(MPM_ERR_VALUE, "no surface built");
     const int from[4] = {0, 1, 2, 3}, to[4] = {1, 2, 3, 4};
     for (int i = 0; i < 4; ++i) HIP_CHECK(hipEventElapsedTime(&ms4[i], s->ev[from[i]], s->ev[to[i]]));
+    if (s->from_upload) {  // the copies ran in mpm_surface_upload (with the bounds); the build only cleared
+      float up = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&up, s->ev_upload[0], s->ev_upload[1]));
+      ms4[0] += up;
+    }
   });
 }
 

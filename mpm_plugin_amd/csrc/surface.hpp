@@ -365,6 +365,34 @@ __global__ void __launch_bounds__(256) k_surf_cubes(SurfGrid G, const float* __r
   }
 }
 
+// per block: the minimum and maximum of each coordinate over its particles (grid-stride), and 1 in slot 6
+// when any coordinate is NaN; 8 floats per block
+__global__ void __launch_bounds__(256) k_surf_bounds(const float* __restrict__ x, int n, float* __restrict__ partial) {
+  __shared__ float s_v[7][256];
+  float v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0f};
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+    for (int a = 0; a < 3; ++a) {
+      const float c = x[3 * p + a];
+      v[a] = fminf(v[a], c);
+      v[3 + a] = fmaxf(v[3 + a], c);
+      if (isnan(c)) v[6] = 1.0f;
+    }
+  for (int q = 0; q < 7; ++q) s_v[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int half = 128; half > 0; half >>= 1) {
+    if ((int)threadIdx.x < half) {
+      const int o = threadIdx.x + half;
+      for (int a = 0; a < 3; ++a) {
+        s_v[a][threadIdx.x] = fminf(s_v[a][threadIdx.x], s_v[a][o]);
+        s_v[3 + a][threadIdx.x] = fmaxf(s_v[3 + a][threadIdx.x], s_v[3 + a][o]);
+      }
+      s_v[6][threadIdx.x] = fmaxf(s_v[6][threadIdx.x], s_v[6][o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 7) partial[8 * blockIdx.x + threadIdx.x] = s_v[threadIdx.x][0];
+}
+
 // adds the total of the preceding chunk of a chunked exclusive scan (its last input and output, which this
 // launch does not touch) to every output of the current chunk
 __global__ void __launch_bounds__(256) k_scan_carry(uint32_t* __restrict__ out, int64_t n,

@@ -71,20 +71,35 @@ class Surfacer:
 
     def build(self, desc: N.SurfaceDesc, x: np.ndarray, v: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """Vertices (grid units), triangles and vertex velocities of the iso surface of x's density."""
-        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
-        v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
-        if len(v) != len(x):
-            raise ValueError("positions and velocities differ in length")
+        x, v = _rows(x, v)
         with self._lock:
             nv, nt = N.C.c_int64(), N.C.c_int64()
             N.check(self._lib.mpm_surface_build(self._handle, N.C.byref(desc), len(x), N.fptr(x), N.fptr(v),
                                                 N.C.byref(nv), N.C.byref(nt)))
-            verts = np.empty((nv.value, 3), np.float32)
-            vel = np.empty((nv.value, 3), np.float32)
-            tris = np.empty((nt.value, 3), np.int32)
-            N.check(self._lib.mpm_surface_fetch(self._handle, N.fptr(verts), N.iptr(tris), N.fptr(vel)))
-            ms = np.zeros(4, np.float32)
-            N.check(self._lib.mpm_surface_times(self._handle, N.fptr(ms)))
+            return self._fetch(nv.value, nt.value)
+
+    def mesh(self, x: np.ndarray, v: np.ndarray, describe):
+        """Upload the particles, let ``describe(lo, hi)`` place the field from their bounds (computed on the
+        device; it returns the description and anything else to hand back, or raises), and mesh them.
+        Returns (vertices in grid units, triangles, vertex velocities, what describe handed back)."""
+        x, v = _rows(x, v)
+        lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        with self._lock:
+            N.check(self._lib.mpm_surface_upload(self._handle, len(x), N.fptr(x), N.fptr(v), N.fptr(lo),
+                                                 N.fptr(hi)))
+            desc, extra = describe(lo, hi)
+            nv, nt = N.C.c_int64(), N.C.c_int64()
+            N.check(self._lib.mpm_surface_build_uploaded(self._handle, N.C.byref(desc), N.C.byref(nv),
+                                                         N.C.byref(nt)))
+            return self._fetch(nv.value, nt.value) + (extra,)
+
+    def _fetch(self, nv: int, nt: int):
+        verts = np.empty((nv, 3), np.float32)
+        vel = np.empty((nv, 3), np.float32)
+        tris = np.empty((nt, 3), np.int32)
+        N.check(self._lib.mpm_surface_fetch(self._handle, N.fptr(verts), N.iptr(tris), N.fptr(vel)))
+        ms = np.zeros(4, np.float32)
+        N.check(self._lib.mpm_surface_times(self._handle, N.fptr(ms)))
         self.last_times = dict(zip(("upload", "splat", "normalize", "marching_cubes"), map(float, ms)))
         return verts, tris, vel
 
@@ -95,6 +110,14 @@ class Surfacer:
         with self._lock:
             N.check(self._lib.mpm_surface_get_field(self._handle, N.fptr(density), N.fptr(velocity)))
         return density, velocity
+
+
+def _rows(x: np.ndarray, v: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+    v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+    if len(v) != len(x):
+        raise ValueError("positions and velocities differ in length")
+    return x, v
 
 
 _surfacers: dict[int, Surfacer] = {}
@@ -125,19 +148,24 @@ def free_device_memory(device: int) -> int:
     return free.value
 
 
-def field_grid(positions: np.ndarray, spacing: float, settings: SurfaceSettings):
-    """The field's origin (float64), node counts, cell and kernel radius (field.py:112-118): the particles'
-    bounds padded by h + 2 cells, so that the density falls below iso before the grid's edge and every
-    mesh closes."""
+def grid_from_bounds(lo: np.ndarray, hi: np.ndarray, spacing: float, settings: SurfaceSettings):
+    """The field's origin (float64), node counts, cell and kernel radius from the particles' float32 bounds
+    (field.py:112-118): the bounds padded by h + 2 cells, so that the density falls below iso before the
+    grid's edge and every mesh closes."""
     cell = spacing / settings.detail
     h = 2.0 * spacing * settings.smoothing
     pad = h + 2.0 * cell
+    origin = np.asarray(lo, np.float32).astype(np.float64) - pad
+    dims = tuple(int(math.ceil(e / cell)) + 1 for e in np.asarray(hi, np.float32) + pad - origin)
+    return origin, dims, cell, h
+
+
+def field_grid(positions: np.ndarray, spacing: float, settings: SurfaceSettings):
+    """grid_from_bounds of the particles' bounds computed on the host."""
     # per column: numpy's axis-0 reduction over (N, 3) rows is ten times slower, with the same result
     lo = np.array([positions[:, a].min() for a in range(3)], np.float32)
     hi = np.array([positions[:, a].max() for a in range(3)], np.float32)
-    origin = lo.astype(np.float64) - pad
-    dims = tuple(int(math.ceil(e / cell)) + 1 for e in hi + pad - origin)
-    return origin, dims, cell, h
+    return grid_from_bounds(lo, hi, spacing, settings)
 
 
 def surface_particles(positions: np.ndarray, velocities: np.ndarray, spacing: float, settings: SurfaceSettings,
@@ -148,23 +176,28 @@ def surface_particles(positions: np.ndarray, velocities: np.ndarray, spacing: fl
     if len(positions) == 0:
         return SurfaceMesh.empty()
     ordinal = device_ordinal(device)
-    origin, dims, cell, h = field_grid(positions, spacing, settings)
-    cells = dims[0] * dims[1] * dims[2]
-    needed = cells * BYTES_PER_CELL
     if max_bytes is None:
         max_bytes = int(0.7 * (free_device_memory(ordinal) or 8 * 1024**3))
-    if needed > max_bytes:
-        raise ValueError(f"surfacing needs a {dims[0]}x{dims[1]}x{dims[2]} grid ({cells:,} cells, about "
-                         f"{needed / 1e9:.1f} GB of GPU memory), over the budget of {max_bytes / 1e9:.1f} GB; "
-                         f"lower the detail or split the emitter")
-    desc = N.SurfaceDesc()
-    desc.dims[:] = dims
-    desc.origin[:] = [float(o) for o in origin]
-    desc.cell = cell
-    desc.h = h
-    desc.inv_rest = 1.0 / rest_density(spacing, h)
-    desc.iso = settings.iso
-    g_verts, tris, vel = surfacer(ordinal).build(desc, positions, velocities)
+
+    def describe(lo, hi):
+        """The field for the particles' bounds (from the device), within the memory budget."""
+        origin, dims, cell, h = grid_from_bounds(lo, hi, spacing, settings)
+        cells = dims[0] * dims[1] * dims[2]
+        needed = cells * BYTES_PER_CELL
+        if needed > max_bytes:
+            raise ValueError(f"surfacing needs a {dims[0]}x{dims[1]}x{dims[2]} grid ({cells:,} cells, about "
+                             f"{needed / 1e9:.1f} GB of GPU memory), over the budget of {max_bytes / 1e9:.1f} GB; "
+                             f"lower the detail or split the emitter")
+        desc = N.SurfaceDesc()
+        desc.dims[:] = dims
+        desc.origin[:] = [float(o) for o in origin]
+        desc.cell = cell
+        desc.h = h
+        desc.inv_rest = 1.0 / rest_density(spacing, h)
+        desc.iso = settings.iso
+        return desc, (origin, cell)
+
+    g_verts, tris, vel, (origin, cell) = surfacer(ordinal).mesh(positions, velocities, describe)
     if len(g_verts) == 0:
         return SurfaceMesh.empty()
     verts = (origin + g_verts.astype(np.float64) * cell).astype(np.float32)

@@ -217,3 +217,26 @@ def test_invalid_descriptions_rejected():
     desc.cell, desc.h, desc.inv_rest, desc.iso = 0.1, 0.2, 1.0, 0.5
     with pytest.raises(ValueError):
         s.build(desc, p, p)
+
+
+def test_device_bounds_match_numpy():
+    """mpm_surface_upload's bounds are numpy's column minima and maxima, so the field is placed exactly as
+    the reference places it; a NaN coordinate gives NaN bounds, and surface_particles raises ValueError
+    (as the reference's math.ceil of a NaN extent does)."""
+    from mpm_plugin_amd import _native as N
+
+    s = surfacer("cuda:0")
+    rng = np.random.default_rng(7)
+    for n in (1, 300, 1_000_003):
+        x = (rng.normal(size=(n, 3)) * [1.0, 3.0, 0.1] + [5.0, -2.0, 0.3]).astype(np.float32)
+        lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        N.check(s._lib.mpm_surface_upload(s._handle, n, N.fptr(x), N.fptr(x), N.fptr(lo), N.fptr(hi)))
+        np.testing.assert_array_equal(lo, [x[:, a].min() for a in range(3)])
+        np.testing.assert_array_equal(hi, [x[:, a].max() for a in range(3)])
+    x[123, 1] = np.nan
+    N.check(s._lib.mpm_surface_upload(s._handle, len(x), N.fptr(x), N.fptr(x), N.fptr(lo), N.fptr(hi)))
+    assert np.isnan(lo).all() and np.isnan(hi).all()
+    p = sphere_particles((0.5, 0.5, 0.5), 0.05, DX)
+    p[7, 2] = np.nan
+    with pytest.raises(ValueError):
+        mesh(p)
