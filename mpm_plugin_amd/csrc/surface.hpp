@@ -147,16 +147,8 @@ __device__ __forceinline__ int64_t surf_node(const SurfGrid& G, int i, int j, in
   return ((int64_t)i * G.dims[1] + j) * G.dims[2] + k;
 }
 
-__device__ __forceinline__ float surf_cubic(float q) {
-  if (q < 0.5f) return 1.0f - 6.0f * q * q + 6.0f * q * q * q;
-  if (q < 1.0f) {
-    const float t = 1.0f - q;
-    return 2.0f * t * t * t;
-  }
-  return 0.0f;
-}
-
-// surf_cubic in straight-line form: both pieces computed, the outer one clamped to zero past q = 1
+// the unnormalised cubic B-spline of support [0, 1) (field.py:24-32) in straight-line form: both pieces
+// computed, the outer one clamped to zero past q = 1
 __device__ __forceinline__ float surf_cubic_select(float q) {
   const float t = fmaxf(1.0f - q, 0.0f);
   const float inner = 1.0f - 6.0f * q * q + 6.0f * q * q * q;
