@@ -16,10 +16,10 @@
 // the face's counter-clockwise walk seen from outside (from where the walk leaves the inside, field > iso,
 // to where it enters it); a face with four crossings (inside corners on one diagonal) cuts each inside
 // corner off. The face rule depends only on the face's four corners, so two cells sharing a face make the
-// same segments there, traversed in opposite directions: the surface is closed and consistently wound. The segments of a configuration chain
-// into loops (every crossed edge starts one segment and ends another), each fanned into triangles wound so
-// that their normals point away from the inside: a cut-off inside corner at the origin gives the triangle
-// (x edge, y edge, z edge), normal +(1, 1, 1).
+// same segments there, traversed in opposite directions: the surface is closed and consistently wound.
+// The segments of a configuration chain into loops (every crossed edge starts one segment and ends
+// another), each fanned into triangles wound so that their normals point away from the inside: a cut-off
+// inside corner at the origin gives the triangle (x edge, y edge, z edge), normal +(1, 1, 1).
 #pragma once
 
 #include "kernels.hpp"
@@ -198,26 +198,37 @@ __global__ void __launch_bounds__(256) k_surf_sort_particles(SurfGrid G, const f
   vs[q] = make_float4(v[3 * p], v[3 * p + 1], v[3 * p + 2], 0.0f);
 }
 
-// wave b: nodes (4 bi + lane / 16, 4 bj + lane / 4 % 4, 4 bk + lane % 4) of brick b; reach = ceil(h / cell)
-__global__ void __launch_bounds__(256) k_surf_gather(SurfGrid G, int reach, const float4* __restrict__ gs,
-                                                     const float4* __restrict__ vs, const uint32_t* __restrict__ start,
-                                                     int nb1, int nb2, int64_t n_bricks, float* __restrict__ density,
-                                                     float* __restrict__ momentum) {
+constexpr int kGatherBricks = 4;  // neighbouring bricks per workgroup: they read many of the same particles
+
+// brick b = kGatherBricks blockIdx.x + (wave % kGatherBricks): its nodes (4 bi + lane / 16, 4 bj + lane / 4
+// % 4, 4 bk + lane % 4), one per lane of each of its kGatherWaves waves; wave half = wave / kGatherBricks
+// sums the particles of the cell planes ci = i0 + half, i0 + half + kGatherWaves, ... and the waves' sums
+// meet in LDS. At the default radius (4 cells) two shorter waves per brick rather than one long one keep
+// the chip busy to the end of the launch, where the bricks inside the material (about 1300 particles each)
+// would otherwise run on after the light ones have drained; at larger radii a row of cells holds a few
+// particles and one wave per brick is faster. reach = ceil(h / cell).
+template <int kGatherWaves>
+__global__ void __launch_bounds__(64 * kGatherWaves * kGatherBricks)
+    k_surf_gather(SurfGrid G, int reach, const float4* __restrict__ gs, const float4* __restrict__ vs,
+                  const uint32_t* __restrict__ start, int nb1, int nb2, int64_t n_bricks, float* __restrict__ density,
+                  float* __restrict__ momentum) {
+  __shared__ float4 s_part[kGatherWaves > 1 ? kGatherWaves - 1 : 1][kGatherBricks][64];
   const int lane = threadIdx.x & 63;
   // the wave's index, read from the first lane so that the compiler knows it (and every particle index
   // derived from it) to be uniform: the particles then come through scalar loads
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-  if (b >= n_bricks) return;  // whole waves
+  const int local = wave % kGatherBricks, half = wave / kGatherBricks;
+  const int64_t b = (int64_t)blockIdx.x * kGatherBricks + local;
+  const bool live = b < n_bricks;  // a brick past the end still meets the workgroup's barrier
   const int bk = (int)(b % nb2) * 4, bj = (int)((b / nb2) % nb1) * 4, bi = (int)(b / ((int64_t)nb2 * nb1)) * 4;
   const int i = bi + (lane >> 4), j = bj + ((lane >> 2) & 3), k = bk + (lane & 3);
   const float fi = (float)i, fj = (float)j, fk = (float)k;
   const float q_per_cell = G.cell / G.h;
-  const int i0 = max(bi - reach, 0), i1 = min(bi + 3 + reach, G.dims[0] - 1);
+  const int i0 = max(bi - reach, 0), i1 = live ? min(bi + 3 + reach, G.dims[0] - 1) : -1;
   const int j0 = max(bj - reach, 0), j1 = min(bj + 3 + reach, G.dims[1] - 1);
   const float r = G.h * G.inv_cell, r2 = r * r;
   float sd = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  for (int ci = i0; ci <= i1; ++ci)
+  for (int ci = i0 + half; ci <= i1; ci += kGatherWaves)
     for (int cj = j0; cj <= j1; ++cj) {
       // a particle of cell (ci, cj, ck) lies at least di, dj (and dk) cells from the brick's nodes along each
       // axis: rows out of reach are skipped, and each row keeps the cells within the radius left for dk
@@ -244,7 +255,19 @@ __global__ void __launch_bounds__(256) k_surf_gather(SurfGrid G, int reach, cons
         pv = nv;
       }
     }
-  if (i < G.dims[0] && j < G.dims[1] && k < G.dims[2]) {
+  if (kGatherWaves > 1) {
+    if (half > 0) s_part[half - 1][local][lane] = make_float4(sd, sx, sy, sz);
+    __syncthreads();
+  }
+  if (live && half == 0 && i < G.dims[0] && j < G.dims[1] && k < G.dims[2]) {
+#pragma unroll
+    for (int h = 0; h < kGatherWaves - 1; ++h) {
+      const float4 o = s_part[h][local][lane];
+      sd += o.x;
+      sx += o.y;
+      sy += o.z;
+      sz += o.w;
+    }
     const int64_t c = surf_node(G, i, j, k);
     density[c] = sd;
     momentum[3 * c] = sx;

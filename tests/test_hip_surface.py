@@ -71,9 +71,9 @@ def device_field(p, v, settings):
                 density=density, velocity=velocity)
 
 
-# a support of radius 3 cells (a thread per particle in the splat) and of 8 cells (a wave per particle)
-BLOB_SETTINGS = {"thread_splat": SurfaceSettings(smoothing=1.5, iso=0.4),
-                 "wave_splat": SurfaceSettings(detail=2.0, iso=0.4)}
+# a kernel reach of 3 cells (the gather's two waves per brick) and of 8 cells (one wave per brick)
+BLOB_SETTINGS = {"reach3_two_waves": SurfaceSettings(smoothing=1.5, iso=0.4),
+                 "reach8_one_wave": SurfaceSettings(detail=2.0, iso=0.4)}
 
 
 @pytest.fixture(scope="module", params=sorted(BLOB_SETTINGS))
