@@ -1598,6 +1598,13 @@ template <class Fn> int32_t run(mpm_ctx* c, Fn&& fn) { return run_on(c ? c->devi
 
 }  // namespace
 
+// the layouts the ctypes binding mirrors (mpm_plugin_amd/_native.py; checked there by tests/test_native_abi.py)
+static_assert(sizeof(mpm_domain_desc) == 184, "mpm_domain_desc layout");
+static_assert(sizeof(mpm_material_desc) == 48, "mpm_material_desc layout");
+static_assert(sizeof(mpm_collider_pose) == 132, "mpm_collider_pose layout");
+static_assert(sizeof(mpm_advance_stats) == 24, "mpm_advance_stats layout");
+static_assert(sizeof(mpm_surface_desc) == 40, "mpm_surface_desc layout");
+
 extern "C" {
 
 int32_t mpm_abi_version(void) { return MPM_ABI_VERSION; }

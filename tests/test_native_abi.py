@@ -59,3 +59,10 @@ def test_device_strings():
     assert device_ordinal(None) == 0 and device_ordinal("cuda:0") == 0 and device_ordinal("hip:3") == 3
     with pytest.raises(ValueError):
         device_ordinal("cpu")
+
+
+def test_struct_layouts_match_the_library():
+    """The ctypes structs have the sizes the library's static_asserts fix for the C structs of mpm_hip.h."""
+    sizes = {"DomainDesc": 184, "MaterialDesc": 48, "ColliderPose": 132, "AdvanceStats": 24, "SurfaceDesc": 40}
+    for name, size in sizes.items():
+        assert ctypes.sizeof(getattr(N, name)) == size, name
