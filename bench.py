@@ -37,7 +37,9 @@ The printed JSON line carries:
                 particle x particles + bytes per active grid node x active nodes) over its mean launch
                 time, against 8 TB/s HBM
   cpu_baseline  the CPU oracle (a restatement of the reference Warp kernels in C++, OpenMP, FP32) timed
-                on a bounded number of substeps of the same workload on this host's cores
+                on a bounded number of substeps of the same workload on this host's cores; with it, the
+                surfacing object gets cpu_splat, the oracle's numpy splat timed on a bounded sample of the
+                surfaced frame
 """
 
 from __future__ import annotations
@@ -156,6 +158,28 @@ def cpu_baseline(w, seconds: float) -> dict:
     return {"value": w.n * steps / elapsed, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} substeps of the same {w.n}-particle workload (FP32 C++ restatement of the "
                       f"reference Warp kernels, OpenMP, {threads} threads on {cpu}), {elapsed:.1f} s"}
+
+
+def cpu_surface_baseline(particles, spacing: float, seconds: float = 5.0) -> dict:
+    """Time the CPU oracle's splat (numpy, the reference's field.py:35-63 restated) on a bounded sample of the
+    surfaced frame: the first particles of emitter 0, as many as about ``seconds`` of work allows. The
+    reference has no CPU surfacer (Warp), so this is the CPU path beside the device splat, not a target."""
+    from mpm_surface.field import field_grid
+    from mpm_surface.settings import SurfaceSettings
+    from oracle import surface as OS
+
+    x = particles.positions[particles.emitter_ids == particles.emitter_ids.min()]
+    v = particles.velocities[particles.emitter_ids == particles.emitter_ids.min()]
+    origin, dims, cell, h = field_grid(x, spacing, SurfaceSettings())
+    done, elapsed, chunk = 0, 0.0, 256
+    t0 = time.perf_counter()
+    while elapsed < seconds and done < len(x):
+        OS.splat(x[done:done + chunk], v[done:done + chunk], origin.astype(np.float32), cell, h, dims)
+        done = min(done + chunk, len(x))
+        elapsed = time.perf_counter() - t0
+    return {"value": done / elapsed, "unit": "particles/s", "cores": 1, "kind": "port",
+            "sample": f"splat of {done} of emitter 0's {len(x)} particles into its {dims[0]}x{dims[1]}x{dims[2]} "
+                      f"field (numpy restatement, one thread), {elapsed:.1f} s"}
 
 
 def run_frames(pairs, frames: int, first_frame: int, advance_all) -> tuple[float, int]:
@@ -424,6 +448,11 @@ def main(argv=None) -> int:
         log("cpu baseline ...")
         baseline = cpu_baseline(w, args.cpu_seconds)
         log(f"cpu baseline {baseline['value']:.3e} particle-substeps/s")
+        if surfaced is not None:
+            surfaced["cpu_splat"] = cpu_surface_baseline(final, w.domain.dx / 2)
+            surfaced["device_splat_particles_per_s"] = surfaced["particles"] / (surfaced["device_ms"]["splat"] / 1e3)
+            log(f"cpu surfacing baseline {surfaced['cpu_splat']['value']:.3e} particles/s splatted "
+                f"(device: {surfaced['device_splat_particles_per_s']:.3e})")
 
     batched = None
     solver.close()
