@@ -24,7 +24,8 @@ PAD = 3
 CFL = 0.4
 MODEL_IDS = {"elastic": 0, "von_mises": 1, "drucker_prager": 2, "snow": 3}
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libmpm_oracle.so")
+# MPM_ORACLE_LIB names another build of cpu_ref.cpp (the mutation tests load deliberately broken copies)
+_LIB_PATH = os.environ.get("MPM_ORACLE_LIB") or os.path.join(_HERE, "libmpm_oracle.so")
 
 
 class OracleDomain(C.Structure):
@@ -379,9 +380,13 @@ class OracleSolver:
         fn(C.byref(self.d), self.m, self.n, *self._args(), _ptr(self.s.mat, C.c_int32), _ptr(self.gv, ct), dt,
            self.threads)
         if len(self.colliders):
-            lib().oracle_project_particles_f32(C.byref(self.d), self.colliders.arr, len(self.colliders), self.n,
-                                               _ptr(self.s.x, C.c_float), _ptr(self.s.v, C.c_float), s_frac,
-                                               1.0 / self.frame_dt, self.threads)
+            self.project_particles(s_frac)
+
+    def project_particles(self, s_frac: float = 1.0) -> None:
+        """project_particles alone (colliders.py:120-149) on the current state."""
+        lib().oracle_project_particles_f32(C.byref(self.d), self.colliders.arr, len(self.colliders), self.n,
+                                           _ptr(self.s.x, C.c_float), _ptr(self.s.v, C.c_float), s_frac,
+                                           1.0 / self.frame_dt, self.threads)
 
 
 def svd3(F: np.ndarray, precision: str = "f64"):

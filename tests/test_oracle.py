@@ -10,7 +10,9 @@
 * Grid update and walls: per-node closed forms.
 * Colliders: analytic distances of spheres, boxes and planes; rigid-motion velocities.
 * FP32 and FP64 oracles agree on a smooth state.
-A mistake planted in any of these (a dropped term, a wrong sign, a transposed operand) fails a test here.
+The transfer, F-update and contact rules that these invariants cannot see (P2G's affine moment, the F update
+and snow's Jp, plane contact and particle projection) are pinned by closed forms in tests/test_oracle_pins.py;
+tests/test_oracle_mutations.py plants misreadings in a copy of the oracle and checks the two files catch each.
 """
 
 import math
