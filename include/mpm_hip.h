@@ -120,6 +120,16 @@ int32_t mpm_substep(mpm_ctx* ctx, double dt, double s, double frame_dt);
 /* ---- queries ------------------------------------------------------------------------------ */
 /* replaces WarpSolver.is_occupied (solver.py:244-255; kernels.py:272-298) */
 int32_t mpm_is_occupied(mpm_ctx* ctx, int64_t m, const float* points, uint8_t* out);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): mpm_is_occupied with the query points
+ * (m,3) float32 and the result (m,) uint8 in DEVICE memory the caller owns, and the volume grid in a fixed
+ * array of MPM_NOALLOC_MAX_NODES floats (32 MiB) that each GPU holds for the life of the process. The call
+ * allocates nothing; it enqueues the work on the context's stream and returns without waiting, so
+ * mpm_synchronize(ctx) comes before `out` is read. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (contexts on other streams, other host threads) race. A grid of more nodes (bricks of 4^3
+ * nodes over grid_shape + 6) than the array holds is refused with MPM_ERR_VALUE: use mpm_is_occupied. */
+#define MPM_NOALLOC_MAX_NODES 8388608
+int32_t mpm_is_occupied_noalloc(mpm_ctx* ctx, int64_t m, const float* points, uint8_t* out);
 /* the lattice part of sample_mesh_volume (src/mpm_core/geometry.py:89-155; the caller adds the jitter):
    points (i + 0.5) spacing inside the closed mesh (verts: n_verts x 3 float64, tris: n_tris x 3 indices)
    by z-ray crossing parity, as n x 3 float64 rows in the reference's order (column i * ny + j, then z).

@@ -77,6 +77,7 @@ def _declare(L: C.CDLL) -> None:
         "mpm_advance": ([ctx, f64, C.POINTER(AdvanceStats)], i32),
         "mpm_substep": ([ctx, f64, f64, f64], i32),
         "mpm_is_occupied": ([ctx, i64, fp, u8], i32),
+        "mpm_is_occupied_noalloc": ([ctx, i64, fp, u8], i32),
         "mpm_lattice_inside": ([ctx, C.POINTER(C.c_double), i64, ip, i64, f64, i64, C.POINTER(C.c_double), i64p],
                                i32),
         "mpm_debug_p2g": ([ctx, f64], i32),

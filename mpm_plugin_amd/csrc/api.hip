@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -30,6 +31,10 @@
 #include "surface.hpp"
 
 using namespace mpm;
+
+// volume grid of mpm_is_occupied_noalloc: one float per node of a context's brick-rounded padded grid, held by
+// each GPU the library runs on until the process ends (MPM_NOALLOC_MAX_NODES * 4 bytes = 32 MiB)
+static __device__ __align__(256) float scratch[MPM_NOALLOC_MAX_NODES];
 
 namespace {
 
@@ -293,6 +298,31 @@ struct DeviceGuard {
   }
   ~DeviceGuard() { (void)hipSetDevice(prev); }
 };
+
+// ---- the fixed volume buffer of mpm_is_occupied_noalloc ------------------------------------------
+// Host code never passes `scratch` itself to a kernel: in host code the name stands for a host-side shadow.
+// Each device's own address comes from hipGetSymbolAddress, taken once per device when the first context is
+// created there, and every use goes through that address.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+float* g_noalloc_volume[kMaxDevices] = {};
+
+void noalloc_volume_setup(int device) {  // with `device` current
+  if (device < 0 || device >= kMaxDevices) fail(MPM_ERR_VALUE, "device ordinal beyond the library's table");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_volume[device]) return;
+  void* p = nullptr;
+  hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (!p) fail(MPM_ERR_RUNTIME, "hipGetSymbolAddress(scratch) returned null");
+  g_noalloc_volume[device] = static_cast<float*>(p);
+}
+
+float* noalloc_volume(int device) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  float* p = (device >= 0 && device < kMaxDevices) ? g_noalloc_volume[device] : nullptr;
+  if (!p) fail(MPM_ERR_RUNTIME, "the fixed volume buffer was not set up on this device");
+  return p;
+}
 
 // ---- launch timing -------------------------------------------------------------------------
 hipEvent_t pool_event(Timing& t) {
@@ -1633,6 +1663,7 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
         fail(MPM_ERR_NOT_IMPLEMENTED, "material model is not supported by the HIP backend");
     for (int a = 0; a < 3; ++a)
       if (domain->dims[a] < 3) fail(MPM_ERR_VALUE, "grid dimensions must be at least 3 nodes");
+    noalloc_volume_setup(device);
     HIP_CHECK(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
     c->stream = c->own_stream;
     c->dom = *domain;
@@ -2034,6 +2065,31 @@ int32_t mpm_is_occupied(mpm_ctx* c, int64_t m, const float* pts, uint8_t* out) {
     check_launch("k_query_fill");
     HIP_CHECK(hipMemcpyAsync(out, dout, (size_t)m, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int32_t mpm_is_occupied_noalloc(mpm_ctx* c, int64_t m, const float* points_dev, uint8_t* out_dev) {
+  if (!c) return MPM_ERR_VALUE;
+  return run(c, [&] {
+    require_idle(c);
+    if (m < 0 || m > INT32_MAX) fail(MPM_ERR_VALUE, "mpm_is_occupied_noalloc: query count out of range");
+    if (m == 0) return;
+    if (!points_dev || !out_dev) fail(MPM_ERR_VALUE, "mpm_is_occupied_noalloc: null query array");
+    if ((int64_t)c->n_nodes > MPM_NOALLOC_MAX_NODES)
+      fail(MPM_ERR_VALUE, "mpm_is_occupied_noalloc: the grid has " + std::to_string(c->n_nodes) +
+                              " nodes (bricks of 4^3), more than the " + std::to_string(MPM_NOALLOC_MAX_NODES) +
+                              " its fixed volume buffer holds; use mpm_is_occupied");
+    float* vol = noalloc_volume(c->device);
+    // the buffer keeps the previous call's volumes: clear the nodes this grid uses with a kernel (no memset
+    // node, so the sequence can sit in a captured graph)
+    k_zero_u32<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(reinterpret_cast<uint32_t*>(vol), c->n_nodes);
+    check_launch("k_zero_u32");
+    if (c->n) {
+      k_scatter_volume<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, (int)c->n, vol);
+      check_launch("k_scatter_volume");
+    }
+    k_query_fill<<<grid_for(m), kBlock, 0, c->stream>>>(c->geom, points_dev, (int)m, c->dom.cell_vol, vol, out_dev);
+    check_launch("k_query_fill");
   });
 }
 
