@@ -1400,12 +1400,15 @@ void batch_finish_scene(mpm_batch* b, int64_t i, mpm_advance_stats* out) {
   float v;
   std::memcpy(&v, &bits, 4);
   b->speed[i] = v;
-  b->speed_ok[i] = 1;
   const int64_t removed = remove_escaped(c);
   if (removed) {
     refresh_present(c);
     b->rec[i] = scene_record(b, c);
   }
+  // the last G2P's speed maximum covers the particles just removed too: after a removal the next frame's CFL
+  // step comes from the survivors (solver.py:176-185 reduces over the particles that remain), so batch_begin
+  // reduces it afresh
+  b->speed_ok[i] = removed ? 0 : 1;
   b->version[i] = c->version;
   *out = mpm_advance_stats{st.steps, b->dt[i], removed};
 }
