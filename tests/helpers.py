@@ -165,3 +165,16 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     b = np.asarray(b, np.float64)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def c_close(a_C: np.ndarray, b_C: np.ndarray, b_v: np.ndarray, dx: float, rel: float = 1e-4) -> tuple[bool, float]:
+    """APIC C against a reference: ||dC|| <= rel ||C|| + 1e-6 (4/dx) ||v||.
+
+    C = 4/dx^2 sum_i w_i v_i (x_i - x_p)^T is a difference of grid velocities; for a body moving nearly
+    rigidly it is rounding, and its size only means something against the velocity-gradient scale 4/dx |v|.
+    FP32 grid velocities put about 1e-7 of that scale into C, so the floor is 1e-6 of it; where C is large the
+    relative bar decides. Returns (ok, error as a fraction of the bound)."""
+    d = float(np.linalg.norm(np.asarray(a_C, np.float64) - np.asarray(b_C, np.float64)))
+    bound = rel * float(np.linalg.norm(np.asarray(b_C, np.float64))) + \
+        1e-6 * 4.0 / dx * float(np.linalg.norm(np.asarray(b_v, np.float64)))
+    return d <= bound, d / bound if bound > 0 else float(d > 0)

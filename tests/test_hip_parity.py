@@ -207,3 +207,37 @@ def test_open_side_removal_matches_oracle():
     a, b = hip.save_state(), ora.save_state()
     for k in ("x", "v", "F"):
         assert rel_err(a[k], b[k]) <= 1e-4, k
+
+
+@pytest.mark.parametrize("model", [0, 1, 2, 3])
+def test_inverted_particles_substep_matches_oracle(model):
+    """Particles with det F < 0 (SVD with sigma_3 < 0, the reflection case the return maps see:
+    kernels.py:252-263, and the log clamp of the Hencky strain, kernels.py:41-60) in one whole substep."""
+    rng = np.random.default_rng(40 + model)
+    state = seeded_state(20000, 64, 40 + model)
+    state["mat"][:] = model
+    n = len(state["x"])
+    flip = rng.random(n) < 0.15
+    s = np.stack([rng.uniform(0.9, 1.1, n), rng.uniform(0.9, 1.1, n), -rng.uniform(0.9, 1.1, n)], 1)
+    F = random_rotation(rng, n) @ (s[:, :, None] * np.swapaxes(random_rotation(rng, n), 1, 2))
+    state["F"][flip] = F[flip].astype(np.float32)
+    state["C"] *= np.float32(0.1)
+    assert (np.linalg.det(state["F"].astype(np.float64)) < 0).sum() == flip.sum() > 1000
+    hip, ora = pair(state, materials=MATERIALS)
+    dt = 1.0e-5
+    gv_h, gm_h = hip.debug_p2g(dt)
+    # The grid is held to the FP64 oracle: on inverted snow the FP32 oracle's own stress is 1.8e-5 from FP64
+    # (hardened stiffness on J < 0), past the grid bar, while the library takes those particles (J < 0.25)
+    # through its FP64 pass.
+    o64 = O.OracleSolver(Domain((0, 0, 0), (1, 1, 1), 64), MATERIALS, (0.0, 0.0, -9.81), precision="f64")
+    o64.load_state(state)
+    gv_o, gm_o = o64.p2g(dt)
+    assert rel_err(gm_h, gm_o) <= 1e-5 and rel_err(gv_h, gv_o) <= 1e-5
+    hip.load_state(state)
+    ora.load_state(state)
+    hip.substep(dt)
+    ora.substep(dt)
+    a, b = hip.save_state(), ora.save_state()
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert rel_err(a[k], b[k]) <= 1e-4, k
+        assert rel_err(a[k][flip], b[k][flip]) <= 1e-4, k
