@@ -5,9 +5,11 @@
 A step is one ``Solver.advance(dt)`` of the workload, which is exactly one substep (dt below the CFL
 step): collider pose upload, CFL speed reduction, P2G, grid update, collider contact, G2P, particle
 projection. Unless --steps is given, the timed steps run to the end of the workload's configured
-substeps (2000 for the multi-material mix, whose collider reaches the material after about 300), after
-the warm-up and the profile pass. P2G's mean launch time comes from a second run of the same substeps with
-only P2G bracketed. Inputs are resident in HBM before the timed region; the timed region is bracketed by
+substeps (2000 for the multi-material mix, whose collider reaches the sand column after about 580), after
+the warm-up and the profile pass. With --steps K there are two windows of K steps: right after the profile
+pass, and from substep 800 (collider in contact), the steps between them run untimed; the value is the
+particle-substeps of both windows over their time. P2G's mean launch time comes from a second run of the
+same substeps with only P2G bracketed. Inputs are resident in HBM before the timed region; the timed region is bracketed by
 device synchronisation. Per-kernel device time comes from HIP events recorded by the library on its
 own stream around every launch.
 
@@ -23,8 +25,9 @@ A third measurement (skipped with --no-serve) runs the bake server (mpm_core.ser
 requests arriving as a Poisson process at 20 per second, each drawn like the scene mix with a
 lognormal frame count (median 48) and 20-200 substeps per frame, admitted against a 250 GB HBM budget:
 once continuously batched and once as serial plain runs (one bake at a time, Solver.advance), with the
-same arrival times. Frames are downloaded to the host every frame as a bake streams them (no files are
-written in the measurement). Reported: per-frame latency p50/p99 and aggregate particle-substeps/s.
+same arrival times. Every bake streams its frames into a cache folder in the runner's format (frame files,
+checkpoint and progress every frame) on a scratch file system (tmpfs when it has room). Reported:
+per-frame latency p50/p99 and aggregate particle-substeps/s.
 
 A fourth measurement (skipped with --no-surface), taken right after the timed runs, meshes their final
 frame with the surfacer (mpm_surface.field.surface_particles, default settings: field cell = particle spacing dx/2,
@@ -34,10 +37,11 @@ marching cubes.
 
 The printed JSON line carries:
   roofline      for the kernel with the largest device time: algorithmic bytes per launch (bytes per
-                particle x particles + bytes per active grid node x active nodes) over its mean launch
-                time, against 8 TB/s HBM
+                particle x particles + bytes per active grid node x active nodes; P2G 88 + 16, G2P 148 + 12)
+                over its mean launch time, against 8 TB/s HBM
   cpu_baseline  the CPU oracle (a restatement of the reference Warp kernels in C++, OpenMP, FP32) timed
-                on a bounded number of substeps of the same workload on this host's cores; with it, the
+                on a bounded number of substeps of the same workload on every usable host core, the median
+                of three runs; with it, the
                 surfacing object gets cpu_splat, the oracle's numpy splat timed on a bounded sample of the
                 surfaced frame
 """
@@ -55,14 +59,17 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 PROFILE_STEPS = 32  # steps with every kernel class timed, before the timed steps (two sort intervals)
 
-# algorithmic bytes per particle and per active node, reference data flow (each array touched once)
-P2G_BYTES_PARTICLE = 104  # x12 v12 C36 F36 mat4 p_mass4
-P2G_BYTES_SNOW = 4  # Jp
+# Algorithmic bytes per particle and per active node (each array touched once). P2G: what it reads, x and v
+# and the 64-byte record {C, Kirchhoff stress, mass} in place of the reference's C, F, mat and p_mass (it
+# never touches F, mat or Jp), and the node's momentum and mass it produces. G2P: the reference's data flow
+# (kernels.py:183-263), which is less than this build moves (it also writes the stress and mass, 28 B more).
+P2G_BYTES_PARTICLE = 88  # read x12 v12 record64
+P2G_BYTES_SNOW = 0
 G2P_BYTES_PARTICLE = 148  # read x12 F36 mat4, write x12 v12 C36 F36
 G2P_BYTES_SNOW = 8  # Jp read + write
-P2G_BYTES_NODE = 16
-G2P_BYTES_NODE = 12
-GRID_BYTES_NODE = 28 + 16  # read accum 16, write velocity 12 (+ zeroing 16 folded into the update)
+P2G_BYTES_NODE = 16  # write momentum 12 + mass 4
+G2P_BYTES_NODE = 12  # read velocity
+CONTACT_SUBSTEP = 800  # the multi-material mix: the collider has been in the sand column for ~200 substeps
 
 
 CU_COUNT = 256
@@ -102,6 +109,9 @@ def parse_args(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-collider", action="store_true")
+    ap.add_argument("--p2g-mode", default="binned", choices=["binned", "atomic", "fixed"],
+                    help="P2G of the single-scene run: the library's default binned form, the reference's "
+                         "per-particle global atomics, or fixed point")
     ap.add_argument("--precision", default="fallback", choices=["fp32", "fallback", "fp64"],
                     help="SVD/return-map precision of the single-scene run (library default: fallback)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched scene-mix measurement")
@@ -128,36 +138,62 @@ def active_nodes(solver) -> int:
     return int((gm > 0).sum())
 
 
-def cpu_baseline(w, seconds: float) -> dict:
-    """Time the FP32 CPU oracle on the same workload for about ``seconds`` of work."""
-    from oracle import oracle as O
+def usable_cores() -> tuple[int, str]:
+    """Host cores this process may run on: its CPU affinity, capped by a cgroup CPU quota if one is set."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    note = f"{cores} in the affinity mask"
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            q = max(1, int(int(quota) // int(period)))
+            if q < cores:
+                cores, note = q, f"{note}, cgroup quota {q}"
+    except (OSError, ValueError):
+        pass
+    return cores, note
 
-    threads = min(os.cpu_count() or 1, 16)
-    ora = O.OracleSolver(w.domain, w.materials, w.gravity, threads=threads)
-    ora.add_particles(w.positions, w.velocities, w.material_ids, w.emitter_ids)
-    if w.colliders:
-        ora.set_colliders(w.colliders)
-    t_sim, steps, elapsed = 0.0, 0, 0.0
-    t0 = time.perf_counter()
-    while elapsed < seconds and steps < w.substeps:
-        if w.colliders:
-            ora.set_collider_transforms(*w.collider_transforms(t_sim, t_sim + w.dt))
-        ora.advance(w.dt)
-        t_sim += w.dt
-        steps += 1
-        elapsed = time.perf_counter() - t0
-    cpu = "unknown"
+
+def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": w.n * steps / elapsed, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} substeps of the same {w.n}-particle workload (FP32 C++ restatement of the "
-                      f"reference Warp kernels, OpenMP, {threads} threads on {cpu}), {elapsed:.1f} s"}
+    return "unknown"
+
+
+def cpu_baseline(w, seconds: float, runs: int = 3) -> dict:
+    """The FP32 CPU oracle on the same workload from its initial state, on every usable host core: the median
+    particle-substeps/s of ``runs`` runs of about ``seconds / runs`` each."""
+    from oracle import oracle as O
+
+    threads, note = usable_cores()
+    rates, counts = [], []
+    for _ in range(runs):
+        ora = O.OracleSolver(w.domain, w.materials, w.gravity, threads=threads)
+        ora.add_particles(w.positions, w.velocities, w.material_ids, w.emitter_ids)
+        if w.colliders:
+            ora.set_colliders(w.colliders)
+        t_sim, steps, elapsed = 0.0, 0, 0.0
+        t0 = time.perf_counter()
+        while elapsed < seconds / runs and steps < w.substeps:
+            if w.colliders:
+                ora.set_collider_transforms(*w.collider_transforms(t_sim, t_sim + w.dt))
+            ora.advance(w.dt)
+            t_sim += w.dt
+            steps += 1
+            elapsed = time.perf_counter() - t0
+        rates.append(w.n * steps / elapsed)
+        counts.append(steps)
+    median = float(np.median(rates))
+    return {"value": median, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "runs": [round(r, 1) for r in rates],
+            "sample": f"median of {runs} runs of {min(counts)}-{max(counts)} substeps each from the initial state of "
+                      f"the same {w.n}-particle workload (FP32 C++ restatement of the reference Warp kernels, "
+                      f"OpenMP, {threads} threads: {note}; {cpu_model()})"}
 
 
 def cpu_surface_baseline(particles, spacing: float, seconds: float = 5.0) -> dict:
@@ -265,33 +301,58 @@ def batch_bench(n_scenes: int, frames: int, tune: bool = True) -> dict:
     return result
 
 
+def cache_root() -> tuple[str, str]:
+    """A scratch folder for the served bakes' caches: tmpfs when it has room, else the system temp folder."""
+    import shutil
+    import tempfile
+
+    for base in ("/dev/shm", tempfile.gettempdir()):
+        try:
+            if os.path.isdir(base) and shutil.disk_usage(base).free > 24e9:
+                return tempfile.mkdtemp(prefix="mpm_served_", dir=base), base
+        except OSError:
+            continue
+    return tempfile.mkdtemp(prefix="mpm_served_"), tempfile.gettempdir()
+
+
 def serve_bench(n_requests: int, rate: float) -> dict:
-    """Poisson-arriving bake requests, continuously batched against serial plain runs. Throughput is
-    compared over the arrival window (first to last arrival), in which both servers always have work;
-    the batched server then drains its queue (latency over every frame), the serial one is stopped at the
-    end of the window (its backlog would take minutes)."""
+    """Poisson-arriving bake requests, continuously batched against serial plain runs. Each bake streams its
+    frames into its own cache folder in the runner's format (frame files per emitter, the checkpoint and
+    progress file every frame, mpm_core.cache) on a scratch file system, so a frame's latency includes its
+    download and file writes. Throughput is compared over the arrival window (first to last arrival), in
+    which both servers always have work; the batched server then drains its queue (latency over every
+    frame), the serial one is stopped at the end of the window (its backlog would take minutes)."""
+    import shutil
+
     from mpm_core.serve import BakeRequest, BakeServer
     from mpm_plugin_amd.workloads import served_requests
 
     reqs = served_requests(n_requests, rate)
     window = reqs[-1][0]
+    root, fs = cache_root()
     out = {"requests": n_requests, "rate_per_s": rate, "hbm_budget_gb": 250, "window_s": round(window, 3),
-           "frames": int(sum(s.frame_end - s.frame_start + 1 for _, s in reqs))}
-    for name, kw, stop in (("batched", dict(batched=True), None),
-                           ("serial", dict(batched=False, max_active=1), window)):
-        server = BakeServer(hbm_budget=250e9, **kw)
-        t0 = time.perf_counter()
-        handles = server.run([(off, BakeRequest(scene, name=f"r{i}")) for i, (off, scene) in enumerate(reqs)],
-                             stop_after=stop)
-        rep = server.report(handles)
-        in_window = server.work_between(t0, t0 + window)
-        rep["window_particle_substeps_per_s"] = in_window / window
-        server.close()
-        rep["wall_s"] = round(time.perf_counter() - t0, 3)
-        out[name] = rep
-        log(f"served ({name}): {in_window / window:.3e} particle-substeps/s over the arrival window; frame "
-            f"latency p50 {rep['frame_latency_ms']['p50']:.1f} ms p99 {rep['frame_latency_ms']['p99']:.1f} ms; "
-            f"{rep['done']}/{n_requests} bakes done in {rep['wall_s']} s")
+           "frames": int(sum(s.frame_end - s.frame_start + 1 for _, s in reqs)),
+           "cache_files": f"every frame and checkpoint written under {fs}"}
+    try:
+        for name, kw, stop in (("batched", dict(batched=True), None),
+                               ("serial", dict(batched=False, max_active=1), window)):
+            server = BakeServer(hbm_budget=250e9, **kw)
+            t0 = time.perf_counter()
+            handles = server.run([(off, BakeRequest(scene, name=f"r{i}",
+                                                    cache_dir=os.path.join(root, name, f"r{i}")))
+                                  for i, (off, scene) in enumerate(reqs)], stop_after=stop)
+            rep = server.report(handles)
+            in_window = server.work_between(t0, t0 + window)
+            rep["window_particle_substeps_per_s"] = in_window / window
+            server.close()
+            rep["wall_s"] = round(time.perf_counter() - t0, 3)
+            out[name] = rep
+            shutil.rmtree(os.path.join(root, name), ignore_errors=True)
+            log(f"served ({name}): {in_window / window:.3e} particle-substeps/s over the arrival window; frame "
+                f"latency p50 {rep['frame_latency_ms']['p50']:.1f} ms p99 {rep['frame_latency_ms']['p99']:.1f} ms; "
+                f"{rep['done']}/{n_requests} bakes done in {rep['wall_s']} s")
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
     out["throughput_speedup"] = (out["batched"]["window_particle_substeps_per_s"]
                                  / out["serial"]["window_particle_substeps_per_s"])
     return out
@@ -349,6 +410,8 @@ def main(argv=None) -> int:
         solver = w.make_solver(device="cuda:0")
         solver.set_option(N.OPT_PRECISION, {"fp32": N.PRECISION_FP32, "fallback": N.PRECISION_FALLBACK,
                                             "fp64": N.PRECISION_FP64}[args.precision])
+        solver.set_option(N.OPT_P2G_MODE, {"binned": N.P2G_BINNED, "atomic": N.P2G_ATOMIC,
+                                           "fixed": N.P2G_FIXED}[args.p2g_mode])
         t_sim = 0.0
 
         def step():
@@ -362,54 +425,70 @@ def main(argv=None) -> int:
 
         return solver, step
 
-    solver, step = make_run()
+    first = args.warmup + PROFILE_STEPS
     if args.steps is None:
-        args.steps = max(1, w.substeps - args.warmup - PROFILE_STEPS)
-    for _ in range(args.warmup):
-        step()
-    solver.synchronize()
-    n_active = active_nodes(solver)
-    n_snow = int(sum((w.material_ids == i).sum() for i, m in enumerate(w.materials) if m.model == "snow"))
-    log(f"warmup done ({args.warmup} steps); active nodes {n_active}")
+        windows = [(first, w.substeps)]  # the rest of the configured run, every phase of it
+    else:
+        windows = [(first, first + args.steps)]
+        if w.colliders and CONTACT_SUBSTEP >= first + args.steps and CONTACT_SUBSTEP + args.steps <= w.substeps:
+            windows.append((CONTACT_SUBSTEP, CONTACT_SUBSTEP + args.steps))  # in collider contact as well
 
-    # per-class kernel times over two sort intervals, every class bracketed with events
-    solver.reset_timing()
-    solver.set_timing(True)
-    for _ in range(PROFILE_STEPS):
-        step()
-    solver.set_timing(False)
-    times = solver.kernel_times()
-    log(f"profile pass ({PROFILE_STEPS} steps), kernel ms: "
-        + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in times.items()))
+    def timed_run(kernel: str, profile: bool):
+        """A fresh run: warm-up, [profile pass], then the windows (untimed steps between them) with only
+        `kernel` bracketed by the library's events. Returns (solver, seconds per window, kernel time, profile)."""
+        solver, step = make_run()
+        for _ in range(args.warmup):
+            step()
+        solver.synchronize()
+        prof = None
+        if profile:
+            # per-class kernel times over two sort intervals, every class bracketed with events
+            solver.reset_timing()
+            solver.set_timing(True)
+            for _ in range(PROFILE_STEPS):
+                step()
+            solver.set_timing(False)
+            prof = solver.kernel_times()
+        else:
+            for _ in range(PROFILE_STEPS):
+                step()
+        k = first
+        secs = []
+        nodes = []
+        solver.reset_timing()
+        for a, b in windows:
+            for _ in range(k, a):
+                step()
+            solver.set_timing(True, phases=[kernel])
+            solver.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(a, b):
+                step()
+            solver.synchronize()
+            secs.append(time.perf_counter() - t0)
+            solver.set_timing(False)
+            nodes.append(active_nodes(solver))
+            k = b
+        return solver, secs, solver.kernel_times()[kernel], prof, nodes
 
     # the timed steps bracket only G2P (every bracketed kernel adds its events' cost to the step)
-    timed = {}
-    solver.reset_timing()
-    solver.set_timing(True, phases=["g2p"])
-    solver.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    solver.synchronize()
-    elapsed = time.perf_counter() - t0
-    solver.set_timing(False)
-    timed["g2p"] = solver.kernel_times()["g2p"]
+    solver, secs, g2p_time, times, nodes = timed_run("g2p", profile=True)
+    # grid nodes with mass at the end of each window, weighted by the window's steps (the mean per launch)
+    n_active = int(round(sum(m * (b - a) for m, (a, b) in zip(nodes, windows)) / sum(b - a for a, b in windows)))
+    n_snow = int(sum((w.material_ids == i).sum() for i, m in enumerate(w.materials) if m.model == "snow"))
+    log(f"profile pass ({PROFILE_STEPS} steps), kernel ms: "
+        + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in times.items()))
+    timed = {"g2p": g2p_time}
+    elapsed = sum(secs)
+    total_steps = sum(b - a for a, b in windows)
     n = solver.particle_count
     solver.close()
-
-    # P2G's launch time over the same substeps, from a second run of the workload with only P2G bracketed
-    solver, step = make_run()
-    for _ in range(args.warmup + PROFILE_STEPS):
-        step()
-    solver.reset_timing()
-    solver.set_timing(True, phases=["p2g"])
-    for _ in range(args.steps):
-        step()
-    solver.set_timing(False)
-    timed["p2g"] = solver.kernel_times()["p2g"]
+    # P2G's launch time over the same substeps, from a second run with only P2G bracketed
+    solver, _, timed["p2g"], _, _ = timed_run("p2g", profile=False)
     final = solver.get_particles() if not args.no_surface else None
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; g2p {timed['g2p'][0]:.2f} ms / {timed['g2p'][1]} launches "
-        f"(p2g {timed['p2g'][0]:.2f} ms / {timed['p2g'][1]} launches over the same substeps of a second run)")
+    log(f"timed windows {windows}: " + ", ".join(f"{s:.3f} s" for s in secs)
+        + f"; g2p {timed['g2p'][0]:.2f} ms / {timed['g2p'][1]} launches (p2g {timed['p2g'][0]:.2f} ms / "
+          f"{timed['p2g'][1]} launches over the same substeps of a second run); active nodes {n_active}")
     bytes_per_launch = {
         "p2g": P2G_BYTES_PARTICLE * n + P2G_BYTES_SNOW * n_snow + P2G_BYTES_NODE * n_active,
         "g2p": G2P_BYTES_PARTICLE * n + G2P_BYTES_SNOW * n_snow + G2P_BYTES_NODE * n_active,
@@ -420,7 +499,7 @@ def main(argv=None) -> int:
         ms, launches = timed[kernel]
         mean_s = ms / 1000.0 / max(launches, 1)
         achieved = bytes_per_launch[kernel] / mean_s / 1e9
-        lib_name = {"p2g": "k_p2g_split", "g2p": "k_g2p"}[kernel]
+        lib_name = {"p2g": "k_p2g_split" if args.p2g_mode == "binned" else "k_p2g_atomic", "g2p": "k_g2p"}[kernel]
         out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc.get(("traffic", lib_name)),
                "kernel": kernel, "mean_launch_us": round(mean_s * 1e6, 2),
@@ -462,15 +541,15 @@ def main(argv=None) -> int:
     if not args.no_serve:
         served = serve_bench(args.serve_requests, args.serve_rate)
 
-    value = n * args.steps / elapsed
+    value = n * total_steps / elapsed
     line = {
         "metric": "particle-substeps/s at 1M pts/128^3",
         "value": value,
         "unit": "particle-substeps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": total_steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1000.0 / args.steps,
+        "ms_per_step": elapsed * 1000.0 / total_steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -478,8 +557,10 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": w.name, "particles": n, "grid": list(w.domain.grid_shape), "dt": w.dt,
                    "materials": [m.model for m in w.materials], "colliders": len(w.colliders),
-                   "substeps_per_step": 1, "svd_precision": args.precision,
-                   "timed_substeps": [args.warmup + PROFILE_STEPS, args.warmup + PROFILE_STEPS + args.steps]},
+                   "substeps_per_step": 1, "svd_precision": args.precision, "p2g_mode": args.p2g_mode,
+                   "timed_substeps": [list(ab) for ab in windows],
+                   "window_values": [round(n * (b - a) / t, 1) for (a, b), t in zip(windows, secs)],
+                   "active_nodes": n_active},
         "roofline": roofline,
         "cpu_baseline": baseline,
         "kernels": kernels,
