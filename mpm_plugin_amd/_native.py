@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libmpm_hip.so")
 ABI_VERSION = 1
 OK, ERR_VALUE, ERR_NOT_IMPLEMENTED, ERR_MEMORY, ERR_RUNTIME = 0, -1, -2, -3, -4
 K_P2G, K_GRID, K_G2P, K_BIN, K_COLLIDERS, K_OTHER = range(6)
-OPT_P2G_MODE, OPT_SORT_INTERVAL, OPT_P2G_BLOCKS, OPT_PRECISION, OPT_P2G_WAVES = range(5)
+OPT_P2G_MODE, OPT_SORT_INTERVAL, OPT_P2G_BLOCKS, OPT_PRECISION, OPT_P2G_WAVES, OPT_P2G_FLUSH = range(6)
+P2G_FLUSH_STAGED, P2G_FLUSH_ATOMIC = 0, 1
 P2G_BINNED, P2G_ATOMIC, P2G_FIXED = 0, 1, 2
 PRECISION_FP32, PRECISION_FALLBACK, PRECISION_FP64 = 0, 1, 2
 (BATCH_OPT_P2G_BLOCKS, BATCH_OPT_GRAPH, BATCH_OPT_SORT_INTERVAL, BATCH_OPT_SUBSTEPS_LAUNCHED, BATCH_OPT_G2P_BLOCKS,

@@ -193,6 +193,8 @@ struct mpm_ctx {
   int64_t sort_interval = 24;
   int64_t p2g_blocks = 0;
   int p2g_waves = 2;             // waves per occupied brick in the binned P2G (1 or 2)
+  int p2g_flush = 0;             // MPM_P2G_FLUSH_STAGED or MPM_P2G_FLUSH_ATOMIC
+  float4* flush_stage() const { return p2g_flush == MPM_P2G_FLUSH_STAGED ? stage : nullptr; }
 
   // colliders
   std::vector<ColliderHost> colliders;
@@ -632,17 +634,17 @@ void run_p2g(mpm_ctx* c, float dt, const DriftLists* pre = nullptr) {
   if (c->p2g_waves == 4) {
     k_p2g_split<4><<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
                                                             c->active, c->n_active, c->accum, c->dirty, c->fixed(),
-                                                            c->stage, dl);
+                                                            c->flush_stage(), dl);
     check_launch("k_p2g_split");
   } else if (c->p2g_waves == 2) {
     k_p2g_split<2><<<(unsigned)blocks, 128, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
                                                             c->active, c->n_active, c->accum, c->dirty, c->fixed(),
-                                                            c->stage, dl);
+                                                            c->flush_stage(), dl);
     check_launch("k_p2g_split");
   } else {
     k_p2g_cells<1><<<(unsigned)blocks, 64, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
                                                          c->active, c->n_active, c->accum, c->dirty, c->fixed(),
-                                                         c->stage, dl);
+                                                         c->flush_stage(), dl);
     check_launch("k_p2g_cells");
   }
 }
@@ -668,7 +670,7 @@ void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
     const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
     k_grid_bricks<<<blocks, 256, 0, c->stream>>>(c->geom, c->sc, dt, c->n_bricks, c->dirty, c->live, c->accum,
                                                  c->vel, c->collider_set(), frames_at(c, s, inv_frame_dt),
-                                                 c->fixed(), c->stage, c->slot, c->n_fix);
+                                                 c->fixed(), c->flush_stage(), c->slot, c->n_fix);
     check_launch("k_grid_bricks");
   }
   c->n_fix_cleared = true;
@@ -2141,7 +2143,7 @@ int32_t mpm_get_grid(mpm_ctx* c, int32_t which, float* grid_v, float* grid_m) {
         k_fixed_to_float<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->fx, c->n_nodes, 1.0 / c->fx_scale, src);
         check_launch("k_fixed_to_float");
       } else {
-        k_p2g_resolve<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->accum, c->n_nodes, c->stage,
+        k_p2g_resolve<<<grid_for(c->n_nodes), kBlock, 0, c->stream>>>(c->geom, c->accum, c->n_nodes, c->flush_stage(),
                                                                      c->slot, src);
         check_launch("k_p2g_resolve");
       }
@@ -2249,6 +2251,10 @@ int32_t mpm_set_option(mpm_ctx* c, int32_t option, int64_t value) {
         if (value < 0) fail(MPM_ERR_VALUE, "block count must be >= 0");
         c->p2g_blocks = value;
         break;
+      case MPM_OPT_P2G_FLUSH:
+        if (value != MPM_P2G_FLUSH_STAGED && value != MPM_P2G_FLUSH_ATOMIC) fail(MPM_ERR_VALUE, "p2g flush must be 0 or 1");
+        c->p2g_flush = (int)value;
+        break;
       case MPM_OPT_P2G_WAVES:
         if (value != 1 && value != 2 && value != 4) fail(MPM_ERR_VALUE, "waves per brick must be 1, 2 or 4");
         c->p2g_waves = (int)value;
@@ -2267,6 +2273,7 @@ int64_t mpm_get_option(const mpm_ctx* c, int32_t option) {
     case MPM_OPT_SORT_INTERVAL: return c->sort_interval;
     case MPM_OPT_P2G_BLOCKS: return c->p2g_blocks;
     case MPM_OPT_P2G_WAVES: return c->p2g_waves;
+    case MPM_OPT_P2G_FLUSH: return c->p2g_flush;
     default: return -1;
   }
 }

@@ -56,6 +56,11 @@ template <> __device__ __forceinline__ double sqrt_a<double>(double v) { return 
 template <class T> __device__ __forceinline__ T rsqrt_a(T v);
 template <> __device__ __forceinline__ float rsqrt_a<float>(float v) { return __builtin_amdgcn_rsqf(v); }
 template <> __device__ __forceinline__ double rsqrt_a<double>(double v) { return 1.0 / __builtin_sqrt(v); }
+// Quotients of the return maps: in FP32 a times the hardware reciprocal of b (about 1.5 ulp; the IEEE
+// division sequence costs ten instructions), in FP64 the exact quotient.
+template <class T> __device__ __forceinline__ T quot(T a, T b);
+template <> __device__ __forceinline__ float quot<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <> __device__ __forceinline__ double quot<double>(double a, double b) { return a / b; }
 template <class T> __device__ __forceinline__ T abs_t(T v) { return v < T(0) ? -v : v; }
 template <class T> __device__ __forceinline__ T max_t(T a, T b) { return a > b ? a : b; }
 template <class T> __device__ __forceinline__ T min_t(T a, T b) { return a < b ? a : b; }
@@ -240,12 +245,12 @@ __device__ __forceinline__ M3<T> hencky_tau(const M3<T>& U, const T (&sig)[3], T
 template <class T>
 __device__ __forceinline__ void project_von_mises(const T (&sig)[3], T mu, T yield_stress, T (&out)[3]) {
   const T e[3] = {safe_log(sig[0]), safe_log(sig[1]), safe_log(sig[2])};
-  const T tr3 = (e[0] + e[1] + e[2]) / T(3);
+  const T tr3 = (e[0] + e[1] + e[2]) * T(1.0 / 3.0);
   const T d[3] = {e[0] - tr3, e[1] - tr3, e[2] - tr3};
   const T n = sqrt_t(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-  const T limit = yield_stress / (T(2) * mu);
+  const T limit = quot(yield_stress, T(2) * mu);
   if (n > limit) {
-    const T k = limit / n;
+    const T k = quot(limit, n);
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[i] = exp_t(tr3 + d[i] * k);
   } else {
@@ -265,12 +270,13 @@ __device__ __forceinline__ void project_drucker_prager(const T (&sig)[3], T mu, 
     out[0] = out[1] = out[2] = T(1);  // separation: stress free
     return;
   }
-  const T d[3] = {e[0] - tr / T(3), e[1] - tr / T(3), e[2] - tr / T(3)};
+  const T tr3 = tr * T(1.0 / 3.0);
+  const T d[3] = {e[0] - tr3, e[1] - tr3, e[2] - tr3};
   const T n = sqrt_t(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   if (n < T(1.0e-10)) return;
-  const T dgamma = n + (T(3) * lam + T(2) * mu) / (T(2) * mu) * tr * alpha;
+  const T dgamma = n + quot(T(3) * lam + T(2) * mu, T(2) * mu) * tr * alpha;
   if (dgamma <= T(0)) return;
-  const T k = dgamma / n;
+  const T k = quot(dgamma, n);
 #pragma unroll
   for (int i = 0; i < 3; ++i) out[i] = exp_t(e[i] - d[i] * k);
 }

@@ -562,7 +562,7 @@ __device__ __forceinline__ void material_update(const MatDev& m, const M3<T>& Fn
       const T lo = T(1) - T(m.crit_c), hi = T(1) + T(m.crit_s);
 #pragma unroll
       for (int a = 0; a < 3; ++a) ns[a] = min_t(max_t(sig[a], lo), hi);
-      jp = T(P.Jp[p]) * (sig[0] * sig[1] * sig[2]) / (ns[0] * ns[1] * ns[2]);
+      jp = quot(T(P.Jp[p]) * (sig[0] * sig[1] * sig[2]), ns[0] * ns[1] * ns[2]);
       P.Jp[p] = (float)jp;
     }
     const M3<T> Fe = usv(U, ns, V);
@@ -634,34 +634,54 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
     for (int i = 0; i < 3; ++i) wx[a][i] = bspline_w(fx[a], i);
   // v = sum w v_n and C = 4/dx^2 sum w v_n (x_n - x)^T (kernels.py:208-224). With o_b in {-1, 0, 1} the
   // stencil offset of the node along axis b less one, (x_n - x)_b = (o_b - (fx_b - 1)) dx, so
-  // C_ab = 4/dx (M_ab - (fx_b - 1) v_a) with M_ab = sum w v_n,a o_b: a node adds into M only along the
-  // axes where its offset is not the middle one, and only by addition or subtraction.
-  float nv[3] = {0.0f, 0.0f, 0.0f};
-  float M[3][3];
+  // C_ab = 4/dx (M_ab - (fx_b - 1) v_a) with M_ab = sum w v_n,a o_b. The weight is a product over the axes,
+  // so the sums are taken one axis at a time: along k, s = sum_k wz_k v and t = sum_k wz_k o_k v for each
+  // (i, j); along j, S = sum_j wy_j s, T = sum_j wy_j t and U = sum_j wy_j o_j s for each i; along i, v,
+  // M_.2 = sum_i wx_i T, M_.1 = sum_i wx_i U and M_.0 = sum_i wx_i o_i S (267 flops, not 27 x 13).
+  // Node addresses split per axis in the brick-major layout: addr = ax[i] + ay[j] + az[k].
+  int ad[3][3];
+  {
+    const int sx = g.bdims[1] * g.bdims[2] * 64, sy = g.bdims[2] * 64;
 #pragma unroll
-  for (int e = 0; e < 9; ++e) M[e / 3][e % 3] = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+      const int X = base[0] + i, Y = base[1] + i, Z = base[2] + i;
+      ad[0][i] = (X >> 2) * sx + ((X & 3) << 4);
+      ad[1][i] = (Y >> 2) * sy + ((Y & 3) << 2);
+      ad[2][i] = (Z >> 2) * 64 + (Z & 3);
+    }
+  }
+  float Si[3][3], Ti[3][3], Ui[3][3];  // [i][component]
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
+    float sj[3][3], tj[3][3];  // [j][component]
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const float wij = wx[0][i] * wx[1][j];
+      const float4* row = vel + (ad[0][i] + ad[1][j]);
+      const float4 g0 = row[ad[2][0]], g1 = row[ad[2][1]], g2 = row[ad[2][2]];
+      const float v0[3] = {g0.x, g0.y, g0.z}, v1[3] = {g1.x, g1.y, g1.z}, v2[3] = {g2.x, g2.y, g2.z};
 #pragma unroll
-      for (int kk = 0; kk < 3; ++kk) {
-        const float w = wij * wx[2][kk];
-        const float4 gv4 = vel[node_addr(g, base[0] + i, base[1] + j, base[2] + kk)];
-        const float wg[3] = {w * gv4.x, w * gv4.y, w * gv4.z};
-        const int o[3] = {i - 1, j - 1, kk - 1};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          nv[a] += wg[a];
-#pragma unroll
-          for (int b = 0; b < 3; ++b) {
-            if (o[b] > 0) M[a][b] += wg[a];
-            if (o[b] < 0) M[a][b] -= wg[a];
-          }
-        }
+      for (int a = 0; a < 3; ++a) {
+        const float p0 = wx[2][0] * v0[a], p2 = wx[2][2] * v2[a];
+        sj[j][a] = (p0 + wx[2][1] * v1[a]) + p2;
+        tj[j][a] = p2 - p0;
       }
     }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float s0 = wx[1][0] * sj[0][a], s2 = wx[1][2] * sj[2][a];
+      Si[i][a] = (s0 + wx[1][1] * sj[1][a]) + s2;
+      Ti[i][a] = wx[1][0] * tj[0][a] + wx[1][1] * tj[1][a] + wx[1][2] * tj[2][a];
+      Ui[i][a] = s2 - s0;
+    }
+  }
+  float nv[3], M[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float S0 = wx[0][0] * Si[0][a], S2 = wx[0][2] * Si[2][a];
+    nv[a] = (S0 + wx[0][1] * Si[1][a]) + S2;
+    M[a][0] = S2 - S0;
+    M[a][1] = wx[0][0] * Ui[0][a] + wx[0][1] * Ui[1][a] + wx[0][2] * Ui[2][a];
+    M[a][2] = wx[0][0] * Ti[0][a] + wx[0][1] * Ti[1][a] + wx[0][2] * Ti[2][a];
   }
   M3<float> nC;
   const float c4dx = 4.0f * g.inv_dx;

@@ -874,6 +874,28 @@ void oracle_collider_velocity_f64(const OracleCollider* c, double s, const doubl
   collider_velocity<double>(*c, M, s, p, inv_frame_dt, out);
 }
 
+// the smallest signed distance of n points (float32 rows) to the colliders at frame fraction s, as
+// project_particles evaluates it (colliders.py:137-140; 1e30 beyond every collider's reach), for tests that
+// must tell particles lying on a surface, where rounding decides the projection, from the others
+void oracle_collider_phi_f32(const OracleDomain* d, const OracleCollider* cs, int32_t n_col, int64_t n,
+                             const float* x, double s, float* phi, int32_t threads) {
+  std::vector<MeshNormals> mns;
+  for (int c = 0; c < n_col; ++c) mns.push_back(mesh_normals(cs[c]));
+  const float thickness = 2.0f * d->dx;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 256)
+  for (int64_t p = 0; p < n; ++p) {
+    const float pos[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+    float best = 1.0e30f;
+    for (int c = 0; c < n_col; ++c) {
+      const Affine<float> M = collider_matrix<float>(cs[c], float(s));
+      float nrm[3], ph;
+      collider_sdf<float>(cs[c], mns[c], M, pos, thickness, nrm, ph);
+      best = std::min(best, ph);
+    }
+    phi[p] = best;
+  }
+}
+
 // full substep: zero, p2g, grid_update, [grid_colliders], g2p, [project_particles] (solver.py:187-211)
 void oracle_substep_f32(const OracleDomain* d, const OracleMaterial* m, const OracleCollider* cs, int32_t n_col,
                         int64_t n, float* x, float* v, float* C, float* F, float* Jp, const int32_t* mat, double dt,

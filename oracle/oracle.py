@@ -78,6 +78,7 @@ def lib() -> C.CDLL:
             "oracle_project_particles_f32": ([D, K, i32, i64, fp, fp, f64, f64, i32], None),
             "oracle_collider_sdf_f64": ([K, f64, dp, f64, dp, dp], None),
             "oracle_collider_velocity_f64": ([K, f64, dp, f64, dp], None),
+            "oracle_collider_phi_f32": ([D, K, i32, i64, fp, f64, fp, i32], None),
             "oracle_substep_f32": ([D, M, K, i32, i64, fp, fp, fp, fp, fp, ip, f64, f64, f64, fp, fp, i32], None),
             "oracle_substep_f64": ([D, M, K, i32, i64, fp, fp, fp, fp, fp, ip, f64, f64, f64, dp, dp, i32], None),
             "oracle_max_speed": ([i64, fp], C.c_float),
@@ -381,6 +382,15 @@ class OracleSolver:
            self.threads)
         if len(self.colliders):
             self.project_particles(s_frac)
+
+    def collider_phi(self, x, s_frac: float = 1.0) -> np.ndarray:
+        """Smallest signed distance of each point to the colliders at frame fraction s_frac."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        out = np.full(len(x), 1e30, np.float32)
+        if len(x) and len(self.colliders):
+            lib().oracle_collider_phi_f32(C.byref(self.d), self.colliders.arr, len(self.colliders), len(x),
+                                          _ptr(x, C.c_float), s_frac, _ptr(out, C.c_float), self.threads)
+        return out
 
     def project_particles(self, s_frac: float = 1.0) -> None:
         """project_particles alone (colliders.py:120-149) on the current state."""

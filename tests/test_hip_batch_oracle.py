@@ -4,7 +4,8 @@
   (32 to 128) and 0 to 2 colliders. From identical states (each scene first runs a frame alone so that C and
   F are not trivial; the oracle then loads that state), one substep of every scene in one batched launch
   matches the oracle's substep at the single-substep bar (x, v, F, Jp rel <= 1e-4; C rel <= 1e-4 with the
-  floor of helpers.c_close for bodies that move nearly rigidly), and a whole frame's
+  floor of helpers.c_close for bodies that move nearly rigidly; v of particles lying on a collider surface,
+  at most 2%, is left out: rounding decides whether they are projected), and a whole frame's
   CFL substep count and dt equal the oracle's exactly (solver.py:176-185, 213-220); the frame itself is then
   held to the multi-step criterion.
 * Open sides: every frame of a batched scene losing particles takes the oracle's substep count for the state
@@ -98,9 +99,19 @@ def test_batched_substep_matches_oracle(warmed):
         assert st.substeps == 1 and st.dt == dt
         assert o.advance(dt)[0] == 1
         a, b = s.save_state(), o.save_state()
-        errs = {k: rel_err(a[k], b[k]) for k in ("x", "v", "F", "Jp")}
+        # A particle that ends on a collider's surface (projected there in an earlier substep) is projected
+        # again or not by the sign of a rounding-sized distance (colliders.py:140-147): its velocity is compared
+        # with the others' only when it lies clear of every surface on both sides.
+        clear = np.ones(len(b["x"]), bool)
+        if w.colliders:
+            tol = 1e-4 * w.domain.dx
+            clear = (np.abs(o.collider_phi(b["x"])) > tol) & (np.abs(o.collider_phi(a["x"])) > tol)
+        assert clear.mean() > 0.98
+        errs = {k: rel_err(a[k], b[k]) for k in ("x", "F", "Jp")}
+        errs["v"] = rel_err(a["v"][clear], b["v"][clear])
         c_ok, c_frac = c_close(a["C"], b["C"], b["v"], w.domain.dx)
-        print(w.name, {k: f"{e:.1e}" for k, e in errs.items()}, f"C {c_frac:.1e} of its bound", flush=True)
+        print(w.name, {k: f"{e:.1e}" for k, e in errs.items()}, f"C {c_frac:.1e} of its bound,",
+              f"{int((~clear).sum())} particles on a collider surface", flush=True)
         for k, e in errs.items():
             assert e <= 1e-4, (w.name, k)
         assert c_ok, (w.name, "C")

@@ -115,3 +115,24 @@ def test_waves_per_brick_match_one(mode, split):
     for k in ("x", "v", "F"):
         assert rel(states[split][k], states[1][k]) <= 1e-5, k
     assert rel(states[split]["C"], states[1]["C"]) <= 1e-4
+
+
+def test_atomic_flush_matches_staged_windows():
+    """Brick windows added into the grid with global atomics (option P2G_FLUSH) against the staged windows
+    the grid update sums, on the drifted state: the grids differ only in the order of float additions."""
+    st = moving_state()
+    out = {}
+    for flush in (N.P2G_FLUSH_STAGED, N.P2G_FLUSH_ATOMIC):
+        s = gpu_solver(ELASTIC, N.PRECISION_FALLBACK, N.P2G_BINNED)
+        s.set_option(N.OPT_SORT_INTERVAL, 8)
+        s.set_option(N.OPT_P2G_FLUSH, flush)
+        assert s.get_option(N.OPT_P2G_FLUSH) == flush
+        s.load_state(st)
+        for _ in range(PRIOR + 1):
+            s.substep(DT, 1.0, DT)
+        out[flush] = (s.save_state(), s.get_grid(1), s.debug_p2g(DT))
+    (a, (va, ma), (pa, qa)), (b, (vb, mb), (pb, qb)) = out[N.P2G_FLUSH_STAGED], out[N.P2G_FLUSH_ATOMIC]
+    assert rel(mb, ma) <= 1e-6 and rel(vb, va) <= 1e-6
+    assert rel(qb, qa) <= 1e-6 and rel(pb, pa) <= 1e-6  # the P2G download resolves either form
+    for k in ("x", "v", "F"):
+        assert rel(b[k], a[k]) <= 1e-5, k
