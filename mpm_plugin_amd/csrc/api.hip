@@ -2135,7 +2135,9 @@ int32_t mpm_get_grid(mpm_ctx* c, int32_t which, float* grid_v, float* grid_m) {
     require_idle(c);
     const int64_t total = (int64_t)c->geom.dims[0] * c->geom.dims[1] * c->geom.dims[2];
     float4* src = which == 0 ? c->accum : c->vel;
-    const bool resolve = which == 0 && c->p2g_mode != MPM_P2G_ATOMIC;
+    // the P2G output is the accumulator alone after the reference-order P2G or an atomic window flush
+    const bool resolve = which == 0 && c->p2g_mode != MPM_P2G_ATOMIC &&
+                         (c->p2g_mode == MPM_P2G_FIXED || c->flush_stage() != nullptr);
     if (resolve) {
       // the P2G output lives in the fixed-point accumulator, or in the accumulator plus the staged windows
       src = static_cast<float4*>(c->scratch.get((size_t)c->n_nodes * sizeof(float4) + total * 4 * sizeof(float)));

@@ -450,6 +450,10 @@ __global__ void __launch_bounds__(256) k_p2g_resolve(GridGeom g, const float4* _
                                                      const int32_t* __restrict__ slot, float4* __restrict__ out) {
   const int addr = blockIdx.x * blockDim.x + threadIdx.x;
   if (addr >= n_nodes) return;
+  if (!stage) {  // windows flushed with atomics: the accumulator holds everything
+    out[addr] = accum[addr];
+    return;
+  }
   int i, j, k;
   node_ijk(g, addr, i, j, k);
   out[addr] = staged_sum(g, i, j, k, accum[addr], stage, slot);
