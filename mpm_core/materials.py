@@ -1,13 +1,13 @@
-"""Material parameters and presets.
+"""Materials and their presets.
 
-Interface follows the reference's ``mpm_core.materials`` (src/mpm_core/materials.py:1-158):
-``MaterialParams`` with the same fields, validation and ``lame()``; presets by id with
-``list_presets`` / ``load_preset`` / ``save_user_preset`` / ``delete_user_preset``.
+The interface is the reference's ``mpm_core.materials`` (src/mpm_core/materials.py:24-158): ``MaterialParams``
+with the same fields, defaults, validation messages and ``lame()``; a ``Look`` and a ``Preset``; presets by id
+through ``list_presets`` / ``load_preset`` / ``user_preset_id`` / ``save_user_preset`` / ``delete_user_preset``.
 
-Built-in presets live in one table, ``presets/builtin.toml``. User presets are single TOML
-files (``label``, optional ``description``, ``[material]``, ``[look]``) in a folder the caller
-names, with ids ``user:<file stem>``; that is the file format the Blender add-on writes, so it is
-kept as the reference defines it.
+Built-in presets are one table, ``presets/builtin.toml`` (one section per preset id). A user preset is a TOML
+file in a folder the caller names: top-level ``label`` and optional ``description``, then ``[material]`` and
+``[look]`` sections; its id is ``user:<file stem>``. The Blender add-on writes and reads that file format, so
+it is kept as the reference defines it.
 """
 
 from __future__ import annotations
@@ -28,10 +28,18 @@ DISPLAYS = ("mesh", "points")
 USER_PREFIX = "user:"
 _BUILTIN_FILE = Path(__file__).with_name("presets") / "builtin.toml"
 
+# (condition a valid material meets, message when it does not), checked after the model name
+_RULES = (
+    (lambda m: m.youngs_modulus > 0, "youngs_modulus must be > 0"),
+    (lambda m: 0 <= m.poisson_ratio < 0.5, "poisson_ratio must be in [0, 0.5)"),
+    (lambda m: m.density > 0, "density must be > 0"),
+)
+
 
 @dataclass(frozen=True)
 class MaterialParams:
-    """One material (src/mpm_core/materials.py:24-59)."""
+    """One material: the constitutive model and its parameters (materials.py:24-59). ``yield_stress`` is von
+    Mises's, ``friction_angle_deg`` Drucker-Prager's, the last four snow's."""
 
     name: str
     model: str
@@ -48,27 +56,24 @@ class MaterialParams:
     def __post_init__(self):
         if self.model not in MODELS:
             raise ValueError(f"unknown material model {self.model!r}, expected one of {MODELS}")
-        if not self.youngs_modulus > 0:
-            raise ValueError("youngs_modulus must be > 0")
-        if not 0 <= self.poisson_ratio < 0.5:
-            raise ValueError("poisson_ratio must be in [0, 0.5)")
-        if not self.density > 0:
-            raise ValueError("density must be > 0")
+        for holds, message in _RULES:
+            if not holds(self):
+                raise ValueError(message)
 
     def lame(self) -> tuple[float, float]:
-        """Lamé parameters (mu, lam) in double precision (materials.py:48-51)."""
-        e, nu = self.youngs_modulus, self.poisson_ratio
-        mu = e / (2.0 * (1.0 + nu))
-        lam = e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
-        return mu, lam
+        """(mu, lambda) from Young's modulus E and Poisson's ratio nu, in double precision:
+        mu = E / (2 (1 + nu)), lambda = E nu / ((1 + nu)(1 - 2 nu))."""
+        e, nu = float(self.youngs_modulus), float(self.poisson_ratio)
+        return e / (2.0 * (1.0 + nu)), e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
 
     def to_dict(self) -> dict:
         return asdict(self)
 
     @classmethod
     def from_dict(cls, d: dict) -> MaterialParams:
-        names = {f.name for f in fields(cls)}
-        return cls(**{k: v for k, v in d.items() if k in names})
+        """Build from a dict, ignoring keys that are not fields (documents may carry extra ones)."""
+        known = {f.name for f in fields(cls)}
+        return cls(**{k: v for k, v in d.items() if k in known})
 
 
 @dataclass(frozen=True)
@@ -96,48 +101,66 @@ class Preset:
         return self.id.startswith(USER_PREFIX)
 
 
-def _builtin_table() -> dict[str, dict]:
+# ---- preset documents ----------------------------------------------------------------------------------
+
+def _preset(preset_id: str, doc: dict) -> Preset:
+    look = doc["look"]
+    return Preset(id=preset_id, label=doc["label"], material=MaterialParams(name=preset_id, **doc["material"]),
+                  look=Look(look["display"], tuple(look["base_color"]), look["roughness"], look["subsurface"]),
+                  description=doc.get("description", ""))
+
+
+def _toml_text(label: str, description: str, material: MaterialParams, look: Look) -> str:
+    """A user preset file. Every value is a string, a number, a boolean or a list of numbers, whose JSON
+    spelling is also valid TOML."""
+    head = {"label": label}
+    if description:
+        head["description"] = description
+    sections = {
+        "material": {k: v for k, v in material.to_dict().items() if k != "name"},
+        "look": {"display": look.display, "base_color": [float(c) for c in look.base_color],
+                 "roughness": float(look.roughness), "subsurface": float(look.subsurface)},
+    }
+    lines = [f"{k} = {json.dumps(v)}" for k, v in head.items()]
+    for name, table in sections.items():
+        lines += ["", f"[{name}]"] + [f"{k} = {json.dumps(v)}" for k, v in table.items()]
+    return "\n".join(lines + [""])
+
+
+def _builtin() -> dict[str, dict]:
     with open(_BUILTIN_FILE, "rb") as f:
         return _toml.load(f)
 
 
-def _user_files(user_dir: Path | None) -> dict[str, Path]:
-    if user_dir is None or not Path(user_dir).is_dir():
+def _user_paths(user_dir: Path | None) -> dict[str, Path]:
+    folder = None if user_dir is None else Path(user_dir)
+    if folder is None or not folder.is_dir():
         return {}
-    return {USER_PREFIX + p.stem: p for p in Path(user_dir).glob("*.toml")}
-
-
-def _preset_from_doc(preset_id: str, doc: dict) -> Preset:
-    look = doc["look"]
-    return Preset(
-        id=preset_id,
-        label=doc["label"],
-        material=MaterialParams(name=preset_id, **doc["material"]),
-        look=Look(look["display"], tuple(look["base_color"]), look["roughness"], look["subsurface"]),
-        description=doc.get("description", ""),
-    )
+    return {USER_PREFIX + p.stem: p for p in folder.glob("*.toml")}
 
 
 def list_presets(user_dir: Path | None = None) -> list[str]:
-    """Built-in ids sorted, then user ids sorted by label (materials.py:98-101)."""
-    users = list(_user_files(user_dir))
-    users.sort(key=lambda pid: load_preset(pid, user_dir).label.lower())
-    return sorted(_builtin_table()) + users
+    """Built-in ids in sorted order, then the user presets of ``user_dir`` ordered by label, case-insensitive
+    (materials.py:98-101)."""
+    labels = {pid: load_preset(pid, user_dir).label.lower() for pid in _user_paths(user_dir)}
+    return sorted(_builtin()) + sorted(labels, key=labels.__getitem__)
 
 
 def load_preset(preset_id: str, user_dir: Path | None = None) -> Preset:
-    builtin = _builtin_table()
-    if preset_id in builtin:
-        return _preset_from_doc(preset_id, builtin[preset_id])
-    user = _user_files(user_dir)
-    if preset_id in user:
-        return _preset_from_doc(preset_id, _toml.loads(user[preset_id].read_text(encoding="utf-8")))
-    raise KeyError(f"unknown preset {preset_id!r}, available: {sorted([*builtin, *user])}")
+    builtin = _builtin()
+    doc = builtin.get(preset_id)
+    if doc is None:
+        users = _user_paths(user_dir)
+        if preset_id not in users:
+            raise KeyError(f"unknown preset {preset_id!r}, available: {sorted(list(builtin) + list(users))}")
+        doc = _toml.loads(users[preset_id].read_text(encoding="utf-8"))
+    return _preset(preset_id, doc)
 
 
 def user_preset_id(label: str) -> str:
-    """Id for a user preset label: lower case, runs of other characters folded to ``_``."""
-    stem = re.sub(r"[^a-z0-9]+", "_", label.strip().lower()).strip("_")
+    """The id of a user preset labelled ``label``: lower case, every run of characters other than a-z and 0-9
+    folded to one ``_``, outer ones dropped (labels that differ only in case or punctuation share an id)."""
+    stem = "_".join(part for part in re.split(r"[^a-z0-9]+", label.strip().lower()) if part)
     if not stem:
         raise ValueError("a preset name needs at least one letter or digit")
     return USER_PREFIX + stem
@@ -145,26 +168,16 @@ def user_preset_id(label: str) -> str:
 
 def save_user_preset(user_dir: Path, label: str, material: MaterialParams, look: Look,
                      description: str = "") -> Preset:
-    """Write (or replace) a user preset; the label of a built-in preset is refused."""
+    """Write a user preset, replacing the one with the same id; a built-in preset's label is refused."""
     label = label.strip()
-    builtin_labels = {doc["label"].lower() for doc in _builtin_table().values()}
-    if label.lower() in builtin_labels:
+    if any(doc["label"].lower() == label.lower() for doc in _builtin().values()):
         raise ValueError(f"{label!r} is the name of a built-in preset")
     preset_id = user_preset_id(label)
-    out = [f"label = {json.dumps(label)}"]
-    if description:
-        out.append(f"description = {json.dumps(description)}")
-    out += ["", "[material]"]
-    out += [f"{k} = {json.dumps(v)}" for k, v in material.to_dict().items() if k != "name"]
-    out += ["", "[look]", f"display = {json.dumps(look.display)}",
-            f"base_color = {json.dumps([float(c) for c in look.base_color])}",
-            f"roughness = {json.dumps(float(look.roughness))}",
-            f"subsurface = {json.dumps(float(look.subsurface))}", ""]
     folder = Path(user_dir)
     folder.mkdir(parents=True, exist_ok=True)
-    path = folder / (preset_id[len(USER_PREFIX):] + ".toml")
-    tmp = path.with_name(path.name + ".tmp")
-    tmp.write_text("\n".join(out), encoding="utf-8")
+    path = folder / f"{preset_id[len(USER_PREFIX):]}.toml"
+    tmp = folder / f"{path.name}.tmp"
+    tmp.write_text(_toml_text(label, description, material, look), encoding="utf-8")
     os.replace(tmp, path)
     return load_preset(preset_id, folder)
 
@@ -172,7 +185,7 @@ def save_user_preset(user_dir: Path, label: str, material: MaterialParams, look:
 def delete_user_preset(user_dir: Path, preset_id: str) -> None:
     if not preset_id.startswith(USER_PREFIX):
         raise ValueError(f"{preset_id!r} is a built-in preset")
-    path = _user_files(user_dir).get(preset_id)
+    path = _user_paths(user_dir).get(preset_id)
     if path is None:
         raise KeyError(f"unknown user preset {preset_id!r}")
     path.unlink()

@@ -1,12 +1,10 @@
-"""Scene description: the contract between the Blender add-on and the bake runner.
+"""Scene folders: what the Blender add-on writes and the bake runner and the bake server read.
 
-The on-disk format is the reference's (src/mpm_core/scene.py:1-12, 148-219) so scene folders written
-by the add-on load unchanged: ``scene.json`` (format version 1) plus one ``.npy`` per array,
-
-    emitter_{i}_verts.npy  (V,3) float32   local-space vertices
-    emitter_{i}_tris.npy   (T,3) int32     triangles
-    emitter_{i}_xf.npy     (F,4,4) float32 world matrix per frame, frame_start..frame_end
-    collider_{i}_*.npy     the same for colliders
+A scene folder is ``scene.json`` (format version 1) plus three ``.npy`` arrays per emitter and collider i:
+``{emitter|collider}_{i}_verts`` (V,3) float32 local-space vertices, ``_tris`` (T,3) int32 triangles and
+``_xf`` (F,4,4) float32 world matrices, one per frame from frame_start to frame_end. The format, its keys,
+defaults and validation messages are the reference's (src/mpm_core/scene.py:1-12, 31-140, 148-219), so
+folders written by the add-on load unchanged; the code is organised around two tables of JSON keys.
 """
 
 from __future__ import annotations
@@ -26,14 +24,32 @@ WALL_SIDES = ("x-", "x+", "y-", "y+", "z-", "z+")
 EMITTER_KINDS = ("fill", "inflow")
 MAX_SUBSTEPS = 16
 
+# JSON keys of an emitter and of a collider, in the order they are written; the mesh and the per-frame
+# transforms of each live in the object's .npy files
+_EMITTER_KEYS = ("name", "kind", "material", "velocity", "use_object_velocity", "frame_start", "frame_end",
+                 "substeps")
+_COLLIDER_KEYS = ("name", "friction", "closed")
+_ARRAYS = ("verts", "tris", "xf")
+# scalar settings of the scene document after domain / materials / emitters / colliders
+_SCENE_KEYS = ("gravity", "fps", "frame_start", "frame_end", "seed", "particles_per_cell")
+
 
 def _closed_box() -> dict[str, bool]:
     return dict.fromkeys(WALL_SIDES, True)
 
 
+def _floats(values) -> tuple[float, ...]:
+    return tuple(map(float, values))
+
+
+def _matrices(values) -> np.ndarray:
+    return np.asarray(values, np.float32).reshape(-1, 4, 4)
+
+
 @dataclass
 class Domain:
-    """Axis-aligned simulation box (scene.py:36-67). ``dx`` is the longest side over ``resolution``."""
+    """The axis-aligned simulation box (scene.py:36-67); ``walls`` maps each side to closed (True) or open.
+    The cell size ``dx`` is the longest side over ``resolution``."""
 
     min: tuple[float, float, float]
     max: tuple[float, float, float]
@@ -42,18 +58,15 @@ class Domain:
     wall_friction: float = 0.5
 
     def __post_init__(self):
-        self.min = tuple(float(v) for v in self.min)
-        self.max = tuple(float(v) for v in self.max)
-        if any(hi <= lo for lo, hi in zip(self.min, self.max)):
+        self.min, self.max = _floats(self.min), _floats(self.max)
+        if not all(hi > lo for lo, hi in zip(self.min, self.max)):
             raise ValueError("domain max must be greater than min on every axis")
         if self.resolution < 4:
             raise ValueError("domain resolution must be at least 4")
-        merged = _closed_box()
-        merged.update(self.walls)
-        self.walls = merged
-        bad = sorted(set(self.walls) - set(WALL_SIDES))
-        if bad:
-            raise ValueError(f"unknown wall sides {bad}")
+        unknown = sorted(k for k in self.walls if k not in WALL_SIDES)
+        if unknown:
+            raise ValueError(f"unknown wall sides {unknown}")
+        self.walls = {side: self.walls.get(side, True) for side in WALL_SIDES}
 
     @property
     def size(self) -> tuple[float, float, float]:
@@ -65,7 +78,7 @@ class Domain:
 
     @property
     def grid_shape(self) -> tuple[int, int, int]:
-        # the 1e-6 keeps exact multiples of dx from rounding up a cell (scene.py:64-67)
+        # cells per axis; the 1e-6 keeps a side that is an exact multiple of dx from gaining a cell
         h = self.dx
         return tuple(max(1, math.ceil(s / h - 1e-6)) for s in self.size)
 
@@ -82,6 +95,9 @@ class MeshData:
 
 @dataclass
 class Emitter:
+    """A mesh that emits particles: ``fill`` once at frame_start, ``inflow`` every frame in its range,
+    ``substeps`` times per frame at evenly spaced times with the pose interpolated."""
+
     name: str
     kind: str
     material: int
@@ -91,15 +107,15 @@ class Emitter:
     use_object_velocity: bool = False
     frame_start: int = 1
     frame_end: int = 1
-    substeps: int = 1  # inflow: emissions per frame
+    substeps: int = 1
 
     def __post_init__(self):
         if self.kind not in EMITTER_KINDS:
             raise ValueError(f"emitter {self.name!r}: kind must be one of {EMITTER_KINDS}")
-        if not 1 <= self.substeps <= MAX_SUBSTEPS:
+        if self.substeps < 1 or self.substeps > MAX_SUBSTEPS:
             raise ValueError(f"emitter {self.name!r}: substeps must be in 1..{MAX_SUBSTEPS}")
-        self.transforms = np.asarray(self.transforms, np.float32).reshape(-1, 4, 4)
-        self.velocity = tuple(float(v) for v in self.velocity)
+        self.transforms = _matrices(self.transforms)
+        self.velocity = _floats(self.velocity)
 
 
 @dataclass
@@ -111,7 +127,7 @@ class Collider:
     closed: bool = True
 
     def __post_init__(self):
-        self.transforms = np.asarray(self.transforms, np.float32).reshape(-1, 4, 4)
+        self.transforms = _matrices(self.transforms)
 
 
 @dataclass
@@ -128,19 +144,19 @@ class SceneDescription:
     particles_per_cell: int = 8
 
     def __post_init__(self):
-        self.gravity = tuple(float(g) for g in self.gravity)
-        if self.frame_end < self.frame_start:
+        self.gravity = _floats(self.gravity)
+        if self.frame_start > self.frame_end:
             raise ValueError("frame_end must be >= frame_start")
-        root = round(self.particles_per_cell ** (1.0 / 3.0))
-        if root < 1 or root ** 3 != self.particles_per_cell:
+        side = round(self.particles_per_cell ** (1.0 / 3.0))
+        if side < 1 or side * side * side != self.particles_per_cell:
             raise ValueError("particles_per_cell must be a perfect cube (1, 8, 27, ...)")
-        n = self.frame_count
-        for obj in (*self.emitters, *self.colliders):
-            if len(obj.transforms) != n:
-                raise ValueError(f"{obj.name!r} has {len(obj.transforms)} transforms, expected {n} "
+        frames = self.frame_count
+        for obj in self.emitters + self.colliders:
+            if len(obj.transforms) != frames:
+                raise ValueError(f"{obj.name!r} has {len(obj.transforms)} transforms, expected {frames} "
                                  f"(frames {self.frame_start}..{self.frame_end})")
         for e in self.emitters:
-            if not 0 <= e.material < len(self.materials):
+            if e.material < 0 or e.material >= len(self.materials):
                 raise ValueError(f"emitter {e.name!r} references missing material {e.material}")
 
     @property
@@ -148,84 +164,74 @@ class SceneDescription:
         return self.frame_end - self.frame_start + 1
 
 
-def _save_object(directory: Path, prefix: str, mesh: MeshData, transforms: np.ndarray) -> None:
-    np.save(directory / f"{prefix}_verts.npy", mesh.verts)
-    np.save(directory / f"{prefix}_tris.npy", mesh.tris)
-    np.save(directory / f"{prefix}_xf.npy", transforms)
+# ---- folder I/O --------------------------------------------------------------------------------------
+
+def _json_value(v):
+    return list(v) if isinstance(v, tuple) else v
 
 
-def _load_object(directory: Path, prefix: str) -> tuple[MeshData, np.ndarray]:
-    verts = np.load(directory / f"{prefix}_verts.npy")
-    tris = np.load(directory / f"{prefix}_tris.npy")
-    return MeshData(verts, tris), np.load(directory / f"{prefix}_xf.npy")
+def _arrays_of(obj) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    return obj.mesh.verts, obj.mesh.tris, obj.transforms
+
+
+def _write_objects(directory: Path, kind: str, objects, keys) -> list[dict]:
+    docs = []
+    for i, obj in enumerate(objects):
+        for suffix, array in zip(_ARRAYS, _arrays_of(obj)):
+            np.save(directory / f"{kind}_{i}_{suffix}.npy", array)
+        docs.append({k: _json_value(getattr(obj, k)) for k in keys})
+    return docs
+
+
+def _read_objects(directory: Path, kind: str, docs: list[dict], cls) -> list:
+    objects = []
+    for i, doc in enumerate(docs):
+        verts, tris, xf = (np.load(directory / f"{kind}_{i}_{suffix}.npy") for suffix in _ARRAYS)
+        objects.append(cls(mesh=MeshData(verts, tris), transforms=xf, **doc))
+    return objects
 
 
 def write_scene(scene: SceneDescription, directory: Path) -> None:
+    """Write the scene folder; arrays of an earlier scene in the same folder are removed first."""
     directory = Path(directory)
     directory.mkdir(parents=True, exist_ok=True)
-    for stale in directory.glob("*.npy"):
-        stale.unlink()
-    emitters = []
-    for i, e in enumerate(scene.emitters):
-        _save_object(directory, f"emitter_{i}", e.mesh, e.transforms)
-        emitters.append({"name": e.name, "kind": e.kind, "material": e.material, "velocity": list(e.velocity),
-                         "use_object_velocity": e.use_object_velocity, "frame_start": e.frame_start,
-                         "frame_end": e.frame_end, "substeps": e.substeps})
-    colliders = []
-    for i, c in enumerate(scene.colliders):
-        _save_object(directory, f"collider_{i}", c.mesh, c.transforms)
-        colliders.append({"name": c.name, "friction": c.friction, "closed": c.closed})
+    for old in directory.glob("*.npy"):
+        old.unlink()
     d = scene.domain
     doc = {
         "version": FORMAT_VERSION,
         "domain": {"min": list(d.min), "max": list(d.max), "resolution": d.resolution, "walls": d.walls,
                    "wall_friction": d.wall_friction},
         "materials": [m.to_dict() for m in scene.materials],
-        "emitters": emitters,
-        "colliders": colliders,
-        "gravity": list(scene.gravity),
-        "fps": scene.fps,
-        "frame_start": scene.frame_start,
-        "frame_end": scene.frame_end,
-        "seed": scene.seed,
-        "particles_per_cell": scene.particles_per_cell,
+        "emitters": _write_objects(directory, "emitter", scene.emitters, _EMITTER_KEYS),
+        "colliders": _write_objects(directory, "collider", scene.colliders, _COLLIDER_KEYS),
     }
-    (directory / "scene.json").write_text(json.dumps(doc, indent=2), encoding="utf-8")
+    doc.update((k, _json_value(getattr(scene, k))) for k in _SCENE_KEYS)
+    directory.joinpath("scene.json").write_text(json.dumps(doc, indent=2), encoding="utf-8")
 
 
 def read_scene(directory: Path) -> SceneDescription:
     directory = Path(directory)
-    doc = json.loads((directory / "scene.json").read_text(encoding="utf-8"))
+    doc = json.loads(directory.joinpath("scene.json").read_text(encoding="utf-8"))
     if doc.get("version") != FORMAT_VERSION:
         raise ValueError(f"unsupported scene format version {doc.get('version')!r}")
-    emitters = []
-    for i, e in enumerate(doc["emitters"]):
-        mesh, xf = _load_object(directory, f"emitter_{i}")
-        emitters.append(Emitter(mesh=mesh, transforms=xf, **e))
-    colliders = []
-    for i, c in enumerate(doc["colliders"]):
-        mesh, xf = _load_object(directory, f"collider_{i}")
-        colliders.append(Collider(mesh=mesh, transforms=xf, **c))
     dom = doc["domain"]
     return SceneDescription(
         domain=Domain(dom["min"], dom["max"], dom["resolution"], dom["walls"], dom["wall_friction"]),
         materials=[MaterialParams.from_dict(m) for m in doc["materials"]],
-        emitters=emitters,
-        colliders=colliders,
-        gravity=doc["gravity"],
-        fps=doc["fps"],
-        frame_start=doc["frame_start"],
-        frame_end=doc["frame_end"],
-        seed=doc["seed"],
-        particles_per_cell=doc["particles_per_cell"],
+        emitters=_read_objects(directory, "emitter", doc["emitters"], Emitter),
+        colliders=_read_objects(directory, "collider", doc["colliders"], Collider),
+        **{k: doc[k] for k in _SCENE_KEYS},
     )
 
 
 def scene_hash(directory: Path) -> str:
-    """sha256 over scene.json then every .npy in sorted name order, each preceded by its name."""
+    """Identity of a scene folder for checkpoint resume: sha256 over scene.json and then every .npy file in
+    sorted name order, each file's name hashed before its bytes (the reference's, scene.py:222-229, so a
+    checkpoint's recorded hash means the same on either side)."""
     directory = Path(directory)
     h = hashlib.sha256()
-    for path in [directory / "scene.json", *sorted(directory.glob("*.npy"))]:
+    for path in [directory / "scene.json"] + sorted(directory.glob("*.npy")):
         h.update(path.name.encode())
         h.update(path.read_bytes())
     return h.hexdigest()
