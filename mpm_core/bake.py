@@ -1,10 +1,23 @@
-"""Bake runner: scene description in, cache out.
+"""Bake runner: a scene folder in, a cache folder out.
 
     python -m mpm_core.bake CACHE_DIR [--backend hip] [--device cuda:0] [--resume] [--watch-stdin]
 
-Reads ``CACHE_DIR/scene``; writes frames, checkpoints and progress (see mpm_core.cache). Exit code 0
-when done or cancelled, 1 on error. Frame loop, emission timing, NaN stop, cancel and resume follow the
-reference runner (src/mpm_core/bake.py:56-244); the default backend is ``"hip"``.
+Reads the scene in ``CACHE_DIR/scene`` and writes frames, checkpoints and ``progress.json`` into CACHE_DIR in the
+formats of mpm_core.cache. Exit status 0 when the bake is done or cancelled, 1 on error.
+
+A bake's frames depend on what the runner does between solver calls, so that is the reference runner's
+(src/mpm_core/bake.py:56-244):
+
+* the timeline: the first frame only emits; every later frame is advanced from the previous one in pieces
+  that end where inflow emitters emit (``frame_pieces``), the colliders' poses interpolated over each piece;
+* emission: jittered lattice points of the emitter's mesh at its interpolated pose, clipped to the domain,
+  inflow points dropped where material already is, velocities from the emitter (plus the object's own
+  motion when asked);
+* the stops: NaN/inf in the particles, a ``cancel`` file or callback, errors into progress.json;
+* resume from the last checkpoint when the scene hash matches.
+
+The same timeline and emission drive mpm_core.serve, which runs many bakes at once. The default backend is
+the HIP solver.
 """
 
 from __future__ import annotations
@@ -14,6 +27,7 @@ import shutil
 import sys
 import time
 from collections.abc import Callable
+from contextlib import ExitStack
 from dataclasses import dataclass
 from fractions import Fraction
 from pathlib import Path
@@ -28,6 +42,7 @@ from mpm_core.scene import Emitter, SceneDescription, read_scene, scene_hash
 from mpm_core.solver import ColliderGeometry, Solver
 
 DEFAULT_BACKEND = "hip"
+FRESH_SUBFOLDERS = ("frames", "state", "surface")  # removed when a bake starts from scratch
 
 
 @dataclass
@@ -41,11 +56,16 @@ class SceneChangedError(RuntimeError):
     pass
 
 
-class _Progress:
-    def __init__(self, cache_dir: Path):
-        self.path = cache_dir / "progress.json"
-        self.doc = {"status": "starting", "frame": None, "frame_end": None, "particles": 0,
-                    "ms_per_frame": None, "message": "", "warnings": []}
+class ProgressFile:
+    """A bake's progress.json, rewritten whole on every update (the reference's keys, bake.py:41-53): status
+    (starting, running, done, cancelled, error), the last frame written, frame_end, the particle count, the
+    mean milliseconds per frame, an error message and the distinct warnings so far. Without a cache folder
+    the document is only kept in memory."""
+
+    def __init__(self, cache_dir: Path | None = None):
+        self.path = None if cache_dir is None else Path(cache_dir) / "progress.json"
+        self.doc = dict(status="starting", frame=None, frame_end=None, particles=0, ms_per_frame=None, message="",
+                        warnings=[])
 
     def warn(self, message: str) -> None:
         if message not in self.doc["warnings"]:
@@ -53,190 +73,220 @@ class _Progress:
 
     def update(self, **fields) -> None:
         self.doc.update(fields)
-        write_json(self.path, self.doc)
+        if self.path is not None:
+            write_json(self.path, self.doc)
 
+
+def require_closed_emitters(scene: SceneDescription) -> None:
+    for e in scene.emitters:
+        if not mesh_is_closed(e.mesh.tris):
+            raise ValueError(f"emitter {e.name!r}: mesh is not closed (every edge must be shared by exactly "
+                             f"two triangles; in Blender, try Merge by Distance)")
+
+
+# ---- the timeline ----------------------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class Piece:
+    """One ``Solver.advance`` of a frame: from fraction ``start`` of the frame to ``end``, after which the
+    inflow emitters in ``emitters`` emit (in emitter order)."""
+
+    start: Fraction
+    end: Fraction
+    emitters: tuple[int, ...] = ()
+
+
+def _inflow_on(e: Emitter, frame: int) -> bool:
+    return e.kind == "inflow" and e.frame_start <= frame <= e.frame_end
+
+
+def first_frame_emitters(scene: SceneDescription) -> list[int]:
+    """Emitters that emit on the scene's first frame, at the frame itself: every fill emitter and every
+    inflow emitter whose range holds that frame (bake.py:81-90)."""
+    return [i for i, e in enumerate(scene.emitters) if e.kind == "fill" or _inflow_on(e, scene.frame_start)]
+
+
+def frame_pieces(scene: SceneDescription, frame: int) -> list[Piece]:
+    """The pieces that advance ``frame - 1`` to ``frame`` (a frame after the first). An inflow emitter with m
+    substeps emits at j/m, j = 1..m, in every frame of its range but its first, where it emits once, at the
+    frame itself (j = m). Pieces are cut at every such time; the last ends at 1, with or without emissions
+    (bake.py:93-113, 175-190)."""
+    due: dict[Fraction, list[int]] = {Fraction(1): []}
+    for i, e in enumerate(scene.emitters):
+        if _inflow_on(e, frame):
+            for j in range(1 if frame > e.frame_start else e.substeps, e.substeps + 1):
+                due.setdefault(Fraction(j, e.substeps), []).append(i)
+    ends = sorted(due)
+    return [Piece(start, end, tuple(due[end])) for start, end in zip([Fraction(0)] + ends[:-1], ends)]
+
+
+def pose_at(transforms: np.ndarray, k: int, s: Fraction) -> np.ndarray:
+    """World matrix a fraction ``s`` of the way from frame index k-1 to k; on frame index 0, that frame's."""
+    return interpolate_transform(transforms[k - 1], transforms[k], float(s)) if k > 0 else transforms[k]
+
+
+def collider_transforms(scene: SceneDescription, frame: int, s0: Fraction, s1: Fraction):
+    """(start, end) matrices of every collider for the piece from ``frame - 1 + s0`` to ``frame - 1 + s1``."""
+    k = frame - scene.frame_start
+    return tuple(np.stack([pose_at(c.transforms, k, s) for c in scene.colliders]) for s in (s0, s1))
+
+
+# ---- emission --------------------------------------------------------------------------------------------
 
 def emission_seed(scene: SceneDescription, emitter_index: int, frame: int, s: Fraction):
-    """Jitter seed: fill emitters ``seed + i``; inflow ``[seed, frame, i, s.num, s.den]`` (bake.py:63-64)."""
-    e = scene.emitters[emitter_index]
-    if e.kind == "fill":
+    """Seed of an emission's jitter: ``seed + i`` for a fill emitter (it emits once); for an inflow emitter
+    ``[seed, frame, i, s.numerator, s.denominator]``, one stream per emission (bake.py:63-64)."""
+    if scene.emitters[emitter_index].kind == "fill":
         return scene.seed + emitter_index
     return [scene.seed, frame, emitter_index, s.numerator, s.denominator]
 
 
 def emitter_points(scene: SceneDescription, emitter_index: int, frame: int, s: Fraction,
-                   progress: _Progress | None = None, sampler=sample_mesh_volume) -> np.ndarray:
-    """Jittered lattice points of one emission, clipped to the domain (bake.py:56-74).
-
-    ``sampler`` has the signature and result of ``mpm_core.geometry.sample_mesh_volume`` (a backend may
-    offer its own, e.g. the HIP solver's, which computes the same points on the GPU)."""
-    e: Emitter = scene.emitters[emitter_index]
-    k = frame - scene.frame_start
-    xf = interpolate_transform(e.transforms[k - 1], e.transforms[k], float(s)) if k > 0 else e.transforms[k]
+                   progress: ProgressFile | None = None, sampler=sample_mesh_volume) -> np.ndarray:
+    """Points of one emission at ``frame - 1 + s``: the emitter's mesh at its interpolated pose sampled on a
+    jittered lattice of spacing dx / cbrt(particles per cell), points outside the domain dropped, each
+    condition reported once as a warning (bake.py:56-74). ``sampler`` has the signature and results of
+    mpm_core.geometry.sample_mesh_volume (the HIP solver offers one that runs on the GPU)."""
+    e = scene.emitters[emitter_index]
     spacing = scene.domain.dx / round(scene.particles_per_cell ** (1.0 / 3.0))
-    pts = sampler(transform_points(xf, e.mesh.verts), e.mesh.tris, spacing,
-                  emission_seed(scene, emitter_index, frame, s))
+    world = transform_points(pose_at(e.transforms, frame - scene.frame_start, s), e.mesh.verts)
+    pts = sampler(world, e.mesh.tris, spacing, emission_seed(scene, emitter_index, frame, s))
+    note = progress.warn if progress is not None else (lambda message: None)
     if len(pts) == 0:
-        if progress is not None:
-            progress.warn(f"{e.name}: no particles, the mesh is smaller than the particle spacing ({spacing:.4g} m)")
+        note(f"{e.name}: no particles, the mesh is smaller than the particle spacing ({spacing:.4g} m)")
         return pts
-    lo, hi = np.asarray(scene.domain.min), np.asarray(scene.domain.max)
-    keep = np.all((pts >= lo) & (pts <= hi), axis=1)
-    if not keep.all():
-        if progress is not None:
-            progress.warn(f"{e.name}: particles outside the domain were dropped")
-        pts = pts[keep]
+    inside = np.logical_and(pts >= np.asarray(scene.domain.min), pts <= np.asarray(scene.domain.max)).all(axis=1)
+    if not inside.all():
+        note(f"{e.name}: particles outside the domain were dropped")
+        pts = pts[inside]
     return pts
 
 
-def _emit(scene: SceneDescription, solver: Solver, emitter_index: int, frame: int, s: Fraction,
-          progress: _Progress) -> None:
-    """Emit at time ``frame - 1 + s`` (at ``frame`` itself on the first frame)."""
+def emit(scene: SceneDescription, solver: Solver, emitter_index: int, frame: int, s: Fraction,
+         progress: ProgressFile) -> int:
+    """One emission into ``solver`` at ``frame - 1 + s`` (the frame itself on the scene's first frame);
+    returns the number of particles added."""
     e = scene.emitters[emitter_index]
     pts = emitter_points(scene, emitter_index, frame, s, progress,
                          getattr(solver, "sample_mesh_volume", sample_mesh_volume))
     if e.kind == "inflow" and len(pts) and solver.particle_count:
-        pts = pts[~solver.is_occupied(pts)]
-    if len(pts) == 0:
-        return
-    vel = np.tile(np.asarray(e.velocity, np.float32), (len(pts), 1))
+        pts = pts[~solver.is_occupied(pts)]  # no new material where material already is
+    n = len(pts)
+    if n == 0:
+        return 0
+    vel = np.repeat(np.asarray(e.velocity, np.float32)[None], n, axis=0)
     k = frame - scene.frame_start
     if e.use_object_velocity and len(e.transforms) > 1:
-        a, b = (k - 1, k) if k > 0 else (0, 1)  # forward difference on the first frame
-        xf = interpolate_transform(e.transforms[k - 1], e.transforms[k], float(s)) if k > 0 else e.transforms[k]
-        local = transform_points(np.linalg.inv(xf), pts)
+        # the object's motion over the frame at the points' local positions; a forward difference on frame 0
+        a, b = (k - 1, k) if k > 0 else (0, 1)
+        local = transform_points(np.linalg.inv(pose_at(e.transforms, k, s)), pts)
         vel += (transform_points(e.transforms[b], local) - transform_points(e.transforms[a], local)) * scene.fps
-    n = len(pts)
     solver.add_particles(pts, vel, np.full(n, e.material, np.int32), np.full(n, emitter_index, np.int32))
+    return n
 
 
-def _active_emitters(scene: SceneDescription, frame: int) -> list[int]:
-    """Emitters that emit at the start frame itself."""
-    out = []
-    for i, e in enumerate(scene.emitters):
-        if (e.kind == "fill" and frame == scene.frame_start) or \
-                (e.kind == "inflow" and e.frame_start <= frame <= e.frame_end):
-            out.append(i)
-    return out
+# ---- the runner ------------------------------------------------------------------------------------------
 
+class _Runner:
+    """One bake of one cache folder, lock held by the caller."""
 
-def _emission_times(scene: SceneDescription, frame: int) -> list[tuple[Fraction, list[int]]]:
-    """Inflow emissions while advancing from ``frame - 1`` to ``frame`` as (fraction of frame, emitters).
+    def __init__(self, cache_dir: Path, backend: str, device: str | None,
+                 should_cancel: Callable[[], bool] | None):
+        self.dir = cache_dir
+        self.backend = backend
+        self.device = device
+        self.should_cancel = should_cancel
+        self.cancel_file = cache_dir / "cancel"
+        self.cancel_file.unlink(missing_ok=True)  # a request left over from an earlier bake
+        self.progress = ProgressFile(cache_dir)
+        self.progress.update()  # replaces the previous bake's status at once
+        self.written = 0
 
-    An inflow with m substeps emits at j/m, j = 1..m, in each active frame, and only at 1 on its first
-    frame. Sorted by time and always ending at 1, possibly with no emitters (bake.py:100-113).
-    """
-    times: dict[Fraction, list[int]] = {Fraction(1): []}
-    for i, e in enumerate(scene.emitters):
-        if e.kind != "inflow" or not e.frame_start <= frame <= e.frame_end:
-            continue
-        js = range(1, e.substeps + 1) if frame > e.frame_start else (e.substeps,)
-        for j in js:
-            times.setdefault(Fraction(j, e.substeps), []).append(i)
-    return sorted(times.items())
+    def _cancel_requested(self) -> bool:
+        if self.cancel_file.exists():
+            self.cancel_file.unlink()
+            return True
+        return bool(self.should_cancel and self.should_cancel())
 
+    def _open(self, resume: bool) -> int:
+        """Read the scene, make the solver and the writer; returns the first frame to produce."""
+        scene_dir = self.dir / "scene"
+        self.scene = scene = read_scene(scene_dir)
+        self.hash = scene_hash(scene_dir)
+        self.progress.doc["frame_end"] = scene.frame_end
+        require_closed_emitters(scene)
+        self.solver = create_solver(self.backend, domain=scene.domain, materials=scene.materials,
+                                    gravity=scene.gravity, particles_per_cell=scene.particles_per_cell,
+                                    device=self.device)
+        self.solver.set_colliders([ColliderGeometry(c.mesh, c.closed, c.friction) for c in scene.colliders])
+        self.writer = CacheWriter(self.dir, [e.name for e in scene.emitters], scene.fps, scene.frame_start,
+                                  scene.frame_end, scene.domain.dx, self.backend)
+        if not resume:
+            for sub in FRESH_SUBFOLDERS:  # including the previous simulation's surface meshes
+                shutil.rmtree(self.dir / sub, ignore_errors=True)
+            (self.dir / "frames").mkdir()
+            return scene.frame_start
+        saved = read_checkpoint(self.dir)
+        if saved is None:
+            raise FileNotFoundError(f"no checkpoint to resume from in {self.dir}")
+        last, state, saved_hash = saved
+        if saved_hash != self.hash:
+            raise SceneChangedError("the scene changed since this bake was started; bake from scratch")
+        self.solver.load_state(state)
+        return last + 1
 
-def collider_transforms(scene: SceneDescription, frame: int, s0: Fraction, s1: Fraction):
-    """(start, end) collider matrices for advancing from ``frame - 1 + s0`` to ``frame - 1 + s1``."""
-    k = frame - scene.frame_start
-    start = np.stack([interpolate_transform(c.transforms[k - 1], c.transforms[k], float(s0))
-                      for c in scene.colliders])
-    end = np.stack([interpolate_transform(c.transforms[k - 1], c.transforms[k], float(s1))
-                    for c in scene.colliders])
-    return start, end
+    def _produce(self, frame: int) -> None:
+        scene, solver = self.scene, self.solver
+        if frame == scene.frame_start:
+            for i in first_frame_emitters(scene):
+                emit(scene, solver, i, frame, Fraction(1), self.progress)
+            return
+        for piece in frame_pieces(scene, frame):
+            if scene.colliders:
+                solver.set_collider_transforms(*collider_transforms(scene, frame, piece.start, piece.end))
+            solver.advance(float(piece.end - piece.start) / scene.fps)
+            for i in piece.emitters:
+                emit(scene, solver, i, frame, piece.end, self.progress)
+
+    def run(self, resume: bool) -> BakeResult:
+        try:
+            first = self._open(resume)
+            ms = []
+            for frame in range(first, self.scene.frame_end + 1):
+                t0 = time.perf_counter()
+                self._produce(frame)
+                particles = self.solver.get_particles()
+                if not (np.isfinite(particles.positions).all() and np.isfinite(particles.velocities).all()):
+                    message = f"simulation became unstable at frame {frame} (NaN/inf in particle positions)"
+                    self.progress.update(status="error", frame=frame, message=message)
+                    return BakeResult("error", self.written, message)
+                self.writer.write_frame(frame, particles)
+                write_checkpoint(self.dir, frame, self.solver.save_state(), self.hash)
+                self.written += 1
+                ms.append(1000.0 * (time.perf_counter() - t0))
+                self.progress.update(status="running", frame=frame, particles=len(particles.positions),
+                                     ms_per_frame=round(float(np.mean(ms)), 1))
+                if frame < self.scene.frame_end and self._cancel_requested():
+                    self.progress.update(status="cancelled")
+                    return BakeResult("cancelled", self.written)
+            self.progress.update(status="done")
+            return BakeResult("done", self.written)
+        except Exception as exc:
+            self.progress.update(status="error", message=f"{type(exc).__name__}: {exc}")
+            raise
 
 
 def bake(cache_dir: Path, *, backend: str = DEFAULT_BACKEND, device: str | None = None, resume: bool = False,
          should_cancel: Callable[[], bool] | None = None) -> BakeResult:
+    """Bake ``cache_dir``. One bake at a time per cache folder: RuntimeError while another holds it.
+    ``should_cancel`` is asked after every frame but the last, like the ``cancel`` file."""
     cache_dir = Path(cache_dir)
-    try:
-        lock = exclusive_lock(cache_dir / "bake.lock")
-        lock.__enter__()
-    except RuntimeError as exc:
-        raise RuntimeError(f"a bake is already running on {cache_dir}") from exc
-    try:
-        return _bake_locked(cache_dir, backend, device, resume, should_cancel)
-    finally:
-        lock.__exit__(None, None, None)
-
-
-def _bake_locked(cache_dir: Path, backend: str, device: str | None, resume: bool,
-                 should_cancel: Callable[[], bool] | None) -> BakeResult:
-    scene_dir = cache_dir / "scene"
-    cancel_file = cache_dir / "cancel"
-    cancel_file.unlink(missing_ok=True)  # a request left over from an earlier bake
-    progress = _Progress(cache_dir)
-    progress.update()
-    written = 0
-
-    def cancelled() -> bool:
-        if cancel_file.exists():
-            cancel_file.unlink()
-            return True
-        return bool(should_cancel and should_cancel())
-
-    try:
-        scene = read_scene(scene_dir)
-        current_hash = scene_hash(scene_dir)
-        progress.doc["frame_end"] = scene.frame_end
-        for e in scene.emitters:
-            if not mesh_is_closed(e.mesh.tris):
-                raise ValueError(f"emitter {e.name!r}: mesh is not closed (every edge must be shared by exactly "
-                                 f"two triangles; in Blender, try Merge by Distance)")
-        solver = create_solver(backend, domain=scene.domain, materials=scene.materials, gravity=scene.gravity,
-                               particles_per_cell=scene.particles_per_cell, device=device)
-        solver.set_colliders([ColliderGeometry(c.mesh, c.closed, c.friction) for c in scene.colliders])
-        writer = CacheWriter(cache_dir, [e.name for e in scene.emitters], scene.fps, scene.frame_start,
-                             scene.frame_end, scene.domain.dx, backend)
-        if resume:
-            checkpoint = read_checkpoint(cache_dir)
-            if checkpoint is None:
-                raise FileNotFoundError(f"no checkpoint to resume from in {cache_dir}")
-            last_frame, state, saved_hash = checkpoint
-            if saved_hash != current_hash:
-                raise SceneChangedError("the scene changed since this bake was started; bake from scratch")
-            solver.load_state(state)
-            first = last_frame + 1
-        else:
-            for sub in ("frames", "state", "surface"):
-                shutil.rmtree(cache_dir / sub, ignore_errors=True)
-            (cache_dir / "frames").mkdir()
-            first = scene.frame_start
-
-        timings = []
-        for frame in range(first, scene.frame_end + 1):
-            t0 = time.perf_counter()
-            if frame == scene.frame_start:
-                for i in _active_emitters(scene, frame):
-                    _emit(scene, solver, i, frame, Fraction(1), progress)
-            else:
-                done = Fraction(0)
-                for s, emitters in _emission_times(scene, frame):  # advance in pieces, emitting in between
-                    if scene.colliders:
-                        solver.set_collider_transforms(*collider_transforms(scene, frame, done, s))
-                    solver.advance(float(s - done) / scene.fps)
-                    for i in emitters:
-                        _emit(scene, solver, i, frame, s, progress)
-                    done = s
-            particles = solver.get_particles()
-            if not (np.isfinite(particles.positions).all() and np.isfinite(particles.velocities).all()):
-                message = f"simulation became unstable at frame {frame} (NaN/inf in particle positions)"
-                progress.update(status="error", frame=frame, message=message)
-                return BakeResult("error", written, message)
-            writer.write_frame(frame, particles)
-            write_checkpoint(cache_dir, frame, solver.save_state(), current_hash)
-            written += 1
-            timings.append((time.perf_counter() - t0) * 1000.0)
-            progress.update(status="running", frame=frame, particles=len(particles.positions),
-                            ms_per_frame=round(float(np.mean(timings)), 1))
-            if frame < scene.frame_end and cancelled():
-                progress.update(status="cancelled")
-                return BakeResult("cancelled", written)
-        progress.update(status="done")
-        return BakeResult("done", written)
-    except Exception as exc:
-        progress.update(status="error", message=f"{type(exc).__name__}: {exc}")
-        raise
+    with ExitStack() as held:
+        try:
+            held.enter_context(exclusive_lock(cache_dir / "bake.lock"))
+        except RuntimeError as exc:
+            raise RuntimeError(f"a bake is already running on {cache_dir}") from exc
+        return _Runner(cache_dir, backend, device, should_cancel).run(resume)
 
 
 def main(argv: list[str] | None = None) -> int:
@@ -249,18 +299,21 @@ def main(argv: list[str] | None = None) -> int:
                     help="stop (as if cancelled) when stdin closes, i.e. when the launching process exits")
     args = ap.parse_args(argv)
     parent_gone = stdin_closed() if args.watch_stdin else None
-    last_mtime = [0.0]
+    progress_file = args.cache_dir / "progress.json"
+    seen = {"mtime": None}
 
-    def report() -> bool:
-        doc = args.cache_dir / "progress.json"
-        if doc.exists() and doc.stat().st_mtime != last_mtime[0]:
-            last_mtime[0] = doc.stat().st_mtime
-            print(doc.read_text(encoding="utf-8").replace("\n", " "), flush=True)
+    def poll() -> bool:
+        """Echo progress.json as one line whenever it changed; cancel once the parent has gone."""
+        if progress_file.exists():
+            mtime = progress_file.stat().st_mtime
+            if mtime != seen["mtime"]:
+                seen["mtime"] = mtime
+                print(" ".join(progress_file.read_text(encoding="utf-8").splitlines()), flush=True)
         return parent_gone is not None and parent_gone.is_set()
 
     try:
         result = bake(args.cache_dir, backend=args.backend, device=args.device, resume=args.resume,
-                      should_cancel=report)
+                      should_cancel=poll)
     except Exception as exc:
         print(f"error: {type(exc).__name__}: {exc}", file=sys.stderr)
         return 1

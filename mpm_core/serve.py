@@ -35,9 +35,9 @@ from pathlib import Path
 
 import numpy as np
 
-from mpm_core.bake import _active_emitters, _emission_times, _emit, _Progress, collider_transforms
+from mpm_core.bake import (Piece, ProgressFile, collider_transforms, emit, first_frame_emitters, frame_pieces,
+                            require_closed_emitters)
 from mpm_core.cache import CacheWriter, write_checkpoint
-from mpm_core.geometry import mesh_is_closed
 from mpm_core.scene import SceneDescription, scene_hash
 from mpm_core.solver import ColliderGeometry, ParticleData
 
@@ -121,9 +121,8 @@ class _Bake:
         self.hash = hash_
         self.footprint = footprint
         self.frame = self.scene.frame_start  # the frame being produced
-        self.pieces: list[tuple[Fraction, list[int]]] = []
+        self.pieces: list[Piece] = []
         self.piece = 0
-        self.done = Fraction(0)
         self.in_piece = False  # a piece was started in the batch and has not ended
         self.last_output = 0.0
         self.last_write: Future | None = None
@@ -269,17 +268,15 @@ class BakeServer:
         from mpm_core.backends import create_solver
 
         req, scene = h.request, h.request.scene
-        progress = _NullProgress()
+        progress = ProgressFile(None)
         try:
-            for e in scene.emitters:
-                if not mesh_is_closed(e.mesh.tris):
-                    raise ValueError(f"emitter {e.name!r}: mesh is not closed")
+            require_closed_emitters(scene)
             writer = None
             hash_ = ""
             if req.cache_dir is not None:
                 cache_dir = Path(req.cache_dir)
                 cache_dir.mkdir(parents=True, exist_ok=True)
-                progress = _Progress(cache_dir)
+                progress = ProgressFile(cache_dir)
                 progress.doc["frame_end"] = scene.frame_end
                 progress.update()
                 if (cache_dir / "scene").exists():
@@ -302,8 +299,8 @@ class BakeServer:
         b.frame_t0 = h.admitted
         b.last_output = h.admitted
         try:
-            for i in _active_emitters(scene, scene.frame_start):
-                _emit(scene, solver, i, scene.frame_start, Fraction(1), progress)
+            for i in first_frame_emitters(scene):
+                emit(scene, solver, i, scene.frame_start, Fraction(1), progress)
             self._frame_out(b)
             self._next_frame(b)
         except Exception as exc:  # noqa: BLE001
@@ -313,11 +310,11 @@ class BakeServer:
     # ---- frames and pieces ----------------------------------------------------------------------------
     def _prepare_piece(self, b: _Bake) -> float:
         """Collider poses of the bake's next piece (bake.py:184-186); returns the piece's length."""
-        s, _ = b.pieces[b.piece]
+        piece = b.pieces[b.piece]
         if b.scene.colliders:
-            b.solver.set_collider_transforms(*collider_transforms(b.scene, b.frame, b.done, s))
+            b.solver.set_collider_transforms(*collider_transforms(b.scene, b.frame, piece.start, piece.end))
         b.count_at_piece = b.solver.particle_count
-        return float(s - b.done) / b.scene.fps
+        return float(piece.end - piece.start) / b.scene.fps
 
     def _next_frame(self, b: _Bake) -> None:
         b.frame += 1
@@ -327,9 +324,8 @@ class BakeServer:
             b.handle.status = "done"
             self._chain(b, lambda prog=b.progress: prog.update(status="done"))
             return
-        b.pieces = _emission_times(b.scene, b.frame)
+        b.pieces = frame_pieces(b.scene, b.frame)
         b.piece = 0
-        b.done = Fraction(0)
 
     def _begin_pieces(self) -> None:
         """Start the next piece of every bake that has none in flight."""
@@ -356,12 +352,11 @@ class BakeServer:
         try:
             self.particle_substeps += b.count_at_piece * substeps
             h.particle_substeps += b.count_at_piece * substeps
-            s, emitters = b.pieces[b.piece]
-            for i in emitters:
-                _emit(b.scene, b.solver, i, b.frame, s, b.progress)
-            b.done = s
+            piece = b.pieces[b.piece]
+            for i in piece.emitters:
+                emit(b.scene, b.solver, i, b.frame, piece.end, b.progress)
             b.piece += 1
-            if s == 1:
+            if piece.end == 1:
                 self._frame_out(b)
                 if b.frame < b.scene.frame_end and self._cancelled(b):
                     h.status = "cancelled"
@@ -503,15 +498,3 @@ def _release(solver) -> None:
     if close is not None:
         close()
 
-
-class _NullProgress:
-    """Progress sink of a request without a cache directory."""
-
-    def __init__(self):
-        self.doc: dict = {}
-
-    def warn(self, message: str) -> None:
-        pass
-
-    def update(self, **fields) -> None:
-        pass

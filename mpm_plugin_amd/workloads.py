@@ -50,41 +50,64 @@ def _scale(x, y=None, z=None) -> np.ndarray:
     return np.diag([x, x if y is None else y, x if z is None else z, 1.0]).astype(np.float32)
 
 
+def _wound_outward(verts: np.ndarray, tris: np.ndarray) -> np.ndarray:
+    """Triangles of a convex mesh, each reordered where needed so that its normal (b - a) x (c - a) points
+    away from the centroid of the vertices."""
+    v = np.asarray(verts, np.float64)
+    a, b, c = (v[tris[:, k]] for k in range(3))
+    away = np.einsum("ij,ij->i", np.cross(b - a, c - a), (a + b + c) / 3.0 - v.mean(axis=0))
+    out = np.array(tris, np.int32)
+    out[away < 0] = out[away < 0][:, ::-1]
+    return out
+
+
 def box_mesh() -> tuple[np.ndarray, np.ndarray]:
-    """Unit cube [0,1]^3 with outward winding."""
-    v = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], np.float32)
-    t = np.array([[0, 2, 1], [1, 2, 3], [4, 5, 6], [5, 7, 6], [0, 1, 4], [1, 5, 4],
-                  [2, 6, 3], [3, 6, 7], [0, 4, 2], [2, 4, 6], [1, 3, 5], [3, 7, 5]], np.int32)
-    return v, t
+    """Unit cube [0,1]^3: corner i at (i & 1, i >> 1 & 1, i >> 2 & 1), two triangles per face, wound outward."""
+    corners = np.array([[(i >> axis) & 1 for axis in range(3)] for i in range(8)], np.float32)
+    tris = []
+    for axis in range(3):
+        u, w = (a for a in range(3) if a != axis)
+        for side in (0, 1):
+            quad = [(side << axis) | (i << u) | (j << w) for i, j in ((0, 0), (1, 0), (1, 1), (0, 1))]
+            tris += [quad[:3], [quad[0], quad[2], quad[3]]]
+    return corners, _wound_outward(corners, np.array(tris))
 
 
 def icosphere(subdivisions: int) -> tuple[np.ndarray, np.ndarray]:
-    """Unit icosphere: 10 * 4^s + 2 vertices, 20 * 4^s triangles."""
-    g = (1 + 5 ** 0.5) / 2
-    verts = [np.array(p, float) for p in
-             ([-1, g, 0], [1, g, 0], [-1, -g, 0], [1, -g, 0], [0, -1, g], [0, 1, g],
-              [0, -1, -g], [0, 1, -g], [g, 0, -1], [g, 0, 1], [-g, 0, -1], [-g, 0, 1])]
-    verts = [p / np.linalg.norm(p) for p in verts]
-    faces = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
-             [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
-             [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    """Unit icosphere, vertices in float32 (see icosphere_f64)."""
+    v, t = icosphere_f64(subdivisions)
+    return v.astype(np.float32), t
+
+
+def icosphere_f64(subdivisions: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere: 10 * 4^s + 2 vertices, 20 * 4^s triangles. The icosahedron's vertices are the cyclic
+    permutations of (0, +-1, +-golden ratio), its faces the triples of mutually nearest vertices; every
+    subdivision splits each triangle into four at its edge midpoints, pushed out to the sphere."""
+    g = (1.0 + 5.0 ** 0.5) / 2.0
+    base = [np.array(p) for s in (-1.0, 1.0) for t in (-g, g) for p in ((0.0, s, t), (s, t, 0.0), (t, 0.0, s))]
+    verts = [p / np.linalg.norm(p) for p in base]
+    dist = np.linalg.norm(np.subtract.outer(np.array(verts), np.array(verts)).diagonal(axis1=1, axis2=3), axis=2)
+    near = np.isclose(dist, dist[dist > 0].min())
+    faces = _wound_outward(np.array(verts), np.array(
+        [(i, j, k) for i in range(12) for j in range(i + 1, 12) for k in range(j + 1, 12)
+         if near[i, j] and near[j, k] and near[i, k]])).tolist()
     for _ in range(subdivisions):
-        mids: dict[tuple[int, int], int] = {}
+        midpoint: dict[frozenset, int] = {}
 
-        def mid(a: int, b: int) -> int:
-            key = (a, b) if a < b else (b, a)
-            if key not in mids:
+        def split(a: int, b: int) -> int:
+            edge = frozenset((a, b))
+            if edge not in midpoint:
                 m = verts[a] + verts[b]
+                midpoint[edge] = len(verts)
                 verts.append(m / np.linalg.norm(m))
-                mids[key] = len(verts) - 1
-            return mids[key]
+            return midpoint[edge]
 
-        nf = []
+        finer = []
         for a, b, c in faces:
-            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
-            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
-        faces = nf
-    return np.array(verts, np.float32), np.array(faces, np.int32)
+            ab, bc, ca = split(a, b), split(b, c), split(c, a)
+            finer += [[a, ab, ca], [ab, b, bc], [ca, bc, c], [ab, bc, ca]]
+        faces = finer
+    return np.array(verts), np.array(faces, np.int32)
 
 
 def box_points(lo, hi, spacing: float, seed) -> np.ndarray:

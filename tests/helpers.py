@@ -1,15 +1,21 @@
-"""Shared test helpers: meshes, matrices and small scenes (the reference's tests/helpers.py roles)."""
+"""Shared test helpers: matrices, meshes, particle bodies, solvers, a small scene and error measures.
+
+Meshes come from the workload module (mpm_plugin_amd.workloads), so tests and benchmarks build bodies and
+colliders from the same geometry.
+"""
 
 from __future__ import annotations
 
 import numpy as np
 
 from mpm_core.materials import load_preset
-from mpm_core.scene import Collider, Domain, Emitter, MeshData, SceneDescription
+from mpm_core.scene import WALL_SIDES, Collider, Domain, Emitter, MeshData, SceneDescription
+from mpm_plugin_amd import workloads
 
-WALL_SIDES = ("x-", "x+", "y-", "y+", "z-", "z+")
 ALL_WALLS = dict.fromkeys(WALL_SIDES, True)
 
+
+# ---- 4x4 matrices (float32, column vectors) -------------------------------------------------------------
 
 def translate(x, y, z) -> np.ndarray:
     m = np.eye(4, dtype=np.float32)
@@ -21,76 +27,53 @@ def scale(x, y=None, z=None) -> np.ndarray:
     return np.diag([x, x if y is None else y, x if z is None else z, 1.0]).astype(np.float32)
 
 
-def rot_x(degrees: float) -> np.ndarray:
-    a = np.radians(degrees)
+def _rotation(axis: int, degrees: float) -> np.ndarray:
+    """Right-handed rotation about coordinate axis ``axis`` by ``degrees``."""
+    c, s = np.cos(np.radians(degrees)), np.sin(np.radians(degrees))
+    i, j = [(1, 2), (2, 0), (0, 1)][axis]  # the plane the rotation turns, in its positive sense
     m = np.eye(4, dtype=np.float32)
-    m[1, 1], m[1, 2], m[2, 1], m[2, 2] = np.cos(a), -np.sin(a), np.sin(a), np.cos(a)
+    m[i, i] = m[j, j] = c
+    m[i, j], m[j, i] = -s, s
     return m
+
+
+def rot_x(degrees: float) -> np.ndarray:
+    return _rotation(0, degrees)
 
 
 def rot_y(degrees: float) -> np.ndarray:
-    a = np.radians(degrees)
-    m = np.eye(4, dtype=np.float32)
-    m[0, 0], m[0, 2], m[2, 0], m[2, 2] = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)
-    return m
+    return _rotation(1, degrees)
 
+
+# ---- meshes -------------------------------------------------------------------------------------------
 
 def unit_cube_mesh(centered: bool = False) -> tuple[np.ndarray, np.ndarray]:
-    """Closed unit cube with outward winding: [0,1]^3, or [-0.5,0.5]^3 when centered."""
-    v = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], np.float32)
-    if centered:
-        v -= 0.5
-    t = np.array([[0, 2, 1], [1, 2, 3], [4, 5, 6], [5, 7, 6], [0, 1, 4], [1, 5, 4],
-                  [2, 6, 3], [3, 6, 7], [0, 4, 2], [2, 4, 6], [1, 3, 5], [3, 7, 5]], np.int32)
-    return v, t
+    """Closed unit cube wound outward: [0,1]^3, or [-0.5,0.5]^3 when ``centered``."""
+    v, t = workloads.box_mesh()
+    return (v - np.float32(0.5) if centered else v), t
 
 
 def plane_mesh(size: float) -> tuple[np.ndarray, np.ndarray]:
-    """Open square in the xy plane centred at the origin, normal +z."""
-    h = size / 2
-    v = np.array([[-h, -h, 0], [h, -h, 0], [h, h, 0], [-h, h, 0]], np.float32)
-    return v, np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    """An open square of side ``size`` in the xy plane, centred on the origin, normal +z."""
+    corners = np.array([(-1, -1), (1, -1), (1, 1), (-1, 1)], np.float32) * np.float32(size / 2)
+    verts = np.concatenate([corners, np.zeros((4, 1), np.float32)], axis=1)
+    return verts, np.array([[0, 1, 2], [0, 2, 3]], np.int32)
 
 
 def icosphere(radius: float, subdivisions: int) -> tuple[np.ndarray, np.ndarray]:
-    """Icosphere by repeated midpoint subdivision of an icosahedron, vertices on the sphere."""
-    g = (1 + 5 ** 0.5) / 2
-    verts = [np.array(p, float) for p in
-             ([-1, g, 0], [1, g, 0], [-1, -g, 0], [1, -g, 0], [0, -1, g], [0, 1, g],
-              [0, -1, -g], [0, 1, -g], [g, 0, -1], [g, 0, 1], [-g, 0, -1], [-g, 0, 1])]
-    verts = [p / np.linalg.norm(p) for p in verts]
-    faces = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
-             [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
-             [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]]
-    for _ in range(subdivisions):
-        mids: dict[tuple[int, int], int] = {}
+    v, t = workloads.icosphere_f64(subdivisions)
+    return (v * radius).astype(np.float32), t
 
-        def mid(a: int, b: int) -> int:
-            key = (a, b) if a < b else (b, a)
-            if key not in mids:
-                m = verts[a] + verts[b]
-                verts.append(m / np.linalg.norm(m))
-                mids[key] = len(verts) - 1
-            return mids[key]
 
-        nf = []
-        for a, b, c in faces:
-            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
-            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
-        faces = nf
-    return (np.array(verts) * radius).astype(np.float32), np.array(faces, np.int32)
-
+# ---- particle bodies and solvers ---------------------------------------------------------------------------
 
 def box_particles(lo, hi, dx: float, seed: int = 0) -> np.ndarray:
-    """Particles filling the box [lo, hi] at 8 per cell (spacing dx/2)."""
-    from mpm_core.geometry import sample_mesh_volume, transform_points
-
-    v, t = unit_cube_mesh()
-    m = translate(*lo) @ scale(*np.subtract(hi, lo))
-    return sample_mesh_volume(transform_points(m, v), t, dx / 2, seed)
+    """The box [lo, hi] sampled at 8 particles per cell (spacing dx/2)."""
+    return workloads.box_points(lo, hi, dx / 2, seed)
 
 
 def sphere_particles(center, radius: float, dx: float, seed: int = 0) -> np.ndarray:
+    """A ball sampled at 8 particles per cell through a 4-times subdivided icosphere."""
     from mpm_core.geometry import sample_mesh_volume
 
     v, t = icosphere(radius, 4)
@@ -105,7 +88,7 @@ def make_solver(device, domain, materials, gravity=(0.0, 0.0, -9.81), backend: s
 
 def add(solver, positions, material_id: int = 0, emitter_id: int = 0, velocity=(0.0, 0.0, 0.0)) -> None:
     n = len(positions)
-    vel = np.tile(np.asarray(velocity, np.float32), (n, 1))
+    vel = np.broadcast_to(np.asarray(velocity, np.float32), (n, 3)).copy()
     solver.add_particles(positions, vel, np.full(n, material_id, np.int32), np.full(n, emitter_id, np.int32))
 
 
@@ -116,7 +99,7 @@ def solver_with_box(device, domain, material, lo, hi):
 
 
 def solver_with_cube(device, domain, lo, hi):
-    """Jelly box."""
+    """A jelly box (the built-in preset)."""
     return solver_with_box(device, domain, load_preset("jelly").material, lo, hi)
 
 
@@ -127,7 +110,7 @@ def solver_with_sphere(device, domain, material, center, radius):
 
 
 def solver_with_presets(device, domain, boxes):
-    """boxes: [(preset_id, lo, hi)]; material and emitter id = list index."""
+    """``boxes``: [(preset id, lo, hi)]; box i gets material i and emitter i."""
     s = make_solver(device, domain, [load_preset(p).material for p, _, _ in boxes])
     for i, (_, lo, hi) in enumerate(boxes):
         add(s, box_particles(lo, hi, domain.dx, seed=i), material_id=i, emitter_id=i)
@@ -139,32 +122,36 @@ def run(solver, frames: int, fps: float = 24.0) -> None:
         solver.advance(1.0 / fps)
 
 
+# ---- a small scene -------------------------------------------------------------------------------------
+
 def small_scene(frame_end: int = 10, gravity=(0.0, 0.0, -9.81), particles_per_cell: int = 8,
                 emitter_frames: int | None = None, emitter_offset=(0.0, 0.0, 0.0), kind: str = "fill",
                 velocity=(0.0, 0.0, 0.0), emit_frames: tuple[int, int] | None = None,
                 resolution: int = 32, preset: str = "jelly") -> SceneDescription:
-    """Unit-cube domain, one cube emitter "emitter_0", one open plane collider at z = 0.05."""
-    n = frame_end if emitter_frames is None else emitter_frames
-    xf = translate(0.4 + emitter_offset[0], 0.4 + emitter_offset[1], 0.1 + emitter_offset[2]) @ scale(0.2)
-    start, end = emit_frames or (1, 1)
+    """Frames 1..frame_end in the unit-cube domain: one 0.2 m cube emitter ``emitter_0`` with its corner at
+    (0.4, 0.4, 0.1) + ``emitter_offset`` (``emitter_frames`` transforms, default one per frame; it emits over
+    ``emit_frames``, default frame 1) and one open 0.8 m square collider ``floor`` at z = 0.05."""
+    frames = frame_end if emitter_frames is None else emitter_frames
+    corner = np.add((0.4, 0.4, 0.1), emitter_offset)
+    first, last = emit_frames if emit_frames is not None else (1, 1)
     emitter = Emitter(name="emitter_0", kind=kind, material=0, mesh=MeshData(*unit_cube_mesh()),
-                      transforms=np.repeat(xf[None], n, 0), velocity=tuple(velocity),
-                      frame_start=start, frame_end=end)
-    collider = Collider(name="floor", mesh=MeshData(*plane_mesh(0.8)),
-                        transforms=np.repeat(translate(0.5, 0.5, 0.05)[None], frame_end, 0),
-                        friction=0.5, closed=False)
+                      transforms=np.stack([translate(*corner) @ scale(0.2)] * frames), velocity=tuple(velocity),
+                      frame_start=first, frame_end=last)
+    floor = Collider(name="floor", mesh=MeshData(*plane_mesh(0.8)),
+                     transforms=np.stack([translate(0.5, 0.5, 0.05)] * frame_end), friction=0.5, closed=False)
     return SceneDescription(domain=Domain((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), resolution),
-                            materials=[load_preset(preset).material], emitters=[emitter], colliders=[collider],
+                            materials=[load_preset(preset).material], emitters=[emitter], colliders=[floor],
                             gravity=tuple(gravity), frame_start=1, frame_end=frame_end,
                             particles_per_cell=particles_per_cell)
 
 
+# ---- error measures ------------------------------------------------------------------------------------
+
 def rel_err(a: np.ndarray, b: np.ndarray) -> float:
-    """Norm-wise relative error ||a - b|| / ||b|| (0 when both vanish)."""
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    nb = np.linalg.norm(b)
-    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+    """Norm-wise relative error ||a - b|| / ||b|| in FP64 (||a|| when b vanishes)."""
+    a64, b64 = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    scale_b = np.linalg.norm(b64)
+    return float(np.linalg.norm(a64 - b64) / scale_b) if scale_b > 0 else float(np.linalg.norm(a64))
 
 
 def c_close(a_C: np.ndarray, b_C: np.ndarray, b_v: np.ndarray, dx: float, rel: float = 1e-4) -> tuple[bool, float]:

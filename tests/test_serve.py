@@ -14,7 +14,7 @@ import json
 import numpy as np
 import pytest
 from helpers import small_scene
-from test_bake import make_cache_dir  # noqa: F401 - registers the "fake" backends
+from test_bake import RecordingSolver  # noqa: F401 - registers the "recording" backends
 
 from mpm_core.bake import bake
 from mpm_core.cache import CacheReader
@@ -59,11 +59,11 @@ def test_mesh_volume_and_footprint():
 
 def test_serial_server_writes_the_runners_cache(tmp_path):
     """Three bakes served together (pieces interleaved) write exactly what three runner bakes write."""
-    server = BakeServer(batched=False, backend="fake", max_active=3)
+    server = BakeServer(batched=False, backend="recording", max_active=3)
     handles = []
     for name, scene in scenes().items():
         ref = write_cache(tmp_path, "ref_" + name, scene)
-        assert bake(ref, backend="fake").status == "done"
+        assert bake(ref, backend="recording").status == "done"
         out = write_cache(tmp_path, "srv_" + name, scene)
         handles.append(server.submit(BakeRequest(read_scene(out / "scene"), cache_dir=out, name=name)))
     while not server.idle:
@@ -86,10 +86,10 @@ def test_serial_server_writes_the_runners_cache(tmp_path):
 
 
 def test_error_and_cancel_do_not_stop_the_others(tmp_path):
-    server = BakeServer(batched=False, backend="fake_nan4", max_active=4)
+    server = BakeServer(batched=False, backend="recording_nan3", max_active=4)
     bad = write_cache(tmp_path, "bad", small_scene(frame_end=8))
     h_bad = server.submit(BakeRequest(read_scene(bad / "scene"), cache_dir=bad))
-    server2 = BakeServer(batched=False, backend="fake", max_active=4)
+    server2 = BakeServer(batched=False, backend="recording", max_active=4)
     keep = write_cache(tmp_path, "keep", small_scene(frame_end=4))
     stop = write_cache(tmp_path, "stop", small_scene(frame_end=8))
     h_keep = server2.submit(BakeRequest(read_scene(keep / "scene"), cache_dir=keep))
@@ -112,7 +112,7 @@ def test_error_and_cancel_do_not_stop_the_others(tmp_path):
 
 def test_admission_waits_for_memory():
     """A request that does not fit the budget waits until a running bake has ended."""
-    server = BakeServer(batched=False, backend="fake", max_active=8, hbm_budget=100)
+    server = BakeServer(batched=False, backend="recording", max_active=8, hbm_budget=100)
     a = server.submit(BakeRequest(small_scene(frame_end=3), footprint=60))
     b = server.submit(BakeRequest(small_scene(frame_end=3), footprint=60))
     c = server.submit(BakeRequest(small_scene(frame_end=2), footprint=30))
@@ -127,7 +127,7 @@ def test_admission_waits_for_memory():
 
 
 def test_arrivals_and_report():
-    server = BakeServer(batched=False, backend="fake", max_active=2)
+    server = BakeServer(batched=False, backend="recording", max_active=2)
     reqs = [(0.01 * i, BakeRequest(small_scene(frame_end=3))) for i in range(4)]
     handles = server.run(reqs)
     server.close()
