@@ -27,7 +27,6 @@ import shutil
 import sys
 import time
 from collections.abc import Callable
-from contextlib import ExitStack
 from dataclasses import dataclass
 from fractions import Fraction
 from pathlib import Path
@@ -35,9 +34,9 @@ from pathlib import Path
 import numpy as np
 
 from mpm_core.backends import create_solver
-from mpm_core.cache import (CacheWriter, exclusive_lock, read_checkpoint, stdin_closed, write_checkpoint,
-                            write_json)
+from mpm_core.cache import CacheWriter, read_checkpoint, write_checkpoint
 from mpm_core.geometry import interpolate_transform, mesh_is_closed, sample_mesh_volume, transform_points
+from mpm_core.jobs import FolderJob, ProgressFile, command_line
 from mpm_core.scene import Emitter, SceneDescription, read_scene, scene_hash
 from mpm_core.solver import ColliderGeometry, Solver
 
@@ -54,27 +53,6 @@ class BakeResult:
 
 class SceneChangedError(RuntimeError):
     pass
-
-
-class ProgressFile:
-    """A bake's progress.json, rewritten whole on every update (the reference's keys, bake.py:41-53): status
-    (starting, running, done, cancelled, error), the last frame written, frame_end, the particle count, the
-    mean milliseconds per frame, an error message and the distinct warnings so far. Without a cache folder
-    the document is only kept in memory."""
-
-    def __init__(self, cache_dir: Path | None = None):
-        self.path = None if cache_dir is None else Path(cache_dir) / "progress.json"
-        self.doc = dict(status="starting", frame=None, frame_end=None, particles=0, ms_per_frame=None, message="",
-                        warnings=[])
-
-    def warn(self, message: str) -> None:
-        if message not in self.doc["warnings"]:
-            self.doc["warnings"].append(message)
-
-    def update(self, **fields) -> None:
-        self.doc.update(fields)
-        if self.path is not None:
-            write_json(self.path, self.doc)
 
 
 def require_closed_emitters(scene: SceneDescription) -> None:
@@ -188,25 +166,15 @@ def emit(scene: SceneDescription, solver: Solver, emitter_index: int, frame: int
 # ---- the runner ------------------------------------------------------------------------------------------
 
 class _Runner:
-    """One bake of one cache folder, lock held by the caller."""
+    """One bake of one cache folder, inside its FolderJob."""
 
-    def __init__(self, cache_dir: Path, backend: str, device: str | None,
-                 should_cancel: Callable[[], bool] | None):
-        self.dir = cache_dir
+    def __init__(self, job: FolderJob, backend: str, device: str | None):
+        self.job = job
+        self.dir = job.folder
+        self.progress = job.progress
         self.backend = backend
         self.device = device
-        self.should_cancel = should_cancel
-        self.cancel_file = cache_dir / "cancel"
-        self.cancel_file.unlink(missing_ok=True)  # a request left over from an earlier bake
-        self.progress = ProgressFile(cache_dir)
-        self.progress.update()  # replaces the previous bake's status at once
         self.written = 0
-
-    def _cancel_requested(self) -> bool:
-        if self.cancel_file.exists():
-            self.cancel_file.unlink()
-            return True
-        return bool(self.should_cancel and self.should_cancel())
 
     def _open(self, resume: bool) -> int:
         """Read the scene, make the solver and the writer; returns the first frame to produce."""
@@ -249,7 +217,7 @@ class _Runner:
                 emit(scene, solver, i, frame, piece.end, self.progress)
 
     def run(self, resume: bool) -> BakeResult:
-        try:
+        with self.job.recording_errors():
             first = self._open(resume)
             ms = []
             for frame in range(first, self.scene.frame_end + 1):
@@ -266,59 +234,45 @@ class _Runner:
                 ms.append(1000.0 * (time.perf_counter() - t0))
                 self.progress.update(status="running", frame=frame, particles=len(particles.positions),
                                      ms_per_frame=round(float(np.mean(ms)), 1))
-                if frame < self.scene.frame_end and self._cancel_requested():
+                if frame < self.scene.frame_end and self.job.cancel_requested():
                     self.progress.update(status="cancelled")
                     return BakeResult("cancelled", self.written)
             self.progress.update(status="done")
             return BakeResult("done", self.written)
-        except Exception as exc:
-            self.progress.update(status="error", message=f"{type(exc).__name__}: {exc}")
-            raise
 
 
 def bake(cache_dir: Path, *, backend: str = DEFAULT_BACKEND, device: str | None = None, resume: bool = False,
          should_cancel: Callable[[], bool] | None = None) -> BakeResult:
     """Bake ``cache_dir``. One bake at a time per cache folder: RuntimeError while another holds it.
-    ``should_cancel`` is asked after every frame but the last, like the ``cancel`` file."""
+    ``should_cancel`` is asked after every frame but the last, when no ``cancel`` file is there."""
     cache_dir = Path(cache_dir)
-    with ExitStack() as held:
-        try:
-            held.enter_context(exclusive_lock(cache_dir / "bake.lock"))
-        except RuntimeError as exc:
-            raise RuntimeError(f"a bake is already running on {cache_dir}") from exc
-        return _Runner(cache_dir, backend, device, should_cancel).run(resume)
+    with FolderJob(cache_dir, "bake.lock", f"a bake is already running on {cache_dir}", should_cancel) as job:
+        return _Runner(job, backend, device).run(resume)
 
 
 def main(argv: list[str] | None = None) -> int:
-    ap = argparse.ArgumentParser(prog="python -m mpm_core.bake", description=__doc__.split("\n\n")[0])
-    ap.add_argument("cache_dir", type=Path)
-    ap.add_argument("--backend", default=DEFAULT_BACKEND)
-    ap.add_argument("--device", default=None)
-    ap.add_argument("--resume", action="store_true")
-    ap.add_argument("--watch-stdin", action="store_true",
-                    help="stop (as if cancelled) when stdin closes, i.e. when the launching process exits")
-    args = ap.parse_args(argv)
-    parent_gone = stdin_closed() if args.watch_stdin else None
-    progress_file = args.cache_dir / "progress.json"
-    seen = {"mtime": None}
+    def options(ap: argparse.ArgumentParser) -> None:
+        ap.add_argument("--backend", default=DEFAULT_BACKEND)
+        ap.add_argument("--resume", action="store_true")
 
-    def poll() -> bool:
-        """Echo progress.json as one line whenever it changed; cancel once the parent has gone."""
-        if progress_file.exists():
-            mtime = progress_file.stat().st_mtime
-            if mtime != seen["mtime"]:
-                seen["mtime"] = mtime
-                print(" ".join(progress_file.read_text(encoding="utf-8").splitlines()), flush=True)
-        return parent_gone is not None and parent_gone.is_set()
+    def run(args, parent_gone):
+        progress_file = args.cache_dir / "progress.json"
+        seen = {"mtime": None}
 
-    try:
-        result = bake(args.cache_dir, backend=args.backend, device=args.device, resume=args.resume,
-                      should_cancel=poll)
-    except Exception as exc:
-        print(f"error: {type(exc).__name__}: {exc}", file=sys.stderr)
-        return 1
-    print(f"{result.status}: {result.frames_written} frames written. {result.message}".strip())
-    return 1 if result.status == "error" else 0
+        def poll() -> bool:
+            """Echo progress.json as one line whenever it changed; cancel once the parent has gone."""
+            if progress_file.exists():
+                mtime = progress_file.stat().st_mtime
+                if mtime != seen["mtime"]:
+                    seen["mtime"] = mtime
+                    print(" ".join(progress_file.read_text(encoding="utf-8").splitlines()), flush=True)
+            return parent_gone is not None and parent_gone()
+
+        return bake(args.cache_dir, backend=args.backend, device=args.device, resume=args.resume,
+                    should_cancel=poll)
+
+    return command_line("python -m mpm_core.bake", __doc__, argv, options, run,
+                        lambda r: f"{r.status}: {r.frames_written} frames written. {r.message}".strip())
 
 
 if __name__ == "__main__":

@@ -35,9 +35,10 @@ from pathlib import Path
 
 import numpy as np
 
-from mpm_core.bake import (Piece, ProgressFile, collider_transforms, emit, first_frame_emitters, frame_pieces,
+from mpm_core.bake import (Piece, collider_transforms, emit, first_frame_emitters, frame_pieces,
                             require_closed_emitters)
 from mpm_core.cache import CacheWriter, write_checkpoint
+from mpm_core.jobs import ProgressFile
 from mpm_core.scene import SceneDescription, scene_hash
 from mpm_core.solver import ColliderGeometry, ParticleData
 
@@ -276,7 +277,7 @@ class BakeServer:
             if req.cache_dir is not None:
                 cache_dir = Path(req.cache_dir)
                 cache_dir.mkdir(parents=True, exist_ok=True)
-                progress = ProgressFile(cache_dir)
+                progress = ProgressFile(cache_dir / "progress.json")
                 progress.doc["frame_end"] = scene.frame_end
                 progress.update()
                 if (cache_dir / "scene").exists():

@@ -1,9 +1,11 @@
-"""Per-emitter surfacing settings and ``<cache>/surface_settings.json``, which the add-on writes
-(reference src/mpm_surface/settings.py:1-64; same file format)::
+"""Surfacing settings per emitter, and ``<cache>/surface_settings.json``, the file the add-on writes.
+
+The file format, defaults, validation messages and the settings fingerprint are the reference's
+(src/mpm_surface/settings.py:1-64)::
 
     {"version": 1, "emitters": {"<emitter name>": {"enabled", "detail", "smoothing", "iso"}}}
 
-An emitter absent from the file is surfaced with the defaults.
+An emitter the file does not name is surfaced with the defaults.
 """
 
 from __future__ import annotations
@@ -18,11 +20,19 @@ from mpm_core.cache import write_json
 FORMAT_VERSION = 1
 FILE_NAME = "surface_settings.json"
 
+# field -> (test a valid value passes, the accepted range as the error message words it)
+_RANGES = {
+    "detail": (lambda v: v > 0, "> 0"),
+    "smoothing": (lambda v: v > 0, "> 0"),
+    "iso": (lambda v: 0 < v < 1, "in (0, 1)"),
+}
+
 
 @dataclass(frozen=True)
 class SurfaceSettings:
-    """settings.py:21-49. detail: field cells per particle spacing; smoothing: kernel radius in units of two
-    particle spacings; iso: the surface level as a fraction of the rest density."""
+    """``detail``: field cells per particle spacing. ``smoothing``: the kernel radius in units of two particle
+    spacings (below about 1 the surface turns lumpy and thin sheets break up). ``iso``: the surface level as a
+    fraction of the rest density (0.5 hides sparse material at smoothing 2)."""
 
     enabled: bool = True
     detail: float = 1.0
@@ -30,11 +40,9 @@ class SurfaceSettings:
     iso: float = 0.35
 
     def __post_init__(self):
-        checks = (("detail", self.detail > 0, "> 0"), ("smoothing", self.smoothing > 0, "> 0"),
-                  ("iso", 0 < self.iso < 1, "in (0, 1)"))
-        for name, ok, want in checks:
-            if not ok:
-                raise ValueError(f"{name} must be {want}")
+        for name, (ok, words) in _RANGES.items():
+            if not ok(getattr(self, name)):
+                raise ValueError(f"{name} must be {words}")
 
     def to_dict(self) -> dict:
         return asdict(self)
@@ -44,13 +52,16 @@ class SurfaceSettings:
         return cls(**d)
 
     def hash(self) -> str:
-        """A fingerprint of the settings (part of each mesh's key, so changed settings re-mesh)."""
-        return hashlib.sha1(json.dumps(self.to_dict(), sort_keys=True).encode()).hexdigest()
+        """Fingerprint of the settings: SHA-1 of their JSON with sorted keys. It is part of every mesh's key,
+        so a mesh made under other settings is made again."""
+        text = json.dumps(self.to_dict(), sort_keys=True)
+        return hashlib.sha1(text.encode()).hexdigest()
 
 
 def read_settings(cache_dir: Path) -> dict[str, SurfaceSettings]:
-    path = Path(cache_dir) / FILE_NAME
-    if not path.exists():
+    """The settings the file names, by emitter; {} when the cache has no settings file."""
+    path = Path(cache_dir, FILE_NAME)
+    if not path.is_file():
         return {}
     doc = json.loads(path.read_text(encoding="utf-8"))
     if doc.get("version") != FORMAT_VERSION:
@@ -59,5 +70,5 @@ def read_settings(cache_dir: Path) -> dict[str, SurfaceSettings]:
 
 
 def write_settings(cache_dir: Path, settings: dict[str, SurfaceSettings]) -> None:
-    write_json(Path(cache_dir) / FILE_NAME,
-               {"version": FORMAT_VERSION, "emitters": {name: s.to_dict() for name, s in settings.items()}})
+    emitters = {name: s.to_dict() for name, s in settings.items()}
+    write_json(Path(cache_dir, FILE_NAME), {"version": FORMAT_VERSION, "emitters": emitters})

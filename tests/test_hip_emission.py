@@ -1,7 +1,8 @@
 """Emission on the GPU: the HIP solver's sample_mesh_volume (lattice test in FP64 on the device, the
 reference's jitter draw on the host) returns the same array, bit for bit, as the host function the runner
-uses (mpm_core.geometry.sample_mesh_volume, a restatement of the reference's geometry.py:89-155 pinned by
-tests/test_geometry.py): emission counts and positions are part of a bake's result."""
+uses (mpm_core.geometry.sample_mesh_volume, a restatement of the reference's geometry.py:89-155 pinned to the
+reference's own output by tests/test_emission_golden.py): emission counts and positions are part of a bake's
+result."""
 
 import numpy as np
 import pytest
