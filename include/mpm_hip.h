@@ -236,6 +236,25 @@ int32_t mpm_batch_progress(const mpm_batch* batch, int32_t* substeps_done, int32
 #define MPM_BATCH_OPT_P2G_WAVES 7         /* waves per occupied brick in P2G: 1 (default), 2 or 4 */
 int32_t mpm_batch_set_option(mpm_batch* batch, int32_t option, int64_t value);
 int64_t mpm_batch_get_option(const mpm_batch* batch, int32_t option);
+/* kernel classes of a batched substep, for mpm_batch_get_kernel_time */
+#define MPM_BK_PLAN 0       /* k_batch_plan: which scenes run and sort, work prefixes, collider frames */
+#define MPM_BK_SORT 1       /* binning of the scenes that sort: cell keys, radix passes, cell bounds, reorder */
+#define MPM_BK_DRIFT 2      /* drift routing (with the occupied-brick prefix) */
+#define MPM_BK_P2G 3
+#define MPM_BK_GRID_SCAN 4  /* dirty bricks to work lists (with their prefix) */
+#define MPM_BK_GRID 5
+#define MPM_BK_G2P 6
+#define MPM_BK_FIXUP 7      /* the FP64 pass */
+#define MPM_BK_ADVANCE 8    /* end-of-substep bookkeeping */
+#define MPM_BK_COUNT 9
+/* HIP events around every kernel class of the batch's substeps (1: on, 0: off, the default). No counterpart
+   in the reference. While on, substeps are launched directly instead of replayed from the captured graphs
+   (a graph's kernels cannot be bracketed by events): the device time of each class is the same, the host's
+   launch overhead is not. */
+int32_t mpm_batch_set_timing(mpm_batch* batch, int32_t on);
+/* accumulated device time and launch count of one class since the last reset */
+int32_t mpm_batch_get_kernel_time(mpm_batch* batch, int32_t kernel, double* total_ms, int64_t* launches);
+int32_t mpm_batch_reset_timing(mpm_batch* batch);
 
 /* ---- surfacing -----------------------------------------------------------------------------
  * Replaces the device work of surface_particles (src/mpm_surface/field.py:100-142): the density splat

@@ -114,6 +114,23 @@ class SceneBatch:
     def get_option(self, option: int) -> int:
         return int(self._lib.mpm_batch_get_option(self._b, int(option)))
 
+    def set_timing(self, enabled: bool) -> None:
+        """Bracket every kernel class of the batched substeps with HIP events (substeps are then launched
+        directly instead of replayed from the captured graphs)."""
+        N.check(self._lib.mpm_batch_set_timing(self._b, 1 if enabled else 0))
+
+    def reset_timing(self) -> None:
+        N.check(self._lib.mpm_batch_reset_timing(self._b))
+
+    def kernel_times(self) -> dict[str, tuple[float, int]]:
+        """(device milliseconds, launches) per kernel class since the last reset."""
+        out = {}
+        for k, name in enumerate(N.BATCH_KERNEL_NAMES):
+            ms, count = C.c_double(), C.c_int64()
+            N.check(self._lib.mpm_batch_get_kernel_time(self._b, k, C.byref(ms), C.byref(count)))
+            out[name] = (ms.value, count.value)
+        return out
+
     @property
     def substeps_launched(self) -> int:
         return self.get_option(N.BATCH_OPT_SUBSTEPS_LAUNCHED)

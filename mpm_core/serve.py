@@ -20,6 +20,15 @@ The server reports per-frame latency (time between a bake's consecutive finished
 frame, from admission) at p50/p99, the queueing delay (arrival to admission), and the aggregate
 particle-substeps/s. With ``batched=False`` and
 ``max_active=1`` it is the serial runner (one bake at a time, ``Solver.advance``), the baseline.
+
+Latency target. Bakes in one batch share the device, so a bake's frame latency grows with the work of every
+other bake in flight. With ``latency_target_ms`` set, the number of bakes admitted at once is capped and the
+cap follows the measured latency: after every ``LATENCY_WINDOW`` finished frames the p99 of the last
+``LATENCY_WINDOW`` frame latencies is compared with the target; above it the cap drops to
+``LATENCY_BACKOFF`` of the bakes in flight (at least one), below ``LATENCY_HEADROOM`` of it the cap grows by
+an eighth (at least one). The cap starts at ``LATENCY_START_CAP``. Requests beyond the cap wait in the queue
+(their wait is reported apart); bakes already admitted are never stopped, so after a drop the cap bounds
+p99 once the bakes admitted under the higher cap have finished.
 """
 
 from __future__ import annotations
@@ -51,6 +60,10 @@ BYTES_PER_NODE = 2 * 16 + 4
 BYTES_PER_BRICK = 4 + 1 + 4 + 4
 BYTES_PER_OCCUPIED_BRICK = 512 * 16 + 64 * 4 + 64 * 4 * 4
 GROWTH = 1.5  # capacity grows by half when particles are added
+LATENCY_WINDOW = 32     # finished frames between two adjustments of the admission cap
+LATENCY_BACKOFF = 0.7   # the cap after a window whose p99 exceeded the target, as a share of bakes in flight
+LATENCY_HEADROOM = 0.7  # the cap grows after a window whose p99 stayed below this share of the target
+LATENCY_START_CAP = 8   # bakes admitted at once before the first latency window
 
 
 def mesh_volume(verts: np.ndarray, tris: np.ndarray) -> float:
@@ -135,13 +148,19 @@ class _Bake:
 class BakeServer:
     def __init__(self, device: str = "cuda:0", hbm_budget: float = 250e9, max_active: int = 1024,
                  batched: bool = True, chunk: int = 4, checkpoint_every: int = 1, writer_threads: int = 4,
-                 backend: str = "hip"):
+                 backend: str = "hip", latency_target_ms: float | None = None):
         from mpm_core.backends.hip.batch import SceneBatch
         from mpm_core.backends.hip.solver import device_ordinal
 
         self.device = device
         self.hbm_budget = float(hbm_budget)
         self.max_active = int(max_active)
+        self.latency_target = None if latency_target_ms is None else float(latency_target_ms) / 1000.0
+        # admission cap: all of max_active, or with a latency target a small start that then follows it
+        self.cap = self.max_active if latency_target_ms is None else min(self.max_active, LATENCY_START_CAP)
+        self.cap_log: list[tuple[float, int]] = [(time.perf_counter(), self.cap)]
+        self.recent: deque[float] = deque(maxlen=LATENCY_WINDOW)
+        self.since_adjust = 0
         self.batched = bool(batched)
         self.chunk = int(chunk)
         self.checkpoint_every = max(1, int(checkpoint_every))
@@ -247,7 +266,7 @@ class BakeServer:
     # ---- admission ------------------------------------------------------------------------------------
     def _admit(self) -> None:
         added = False
-        while self.queue and len(self.active) < self.max_active:
+        while self.queue and len(self.active) < min(self.max_active, self.cap):
             h = self.queue[0]
             fp = h.request.footprint if h.request.footprint is not None else hbm_footprint(h.request.scene)
             if self.reserved + fp > self.hbm_budget and self.active:
@@ -384,6 +403,7 @@ class BakeServer:
             raise FloatingPointError(f"simulation became unstable at frame {frame} (NaN/inf in particle positions)")
         now = time.perf_counter()
         h.frame_latency.append(now - b.last_output)
+        self._track_latency(now - b.last_output)
         b.last_output = now
         b.timings.append((now - b.frame_t0) * 1000.0)
         self.t_last = now
@@ -404,6 +424,26 @@ class BakeServer:
             b.progress.update(status="running", frame=frame, particles=len(particles.positions), ms_per_frame=ms)
 
         self._chain(b, write)
+
+    def _track_latency(self, seconds: float) -> None:
+        """Adjust the admission cap to the latency target (see the module notes)."""
+        if self.latency_target is None:
+            return
+        self.recent.append(seconds)
+        self.since_adjust += 1
+        if self.since_adjust < LATENCY_WINDOW:
+            return
+        self.since_adjust = 0
+        p99 = float(np.percentile(np.asarray(self.recent), 99))
+        if p99 > self.latency_target:
+            cap = max(1, int(LATENCY_BACKOFF * len(self.active)))
+        elif p99 < LATENCY_HEADROOM * self.latency_target:
+            cap = min(self.max_active, self.cap + max(1, self.cap // 8))
+        else:
+            return
+        if cap != self.cap:
+            self.cap = cap
+            self.cap_log.append((time.perf_counter(), cap))
 
     def _chain(self, b: _Bake, fn) -> None:
         """Run fn on a writer thread after the bake's previous write (its files stay in order)."""
@@ -490,6 +530,9 @@ class BakeServer:
             "particle_substeps_per_s": self.particle_substeps / span if span > 0 else None,
             "substep_launches": self.launches,
             "seconds_in_batch_runs": round(self.run_seconds, 3),
+            "latency_target_ms": None if self.latency_target is None else self.latency_target * 1000.0,
+            "admission_cap": {"last": self.cap, "min": min(c for _, c in self.cap_log),
+                              "max": max(c for _, c in self.cap_log), "changes": len(self.cap_log) - 1},
         }
 
 

@@ -22,6 +22,8 @@ PRECISION_FP32, PRECISION_FALLBACK, PRECISION_FP64 = 0, 1, 2
 (BATCH_OPT_P2G_BLOCKS, BATCH_OPT_GRAPH, BATCH_OPT_SORT_INTERVAL, BATCH_OPT_SUBSTEPS_LAUNCHED, BATCH_OPT_G2P_BLOCKS,
  BATCH_OPT_GRID_BLOCKS, BATCH_OPT_G2P_TILE, BATCH_OPT_P2G_WAVES) = range(8)
 KERNEL_NAMES = ("p2g", "grid", "g2p", "bin", "colliders", "other")
+# kernel classes of a batched substep (MPM_BK_*), in their order
+BATCH_KERNEL_NAMES = ("plan", "sort", "drift", "p2g", "grid_scan", "grid", "g2p", "fixup", "advance")
 
 
 class DomainDesc(C.Structure):
@@ -104,6 +106,9 @@ def _declare(L: C.CDLL) -> None:
         "mpm_batch_progress": ([ctx, ip, ip], i32),
         "mpm_batch_set_option": ([ctx, i32, i64], i32),
         "mpm_batch_get_option": ([ctx, i32], i64),
+        "mpm_batch_set_timing": ([ctx, i32], i32),
+        "mpm_batch_get_kernel_time": ([ctx, i32, C.POINTER(f64), i64p], i32),
+        "mpm_batch_reset_timing": ([ctx], i32),
         "mpm_surface_create": ([i32, C.POINTER(ctx)], i32),
         "mpm_surface_destroy": ([ctx], i32),
         "mpm_surface_build": ([ctx, C.POINTER(SurfaceDesc), i64, fp, fp, i64p, i64p], i32),

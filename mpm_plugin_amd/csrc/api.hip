@@ -94,16 +94,19 @@ struct Scratch {
   }
 };
 
+constexpr int kTimedClasses = 16;  // room for a context's (MPM_K_*) and a batch's (MPM_BK_*) kernel classes
+static_assert(MPM_K_COUNT <= kTimedClasses && MPM_BK_COUNT <= kTimedClasses, "kernel classes");
+
 struct Timing {
-  uint32_t mask = 0;  // kernel classes (1 << MPM_K_*) bracketed with events
+  uint32_t mask = 0;  // kernel classes bracketed with events (bit per class)
   struct Pending {
     int kernel;
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
-  double total_ms[MPM_K_COUNT] = {};
-  int64_t launches[MPM_K_COUNT] = {};
+  double total_ms[kTimedClasses] = {};
+  int64_t launches[kTimedClasses] = {};
 };
 
 // all per-slot particle arrays of one buffer set, in one allocation: x 3, v 3, F 9, Jp 1 float planes,
@@ -266,6 +269,7 @@ struct mpm_batch {
   uint32_t* d_pre = nullptr;  // work-unit prefixes (kPrefixKinds x (scenes + 1))
   float* d_speed = nullptr;
   uint32_t* d_speed_slots = nullptr;  // kSpeedSlots per scene: a substep's G2P speed maxima
+  uint32_t* d_tickets = nullptr;      // kTicketKinds counters, zero between launches
   SceneDev* h_scenes = nullptr;
   SceneStep* h_steps = nullptr;
   float* h_speed = nullptr;
@@ -285,6 +289,8 @@ struct mpm_batch {
   int64_t g2p_tile = 256;
   int64_t p2g_waves = 1;  // waves per occupied brick in P2G (the mix's many bricks fill the machine)
   int64_t use_graph = 1;
+  bool capturing = false;  // batch_enqueue is recording into a graph
+  Timing timing;           // per MPM_BK_* class; while on, substeps are launched directly
   bool fp64_pass = false;  // some scene defers particles to the FP64 pass (part of the captured sequence)
   int64_t sort_interval = 16;  // 0: each context's own (the mix's fast scenes drift more per substep)
   int64_t substeps_launched = 0;
@@ -338,37 +344,54 @@ hipEvent_t pool_event(Timing& t) {
   return e;
 }
 
+// events around the launches of one kernel class, when the class is timed and nothing is being captured
 struct Timed {
-  mpm_ctx* c;
+  Timing* t = nullptr;
+  hipStream_t s = nullptr;
   int kernel;
   hipEvent_t a = nullptr;
-  Timed(mpm_ctx* ctx, int k) : c(ctx), kernel(k) {
-    if (((c->timing.mask >> kernel) & 1u) && !c->capturing) {
-      a = pool_event(c->timing);
-      HIP_CHECK(hipEventRecord(a, c->stream));
+  Timed(Timing& timing, hipStream_t stream, int k, bool capturing) : kernel(k) {
+    if (((timing.mask >> k) & 1u) && !capturing) {
+      t = &timing;
+      s = stream;
+      a = pool_event(timing);
+      HIP_CHECK(hipEventRecord(a, s));
     }
   }
+  Timed(mpm_ctx* c, int k) : Timed(c->timing, c->stream, k, c->capturing) {}
   ~Timed() noexcept(false) {
     if (a) {
-      hipEvent_t b = pool_event(c->timing);
-      HIP_CHECK(hipEventRecord(b, c->stream));
-      c->timing.pending.push_back({kernel, a, b});
+      hipEvent_t b = pool_event(*t);
+      HIP_CHECK(hipEventRecord(b, s));
+      t->pending.push_back({kernel, a, b});
     }
   }
 };
 
-void collect_timing(mpm_ctx* c) {
-  if (c->timing.pending.empty()) return;
-  HIP_CHECK(hipStreamSynchronize(c->stream));
-  for (auto& p : c->timing.pending) {
+void collect_timing(Timing& t, hipStream_t stream) {
+  if (t.pending.empty()) return;
+  HIP_CHECK(hipStreamSynchronize(stream));
+  for (auto& p : t.pending) {
     float ms = 0.0f;
     HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
-    c->timing.total_ms[p.kernel] += ms;
-    c->timing.launches[p.kernel] += 1;
-    c->timing.pool.push_back(p.a);
-    c->timing.pool.push_back(p.b);
+    t.total_ms[p.kernel] += ms;
+    t.launches[p.kernel] += 1;
+    t.pool.push_back(p.a);
+    t.pool.push_back(p.b);
   }
-  c->timing.pending.clear();
+  t.pending.clear();
+}
+
+void collect_timing(mpm_ctx* c) { collect_timing(c->timing, c->stream); }
+
+void release_timing(Timing& t) {
+  for (auto& p : t.pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : t.pool) (void)hipEventDestroy(e);
+  t.pending.clear();
+  t.pool.clear();
 }
 
 void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
@@ -1127,7 +1150,7 @@ void batch_drop_graphs(mpm_batch* b) {
 
 void batch_free_tables(mpm_batch* b) {
   for (void* p : {(void*)b->d_scenes, (void*)b->d_steps, (void*)b->d_frames, (void*)b->d_pre, (void*)b->d_speed,
-                  (void*)b->d_speed_slots})
+                  (void*)b->d_speed_slots, (void*)b->d_tickets})
     dev_free(p);
   for (void* p : {(void*)b->h_scenes, (void*)b->h_steps, (void*)b->h_speed})
     if (p) (void)hipHostFree(p);
@@ -1137,6 +1160,7 @@ void batch_free_tables(mpm_batch* b) {
   b->d_pre = nullptr;
   b->d_speed = nullptr;
   b->d_speed_slots = nullptr;
+  b->d_tickets = nullptr;
   b->h_scenes = nullptr;
   b->h_steps = nullptr;
   b->h_speed = nullptr;
@@ -1165,6 +1189,10 @@ void batch_reserve_tables(mpm_batch* b, int64_t n) {
   b->d_pre = dev_alloc<uint32_t>((size_t)kPrefixKinds * (cap + 1));
   b->d_speed = dev_alloc<float>(cap);
   b->d_speed_slots = dev_alloc<uint32_t>((size_t)cap * kSpeedSlots);
+  b->d_tickets = dev_alloc<uint32_t>(kTicketKinds);
+  // the ticket counters start at zero (ordered before every later launch on the batch stream; never
+  // inside a capture: tables are sized before one) and every kernel that takes tickets leaves them so
+  HIP_CHECK(hipMemsetAsync(b->d_tickets, 0, kTicketKinds * sizeof(uint32_t), b->stream));
   b->h_scenes = host_alloc<SceneDev>(cap);
   b->h_steps = host_alloc<SceneStep>(cap);
   b->h_speed = host_alloc<float>(cap);
@@ -1203,11 +1231,16 @@ unsigned batch_blocks(int64_t option, int64_t automatic) { return (unsigned)(opt
 void batch_enqueue(mpm_batch* b, bool sorting) {
   const int n = (int)b->scenes.size();
   hipStream_t s = b->stream;
+  auto timed = [&](int k) { return Timed(b->timing, s, k, b->capturing); };
   // persistent grids: a few workgroups per CU, striding over the running scenes' work
-  k_batch_plan<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, b->d_frames, n, sorting ? 1 : 0,
-                                          (int)b->g2p_tile, b->d_pre, b->d_speed_slots);
-  check_launch("k_batch_plan");
+  {
+    const Timed t = timed(MPM_BK_PLAN);
+    k_batch_plan<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, b->d_frames, n, sorting ? 1 : 0,
+                                            (int)b->g2p_tile, b->d_pre, b->d_speed_slots);
+    check_launch("k_batch_plan");
+  }
   if (sorting) {
+    const Timed t = timed(MPM_BK_SORT);
     k_batch_cell_keys<<<2048, kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
     check_launch("k_batch_cell_keys");
     uint32_t* hist = static_cast<uint32_t*>(b->hist.ptr);
@@ -1223,10 +1256,13 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
     k_batch_reorder<<<2048, kTile, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
     check_launch("k_batch_reorder");
   }
-  k_batch_prefix<<<1, kPlanThreads, 0, s>>>(b->d_scenes, b->d_steps, n, b->d_pre);
-  check_launch("k_batch_prefix");
-  k_batch_drift<<<2048, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n, (int)b->g2p_tile);
-  check_launch("k_batch_drift");
+  {
+    const Timed t = timed(MPM_BK_DRIFT);
+    k_batch_drift<<<2048, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n, (int)b->g2p_tile, b->d_tickets);
+    check_launch("k_batch_drift");
+  }
+  {
+  const Timed t = timed(MPM_BK_P2G);
   if (b->p2g_waves == 1) {
     k_batch_p2g<1><<<batch_blocks(b->p2g_blocks, 8192), 64, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
   } else if (b->p2g_waves == 2) {
@@ -1235,33 +1271,45 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
     k_batch_p2g_split<4><<<batch_blocks(b->p2g_blocks, 8192), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
   }
   check_launch("k_batch_p2g");
-  k_batch_grid_scan<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
-  check_launch("k_batch_grid_scan");
-  k_batch_dirty_prefix<<<1, kPlanThreads, 0, s>>>(b->d_steps, n, b->d_pre);
-  check_launch("k_batch_dirty_prefix");
-  k_batch_grid<<<batch_blocks(b->grid_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n,
-                                                                  b->d_frames);
-  check_launch("k_batch_grid");
-  k_batch_g2p<<<batch_blocks(b->g2p_blocks, 2048), (unsigned)b->g2p_tile, 0, s>>>(b->d_scenes, b->d_steps,
-                                                                                  b->d_pre, n, b->d_frames,
-                                                                                  b->d_speed_slots);
-  check_launch("k_batch_g2p");
+  }
+  {
+    const Timed t = timed(MPM_BK_GRID_SCAN);
+    k_batch_grid_scan<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n, b->d_tickets);
+    check_launch("k_batch_grid_scan");
+  }
+  {
+    const Timed t = timed(MPM_BK_GRID);
+    k_batch_grid<<<batch_blocks(b->grid_blocks, 2048), 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n,
+                                                                    b->d_frames);
+    check_launch("k_batch_grid");
+  }
+  {
+    const Timed t = timed(MPM_BK_G2P);
+    k_batch_g2p<<<batch_blocks(b->g2p_blocks, 2048), (unsigned)b->g2p_tile, 0, s>>>(
+        b->d_scenes, b->d_steps, b->d_pre, n, b->d_frames, b->d_speed_slots, b->fp64_pass ? 1 : 0, b->d_tickets);
+    check_launch("k_batch_g2p");
+  }
   if (b->fp64_pass) {
-    k_batch_fix_prefix<<<1, kPlanThreads, 0, s>>>(b->d_steps, n, b->d_pre);
-    check_launch("k_batch_fix_prefix");
+    const Timed t = timed(MPM_BK_FIXUP);
     k_batch_fixup<<<1024, 256, 0, s>>>(b->d_scenes, b->d_steps, b->d_pre, n);
     check_launch("k_batch_fixup");
   }
-  k_batch_advance<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(b->d_steps, n, b->d_speed_slots);
-  check_launch("k_batch_advance");
+  {
+    const Timed t = timed(MPM_BK_ADVANCE);
+    k_batch_advance<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(b->d_steps, n, b->d_speed_slots);
+    check_launch("k_batch_advance");
+  }
 }
 
 hipGraphExec_t batch_capture(mpm_batch* b, bool sorting) {
   hipGraph_t graph = nullptr;
   HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+  b->capturing = true;
   try {
     batch_enqueue(b, sorting);
+    b->capturing = false;
   } catch (...) {
+    b->capturing = false;
     (void)hipStreamEndCapture(b->stream, &graph);
     if (graph) (void)hipGraphDestroy(graph);
     throw;
@@ -1451,12 +1499,14 @@ void batch_run(mpm_batch* b, int64_t max_launches, int64_t* launched, int32_t* f
   std::copy(b->step.begin(), b->step.end(), b->h_steps);
   HIP_CHECK(hipMemcpyAsync(b->d_scenes, b->h_scenes, n * sizeof(SceneDev), hipMemcpyHostToDevice, b->stream));
   HIP_CHECK(hipMemcpyAsync(b->d_steps, b->h_steps, n * sizeof(SceneStep), hipMemcpyHostToDevice, b->stream));
-  if (b->use_graph && !b->g_sort) {
+  // graphs unless kernels are being timed (events cannot bracket a graph's kernels)
+  const bool graphs = b->use_graph && b->timing.mask == 0;
+  if (graphs && !b->g_sort) {
     HIP_CHECK(hipStreamSynchronize(b->stream));
     b->g_sort = batch_capture(b, true);
     b->g_plain = batch_capture(b, false);
   }
-  // the host replays the device's bookkeeping (k_batch_plan, k_batch_prefix, k_batch_advance) to pick the
+  // the host replays the device's bookkeeping (k_batch_plan, k_batch_advance) to pick the
   // graph of each substep, and checks it against the device afterwards
   std::vector<SceneStep> sim = b->step;
   std::vector<char> was_running(n);
@@ -1475,7 +1525,7 @@ void batch_run(mpm_batch* b, int64_t max_launches, int64_t* launched, int32_t* f
       t.since_sort += 1;
       t.k += 1;
     }
-    if (b->use_graph) HIP_CHECK(hipGraphLaunch(sorting ? b->g_sort : b->g_plain, b->stream));
+    if (graphs) HIP_CHECK(hipGraphLaunch(sorting ? b->g_sort : b->g_plain, b->stream));
     else batch_enqueue(b, sorting);
   }
   b->substeps_launched += L;
@@ -1760,11 +1810,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   c->emit_scratch.release();
   c->hist.release();
   c->scan_sums.release();
-  for (auto& p : c->timing.pending) {
-    (void)hipEventDestroy(p.a);
-    (void)hipEventDestroy(p.b);
-  }
-  for (auto e : c->timing.pool) (void)hipEventDestroy(e);
+  release_timing(c->timing);
   if (c->h_vmax) (void)hipHostFree(c->h_vmax);
   if (c->vmax_done) (void)hipEventDestroy(c->vmax_done);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -2349,6 +2395,7 @@ int32_t mpm_batch_destroy(mpm_batch* b) {
   batch_free_tables(b);
   b->hist.release();
   b->scan_sums.release();
+  release_timing(b->timing);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   (void)hipSetDevice(prev);
   delete b;
@@ -2449,6 +2496,34 @@ int64_t mpm_batch_get_option(const mpm_batch* b, int32_t option) {
     default:
       return -1;
   }
+}
+
+int32_t mpm_batch_set_timing(mpm_batch* b, int32_t on) {
+  if (!b || (on != 0 && on != 1)) return MPM_ERR_VALUE;
+  return run_on(b->device, [&] {
+    collect_timing(b->timing, b->stream);
+    b->timing.mask = on ? (uint32_t)((1 << MPM_BK_COUNT) - 1) : 0u;
+  });
+}
+
+int32_t mpm_batch_get_kernel_time(mpm_batch* b, int32_t kernel, double* total_ms, int64_t* launches) {
+  if (!b || kernel < 0 || kernel >= MPM_BK_COUNT || !total_ms || !launches) return MPM_ERR_VALUE;
+  return run_on(b->device, [&] {
+    collect_timing(b->timing, b->stream);
+    *total_ms = b->timing.total_ms[kernel];
+    *launches = b->timing.launches[kernel];
+  });
+}
+
+int32_t mpm_batch_reset_timing(mpm_batch* b) {
+  if (!b) return MPM_ERR_VALUE;
+  return run_on(b->device, [&] {
+    collect_timing(b->timing, b->stream);
+    for (int k = 0; k < kTimedClasses; ++k) {
+      b->timing.total_ms[k] = 0.0;
+      b->timing.launches[k] = 0;
+    }
+  });
 }
 
 }  // extern "C"

@@ -20,11 +20,18 @@
 // One substep (semantics of solver.py:176-211 per scene):
 //   k_batch_plan            per scene: runs?, frame fraction s, sort now?, work prefixes, collider frames
 //   [k_batch_cell_keys, radix passes, k_batch_cell_bounds, k_batch_reorder]   only in the sorting graph
-//   k_batch_prefix          switches scenes that sorted to their reordered set, occupied-brick prefix
-//   k_batch_drift           scenes that did not sort: drifted particles to their current cell's lane
-//   k_batch_p2g, k_batch_grid_scan, k_batch_dirty_prefix, k_batch_grid, k_batch_g2p,
-//   [k_batch_fix_prefix, k_batch_fixup]   when a scene defers particles to the FP64 pass
-//   k_batch_advance
+//   k_batch_drift           scenes that did not sort: drifted particles to their current cell's lane;
+//                           its last workgroup then scans the running scenes' occupied-brick counts
+//   k_batch_p2g             over those bricks
+//   k_batch_grid_scan       dirty bricks to per-scene work lists; its last workgroup scans their counts
+//   k_batch_grid
+//   k_batch_g2p             its last workgroup scans the counts of particles deferred to the FP64 pass
+//   [k_batch_fixup]         when a scene's precision has an FP64 pass
+//   k_batch_advance         bookkeeping; a scene that sorted switches to its reordered set here
+// A prefix needed between two kernels is computed by whichever workgroup of the first kernel finishes
+// last (a ticket counter tells it), not by a kernel of its own: one launch and one drain fewer each.
+// Between the sort and k_batch_advance a scene's live particle set is live_set(): the reordered one if
+// it sorted in this substep.
 #pragma once
 
 #include "binning.hpp"
@@ -78,6 +85,10 @@ struct SceneStep {
   uint32_t n_fix;       // entries of fix_list this substep
 };
 
+// the particle set the kernels after the sort work on: a scene that sorted in this substep has its
+// particles in the reordered set (k_batch_advance makes that the current one)
+__device__ __forceinline__ int live_set(const SceneStep& T) { return T.cur ^ T.sort_now; }
+
 // work-unit prefixes, each n_scenes + 1 entries, in one array: pre + q * (n_scenes + 1)
 enum PrefixKind {
   kPreTiles = 0,       // particle tiles of running scenes (G2P; the tile is the G2P workgroup size)
@@ -108,6 +119,51 @@ __device__ __forceinline__ int scene_of(const uint32_t* __restrict__ pre, int n_
 }
 
 __device__ __forceinline__ uint32_t ceil_div_u(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// Called by every thread of every workgroup of a kernel at its end (no workgroup may have returned
+// early): true, in all threads of one workgroup, for the workgroup that finishes last. `done` counts
+// tickets; it starts at 0 and the last workgroup resets it for the kernel's next launch. Nothing waits: a
+// workgroup that is not last just ends. `s_flag` is a shared word used for nothing else.
+// What the last workgroup may read of the other workgroups' work: counters they updated with returning
+// agent-scope atomics (each wave uses the returned value, so the update is performed before the wave
+// reaches the barrier below, hence before its workgroup's ticket), read back with load_agent. There is
+// no device-scope fence: on gfx950 one writes back the XCD's L2, and a fence per wave of a persistent
+// grid cost hundreds of microseconds per substep. Plain stores of this kernel reach the next kernel
+// through the kernel boundary as usual.
+__device__ __forceinline__ bool last_workgroup(uint32_t* done, uint32_t* s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const bool last = atomicAdd(done, 1u) == gridDim.x - 1;
+    if (last) atomicExch(done, 0u);
+    *s_flag = last ? 1u : 0u;
+  }
+  __syncthreads();
+  return *s_flag != 0u;
+}
+
+// a counter updated by other workgroups' agent-scope atomics, read at the coherence point those use
+__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Whole workgroup: pre[0..n_scenes] = exclusive prefix of count(i) over the scenes (pre[n_scenes] is the
+// total). Any workgroup size that is a multiple of 64.
+template <typename Count>
+__device__ void workgroup_prefix(uint32_t* __restrict__ pre, int n_scenes, uint32_t* s_warp, Count count) {
+  uint32_t carry = 0;
+  for (int base = 0; base < n_scenes; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    const uint32_t c = i < n_scenes ? count(i) : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(c, s_warp, total);
+    if (i < n_scenes) pre[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) pre[n_scenes] = carry;
+}
+
+// ticket counters of the kernels whose last workgroup computes a prefix
+enum TicketKind { kTicketDrift = 0, kTicketGridScan = 1, kTicketG2p = 2, kTicketKinds = 3 };
 
 // radix pass `pass` reads keys0/vals2 (pass 0), keys1/vals1 (odd) or keys2/vals2 (even) and writes the
 // other pair; the sort's result is the output of pass passes-1
@@ -387,38 +443,15 @@ __global__ void __launch_bounds__(kTile) k_batch_reorder(const SceneDev* __restr
   }
 }
 
-// Single workgroup: scenes that sorted switch to the reordered set; prefix of the running scenes'
-// occupied bricks.
-__global__ void __launch_bounds__(kPlanThreads) k_batch_prefix(const SceneDev* __restrict__ sd,
-                                                               SceneStep* __restrict__ st, int n_scenes,
-                                                               uint32_t* __restrict__ pre) {
-  __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
-  uint32_t* P0 = pre + kPreBricks * (n_scenes + 1);
-  uint32_t carry = 0;
-  for (int base = 0; base < n_scenes; base += kPlanThreads) {
-    const int i = base + threadIdx.x;
-    uint32_t count = 0;
-    if (i < n_scenes) {
-      SceneStep& T = st[i];
-      if (T.sort_now) {
-        T.cur = 1 - T.cur;
-        T.sorted = 1;
-        T.since_sort = 0;
-      }
-      if (T.run) count = *sd[i].n_active;
-    }
-    uint32_t total;
-    const uint32_t ex = block_exclusive_scan(count, s_warp, total);
-    if (i < n_scenes) P0[i] = carry + ex;
-    carry += total;
-  }
-  if (threadIdx.x == 0) P0[n_scenes] = carry;
-}
-
-// drift routing (k_drift) of the running scenes that did not sort this substep, over their G2P tiles
+// drift routing (k_drift) of the running scenes that did not sort this substep, over their G2P tiles;
+// the last workgroup then writes the prefix of the running scenes' occupied bricks (final now: the
+// sorting graph's binning kernels have run) for P2G
 __global__ void __launch_bounds__(256) k_batch_drift(const SceneDev* __restrict__ sd,
                                                      const SceneStep* __restrict__ st,
-                                                     const uint32_t* __restrict__ pre, int n_scenes, int tile) {
+                                                     uint32_t* __restrict__ pre, int n_scenes, int tile,
+                                                     uint32_t* __restrict__ tickets) {
+  __shared__ uint32_t s_warp[256 / 64 + 1];
+  __shared__ uint32_t s_last;
   const uint32_t* P0 = pre + kPreTiles * (n_scenes + 1);
   const uint32_t total = P0[n_scenes];
   int i = 0;
@@ -431,6 +464,9 @@ __global__ void __launch_bounds__(256) k_batch_drift(const SceneDev* __restrict_
     for (int p = (int)(u - P0[i]) * tile + threadIdx.x; p < end; p += blockDim.x)
       drift_route(P, S.g, p, sorted_keys(S), S.slot, S.in_cnt, S.in_lst, S.fwd);
   }
+  if (last_workgroup(tickets + kTicketDrift, &s_last))
+    workgroup_prefix(pre + kPreBricks * (n_scenes + 1), n_scenes, s_warp,
+                     [&](int j) { return st[j].run ? *sd[j].n_active : 0u; });
 }
 
 // P2G over every running scene's occupied bricks, one wave per brick (p2g_brick)
@@ -447,7 +483,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_batch_p2g(const SceneDev* __res
   for (uint32_t w = blockIdx.x * kWaves + wave; w < total; w += gridDim.x * kWaves) {
     i = uniform(scene_of(P0, n_scenes, w, i));
     const SceneDev& S = sd[i];
-    const int cur = st[i].cur;
+    const int cur = live_set(st[i]);
     const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
     // a scene that sorted this substep has no drifted particles (and k_batch_drift skipped it)
     const DriftLists dl = st[i].sort_now ? DriftLists{} : DriftLists{S.in_cnt, S.in_lst, S.fwd};
@@ -470,7 +506,7 @@ __global__ void __launch_bounds__(64 * kSplit) k_batch_p2g_split(const SceneDev*
   for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
     i = uniform(scene_of(P0, n_scenes, w, i));
     const SceneDev& S = sd[i];
-    const int cur = st[i].cur;
+    const int cur = live_set(st[i]);
     const float k = -st[i].dt * S.sc.p_vol * 4.0f * S.g.inv_dx * S.g.inv_dx;
     const DriftLists dl = st[i].sort_now ? DriftLists{} : DriftLists{S.in_cnt, S.in_lst, S.fwd};
     p2g_brick<kSplit>(S.set[cur], S.g, k, S.offsets, S.active[w - P0[i]], s_win, S.accum, S.dirty, lane,
@@ -480,9 +516,13 @@ __global__ void __launch_bounds__(64 * kSplit) k_batch_p2g_split(const SceneDev*
 
 // Grid update, first pass: one wave per group of 64 bricks of a running scene; each lane reads one
 // brick's dirty flag, records whether the brick holds a fresh velocity, clears the flag, and the dirty
-// bricks are appended to the scene's work list (one atomic per wave).
+// bricks are appended to the scene's work list (one atomic per wave). The last workgroup then writes
+// the prefix of the running scenes' dirty-brick counts for the second pass.
 __global__ void __launch_bounds__(256) k_batch_grid_scan(const SceneDev* __restrict__ sd, SceneStep* __restrict__ st,
-                                                         const uint32_t* __restrict__ pre, int n_scenes) {
+                                                         uint32_t* __restrict__ pre, int n_scenes,
+                                                         uint32_t* __restrict__ tickets) {
+  __shared__ uint32_t s_warp[256 / 64 + 1];
+  __shared__ uint32_t s_last;
   const uint32_t* P0 = pre + kPreGroups * (n_scenes + 1);
   const uint32_t total = P0[n_scenes];
   const int wave = uniform(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -505,23 +545,9 @@ __global__ void __launch_bounds__(256) k_batch_grid_scan(const SceneDev* __restr
     base = __shfl(base, 0, 64);
     if (d) S.dirty_list[base + (uint32_t)__popcll(mask & lt_mask)] = (uint32_t)brick;
   }
-}
-
-// single workgroup: prefix of the running scenes' dirty-brick counts
-__global__ void __launch_bounds__(kPlanThreads) k_batch_dirty_prefix(const SceneStep* __restrict__ st, int n_scenes,
-                                                                     uint32_t* __restrict__ pre) {
-  __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
-  uint32_t* P0 = pre + kPreDirty * (n_scenes + 1);
-  uint32_t carry = 0;
-  for (int base = 0; base < n_scenes; base += kPlanThreads) {
-    const int i = base + threadIdx.x;
-    const uint32_t count = (i < n_scenes && st[i].run) ? st[i].n_dirty : 0u;
-    uint32_t total;
-    const uint32_t ex = block_exclusive_scan(count, s_warp, total);
-    if (i < n_scenes) P0[i] = carry + ex;
-    carry += total;
-  }
-  if (threadIdx.x == 0) P0[n_scenes] = carry;
+  if (last_workgroup(tickets + kTicketGridScan, &s_last))
+    workgroup_prefix(pre + kPreDirty * (n_scenes + 1), n_scenes, s_warp,
+                     [&](int j) { return st[j].run ? load_agent(&st[j].n_dirty) : 0u; });
 }
 
 // Grid update, second pass: one wave per dirty brick of every running scene (grid_brick)
@@ -542,21 +568,25 @@ __global__ void __launch_bounds__(256) k_batch_grid(const SceneDev* __restrict__
 }
 
 // G2P of every running scene (g2p_particle), one workgroup per tile of blockDim.x particles (64, 128 or
-// 256: a launch-shape choice); each tile's speed maximum goes to the scene's record
+// 256: a launch-shape choice); each tile's speed maximum goes to the scene's record. With an FP64 pass
+// the last workgroup then writes the prefix of the running scenes' deferred-particle counts.
 __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ sd, SceneStep* __restrict__ st,
-                                                   const uint32_t* __restrict__ pre, int n_scenes,
+                                                   uint32_t* __restrict__ pre, int n_scenes,
                                                    const ColliderFrame* __restrict__ frames,
-                                                   uint32_t* __restrict__ speed_slots) {
+                                                   uint32_t* __restrict__ speed_slots, int fp64_pass,
+                                                   uint32_t* __restrict__ tickets) {
   const uint32_t* P0 = pre + kPreTiles * (n_scenes + 1);
   const uint32_t total = P0[n_scenes];
   const int tile = (int)blockDim.x, waves = tile >> 6;
   __shared__ float s_max[4];
+  __shared__ uint32_t s_warp[256 / 64 + 1];
+  __shared__ uint32_t s_last;
   int i = 0;
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     i = uniform(scene_of(P0, n_scenes, u, i));
     const SceneDev& S = sd[i];
     const int p = (int)(u - P0[i]) * tile + threadIdx.x;
-    const int cur = st[i].cur;
+    const int cur = live_set(st[i]);
     Particles P = S.set[cur];
     float speed = 0.0f;
     bool defer = false;
@@ -580,23 +610,9 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ 
     }
     __syncthreads();
   }
-}
-
-// single workgroup: prefix of the running scenes' deferred-particle counts
-__global__ void __launch_bounds__(kPlanThreads) k_batch_fix_prefix(const SceneStep* __restrict__ st, int n_scenes,
-                                                                   uint32_t* __restrict__ pre) {
-  __shared__ uint32_t s_warp[kPlanThreads / 64 + 1];
-  uint32_t* P0 = pre + kPreFix * (n_scenes + 1);
-  uint32_t carry = 0;
-  for (int base = 0; base < n_scenes; base += kPlanThreads) {
-    const int i = base + threadIdx.x;
-    const uint32_t count = (i < n_scenes && st[i].run) ? st[i].n_fix : 0u;
-    uint32_t total;
-    const uint32_t ex = block_exclusive_scan(count, s_warp, total);
-    if (i < n_scenes) P0[i] = carry + ex;
-    carry += total;
-  }
-  if (threadIdx.x == 0) P0[n_scenes] = carry;
+  if (fp64_pass && last_workgroup(tickets + kTicketG2p, &s_last))
+    workgroup_prefix(pre + kPreFix * (n_scenes + 1), n_scenes, s_warp,
+                     [&](int j) { return st[j].run ? load_agent(&st[j].n_fix) : 0u; });
 }
 
 // the FP64 pass over every running scene's deferred particles (k_fp64_fixup)
@@ -608,18 +624,23 @@ __global__ void __launch_bounds__(256) k_batch_fixup(const SceneDev* __restrict_
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
     i = scene_of(P0, n_scenes, u, i);
     const SceneDev& S = sd[i];
-    const int cur = st[i].cur;
-    Particles P = S.set[cur];
+    Particles P = S.set[live_set(st[i])];
     const int64_t p = S.fix_list[u - P0[i]];
     material_update_fp64(S.mats[P.mat[p]], P, p);
   }
 }
 
-// end of a substep: bookkeeping, and the speed maximum of the scene's G2P from its slots
+// end of a substep: bookkeeping (a scene that sorted now lives in its reordered set), and the speed
+// maximum of the scene's G2P from its slots
 __global__ void __launch_bounds__(256) k_batch_advance(SceneStep* __restrict__ st, int n_scenes,
                                                        const uint32_t* __restrict__ speed_slots) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_scenes || !st[i].run) return;
+  if (st[i].sort_now) {
+    st[i].cur = 1 - st[i].cur;
+    st[i].sorted = 1;
+    st[i].since_sort = 0;
+  }
   st[i].since_sort += 1;
   st[i].k += 1;
   uint32_t m = 0u;

@@ -124,6 +124,37 @@ def test_graph_replays_match_direct_launches():
         assert_aggregates_close(a, b)
 
 
+def test_timed_substeps_match_graph_replays():
+    """With kernel timing on the batch launches its substeps directly: same substep counts and states as
+    the graph replays, and every kernel class timed once per substep (the binning once per sorting
+    substep)."""
+    specs = [dict(model="elastic", n_colliders=1), dict(model="snow", n_colliders=0),
+             dict(model="von_mises", n_colliders=2)]
+    runs = []
+    for timed in (False, True):
+        pairs = [make(i, **kw) for i, kw in enumerate(specs)]
+        batch = SceneBatch([s for _, s in pairs])
+        batch.set_timing(timed)
+        before = batch.substeps_launched
+        for f in range(2):
+            for w, s in pairs:
+                set_poses(w, s, f)
+            counts = [st.substeps for st in batch.advance([w.frame_dt for w, _ in pairs])]
+        launched = batch.substeps_launched - before
+        times = batch.kernel_times()
+        runs.append(([s.save_state() for _, s in pairs], counts, launched, times))
+    (g_states, g_counts, g_launched, g_times), (t_states, t_counts, t_launched, t_times) = runs
+    assert g_counts == t_counts and g_launched == t_launched
+    assert all(n == 0 for _, n in g_times.values())  # nothing is timed while timing is off
+    print({k: (round(ms, 3), n) for k, (ms, n) in t_times.items()}, flush=True)
+    for name in ("plan", "drift", "p2g", "grid_scan", "grid", "g2p", "fixup", "advance"):
+        assert t_times[name][1] == t_launched and t_times[name][0] > 0, name
+    assert 0 < t_times["sort"][1] < t_launched
+    assert rel_err(g_states[0]["x"], t_states[0]["x"]) <= 1e-4
+    for a, b in zip(g_states, t_states):
+        assert_aggregates_close(a, b)
+
+
 def test_members_change_between_frames():
     """Emission into a member (capacity growth), a skipped member (frame_dt 0), a member leaving, and a
     member closed while in the batch."""

@@ -172,3 +172,42 @@ def test_batched_server_matches_the_runner(tmp_path):
                 ke_a, ke_b = (va.astype(np.float64) ** 2).sum(), (vb.astype(np.float64) ** 2).sum()
                 assert abs(ke_a - ke_b) <= 0.01 * ke_b + 1e-9
         assert progress(tmp_path / ("srv_" + h.request.name))["status"] == "done"
+
+
+def test_latency_target_moves_the_admission_cap():
+    """The cap starts small, drops to LATENCY_BACKOFF of the bakes in flight after a window whose p99 is over
+    the target, and grows by an eighth (at least one) after a window well under it."""
+    from mpm_core import serve
+
+    server = BakeServer(batched=False, backend="recording", max_active=64, latency_target_ms=100.0)
+    assert server.cap == serve.LATENCY_START_CAP
+    server.active = [object()] * 10  # ten bakes in flight
+    for _ in range(serve.LATENCY_WINDOW - 1):
+        server._track_latency(0.25)
+    assert server.cap == serve.LATENCY_START_CAP  # no decision before a full window
+    server._track_latency(0.25)
+    assert server.cap == int(serve.LATENCY_BACKOFF * 10)
+    for _ in range(serve.LATENCY_WINDOW):
+        server._track_latency(0.08)  # between the headroom and the target: the cap holds
+    assert server.cap == int(serve.LATENCY_BACKOFF * 10)
+    for _ in range(serve.LATENCY_WINDOW):
+        server._track_latency(0.01)
+    assert server.cap == int(serve.LATENCY_BACKOFF * 10) + 1
+    server.active = []
+    report = server.report([])
+    assert report["latency_target_ms"] == 100.0 and report["admission_cap"]["changes"] == 2
+    server.close()
+
+
+def test_admission_cap_holds_requests_in_the_queue(tmp_path):
+    server = BakeServer(batched=False, backend="recording", max_active=64, latency_target_ms=1e6)
+    server.cap = 2
+    handles = [server.submit(BakeRequest(small_scene(frame_end=3), cache_dir=write_cache(tmp_path, f"c{i}",
+                                                                                     small_scene(frame_end=3))))
+               for i in range(5)]
+    server._admit()
+    assert len(server.active) == 2 and len(server.queue) == 3
+    while not server.idle:
+        server.step()
+    server.close()
+    assert all(h.status == "done" for h in handles)
