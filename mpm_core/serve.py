@@ -53,12 +53,12 @@ from mpm_core.solver import ColliderGeometry, ParticleData
 
 # device bytes per particle slot (two particle sets of 32 float + 2 int planes, emitter ids, 5 sort
 # arrays, the FP64 work list, the drift-routing flag), per grid node (two float4 arrays, cell offsets),
-# per brick (flags, occupied list, slot map) and per occupied brick (its staged 8^3 P2G window and the
+# per brick (flags, occupied list, slot map) and per occupied brick (the staged 6^3 core of its P2G window and the
 # drift-routing counters and lists of its 64 cells); a scene holds min(bricks, particle slots) of those
 BYTES_PER_SLOT = 2 * 34 * 4 + 4 + 6 * 4 + 1
 BYTES_PER_NODE = 2 * 16 + 4
 BYTES_PER_BRICK = 4 + 1 + 4 + 4
-BYTES_PER_OCCUPIED_BRICK = 512 * 16 + 64 * 4 + 64 * 4 * 4
+BYTES_PER_OCCUPIED_BRICK = 216 * 16 + 64 * 4 + 64 * 4 * 4
 GROWTH = 1.5  # capacity grows by half when particles are added
 LATENCY_WINDOW = 32     # finished frames between two adjustments of the admission cap
 LATENCY_BACKOFF = 0.7   # the cap after a window whose p99 exceeded the target, as a share of bakes in flight

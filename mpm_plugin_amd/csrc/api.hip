@@ -470,7 +470,7 @@ void reserve(mpm_ctx* c, int64_t need) {
     for (void* p : {(void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst}) dev_free(p);
     c->stage = nullptr;
     c->in_cnt = c->in_lst = nullptr;
-    c->stage = dev_alloc<float4>((size_t)stage_need * kWinNodes);
+    c->stage = dev_alloc<float4>((size_t)stage_need * kStageNodes);
     c->in_cnt = dev_alloc<uint32_t>((size_t)stage_need * 64);
     c->in_lst = dev_alloc<uint32_t>((size_t)stage_need * 64 * kInPerCell);
     HIP_CHECK(hipMemsetAsync(c->in_cnt, 0, (size_t)stage_need * 64 * sizeof(uint32_t), c->stream));

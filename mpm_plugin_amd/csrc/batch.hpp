@@ -581,6 +581,7 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ 
   __shared__ float s_max[4];
   __shared__ uint32_t s_warp[256 / 64 + 1];
   __shared__ uint32_t s_last;
+  __shared__ float s_c[9][256];  // each thread's C across its SVD (g2p_particle)
   int i = 0;
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     i = uniform(scene_of(P0, n_scenes, u, i));
@@ -591,7 +592,8 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const SceneDev* __restrict__ 
     float speed = 0.0f;
     bool defer = false;
     if (p < S.n)
-      defer = g2p_particle(P, S.mats, S.g, S.sc, st[i].dt, S.vel, p, S.cs, frames + i * kMaxColliders, speed);
+      defer = g2p_particle(P, S.mats, S.g, S.sc, st[i].dt, S.vel, p, S.cs, frames + i * kMaxColliders, speed,
+                           &s_c[0][threadIdx.x], 256);
     if (S.sc.precision != kPrecisionFp32) wave_append(FixList{S.fix_list, &st[i].n_fix}, defer, (uint32_t)p);
     if (!(speed == speed)) speed = __int_as_float(0x7f800000);
     const float wm = wave_max(speed);

@@ -21,6 +21,12 @@ namespace mpm {
 
 constexpr int kWin = 8;
 constexpr int kWinNodes = kWin * kWin * kWin;
+// The staged part of a window: its core, window indices 1..6 per axis (nodes 4B .. 4B + 5 of brick B).
+// A particle binned in brick B has its stencil base in the brick's cells 4B .. 4B + 3, so its 3 nodes
+// per axis lie in the core; only a drifted particle scattered by its holder could reach the outer planes,
+// and such a particle goes to the HBM accumulator instead (p2g_scatter_atomic).
+constexpr int kCore = 6;
+constexpr int kStageNodes = kCore * kCore * kCore;
 
 // Optional fixed-point P2G accumulator: 4 int64 per node (momentum xyz, mass) scaled by `scale`. Every
 // contribution reaching it is rounded to an integer first and integer addition is associative, so the
@@ -101,9 +107,10 @@ __device__ __forceinline__ void p2g_scatter_atomic(const P2GParticle& q, const G
                                                 float (*win)[kWinNodes], float4* __restrict__ accum,
                                                 uint32_t* __restrict__ dirty, const FixedAccum& fa) {
   const int l0 = q.base[0] - wo[0], l1 = q.base[1] - wo[1], l2 = q.base[2] - wo[2];
-  // fixed point: straight to the integer grid (the window's float atomics would not add in a fixed order)
-  const bool in_window = fa.fx == nullptr && l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 <= kWin - 3 &&
-                         l1 <= kWin - 3 && l2 <= kWin - 3;
+  // in the window when the whole stencil lies in its staged core (window indices 1 .. kCore); fixed
+  // point: straight to the integer grid (the window's float atomics would not add in a fixed order)
+  const bool in_window = fa.fx == nullptr && l0 >= 1 && l1 >= 1 && l2 >= 1 && l0 <= kCore - 2 &&
+                         l1 <= kCore - 2 && l2 <= kCore - 2;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       const float wij = q.wx[0][i] * q.wx[1][j];
@@ -348,11 +355,14 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
     return v;
   };
   if (stage && !fa.fx) {
-    // the window goes to the brick's staging slot with plain stores; the grid update sums the (up to 8)
-    // windows that cover each node, in a fixed order
-    float4* out = stage + (int64_t)slot * kWinNodes;
-    for (int i = tid; i < kWinNodes; i += 64 * kSplit)
-      out[i] = make_float4(node_value(i, 0), node_value(i, 1), node_value(i, 2), node_value(i, 3));
+    // the window's core goes to the brick's staging slot with plain stores; the grid update sums the (1 to
+    // 8) cores that cover each node, in a fixed order
+    float4* out = stage + (int64_t)slot * kStageNodes;
+    for (int c = tid; c < kStageNodes; c += 64 * kSplit) {
+      const int x = c / (kCore * kCore), y = (c / kCore) % kCore, z = c % kCore;
+      const int i = ((x + 1) * kWin + (y + 1)) * kWin + (z + 1);
+      out[c] = make_float4(node_value(i, 0), node_value(i, 1), node_value(i, 2), node_value(i, 3));
+    }
   } else for (int i = tid; i < kWinNodes; i += 64 * kSplit) {
     const int gi = wo[0] + (i >> 6), gj = wo[1] + ((i >> 3) & 7), gk = wo[2] + (i & 7);
     if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
@@ -372,7 +382,8 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
     if (vz != 0.0f) atomicAdd(&node->z, vz);
     if (vm != 0.0f) atomicAdd(&node->w, vm);
   }
-  if (half == 0 && lane < 27) mark_brick(g, dirty, bx - 1 + lane / 9, by - 1 + (lane / 3) % 3, bz - 1 + lane % 3);
+  // the core spans this brick and the next one along each axis
+  if (half == 0 && lane < 8) mark_brick(g, dirty, bx + (lane >> 2), by + ((lane >> 1) & 1), bz + (lane & 1));
   if (kSplit > 1) __syncthreads();  // every window read before the next brick clears them
   else wave_lds_fence();
 }
@@ -415,27 +426,28 @@ __global__ void __launch_bounds__(64 * kSplit) k_p2g_split(Particles P, GridGeom
 // grid update of one dirty brick by one wave (lane = node): momentum -> velocity, gravity, walls,
 // collider contact; clears the accumulator behind itself
 // P2G result of node (i, j, k): what reached the HBM accumulator (drifted particles) plus, when windows
-// are staged, the windows of the occupied bricks whose 8^3 window covers the node. Window B' spans nodes
-// 4B' - 1 .. 4B' + 6 per axis, so node 4B + r is covered by bricks B - 1 and B (r <= 2) or B and B + 1
-// (r = 3); the sum runs in that fixed order.
+// are staged, the staged cores of the occupied bricks that cover the node. Core B' spans nodes 4B' .. 4B' + 5
+// per axis, so node 4B + r is covered by bricks B - 1 and B (r <= 1) or by B alone (r >= 2); the sum runs
+// over bricks in increasing order.
 __device__ __forceinline__ float4 staged_sum(const GridGeom& g, int i, int j, int k, float4 a,
                                              const float4* __restrict__ stage, const int32_t* __restrict__ slot) {
   const int n[3] = {i, j, k};
-  int b0[3];
+  int b0[3], nb[3];
 #pragma unroll
-  for (int ax = 0; ax < 3; ++ax) b0[ax] = (n[ax] >> 2) - ((n[ax] & 3) <= 2 ? 1 : 0);
-#pragma unroll
-  for (int dx = 0; dx < 2; ++dx)
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-      for (int dz = 0; dz < 2; ++dz) {
+  for (int ax = 0; ax < 3; ++ax) {
+    const bool two = (n[ax] & 3) <= 1;
+    b0[ax] = (n[ax] >> 2) - (two ? 1 : 0);
+    nb[ax] = two ? 2 : 1;
+  }
+  for (int dx = 0; dx < nb[0]; ++dx)
+    for (int dy = 0; dy < nb[1]; ++dy)
+      for (int dz = 0; dz < nb[2]; ++dz) {
         const int bx = b0[0] + dx, by = b0[1] + dy, bz = b0[2] + dz;
         if (bx < 0 || by < 0 || bz < 0 || bx >= g.bdims[0] || by >= g.bdims[1] || bz >= g.bdims[2]) continue;
         const int32_t s = slot[(bx * g.bdims[1] + by) * g.bdims[2] + bz];
         if (s < 0) continue;
-        const int li = ((i - 4 * bx + 1) * kWin + (j - 4 * by + 1)) * kWin + (k - 4 * bz + 1);
-        const float4 w = stage[(int64_t)s * kWinNodes + li];
+        const int li = ((i - 4 * bx) * kCore + (j - 4 * by)) * kCore + (k - 4 * bz);
+        const float4 w = stage[(int64_t)s * kStageNodes + li];
         a.x += w.x;
         a.y += w.y;
         a.z += w.z;
@@ -622,10 +634,15 @@ __global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* _
   }
 }
 
+// c_stash: this thread's 9 LDS words, `stride` apart (one column per thread of the workgroup: no bank
+// conflicts). The new C waits there across the SVD instead of in registers, so the 64-byte record is written
+// whole at the end: written a float4 at a time at two points of the kernel, its lines left the L2 partly
+// written and cost about a quarter more HBM write traffic.
 __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restrict__ mats, const GridGeom& g,
                                              const StepConsts& sc, float dt, const float4* __restrict__ vel,
                                              int64_t p, const ColliderSet& cs,
-                                             const ColliderFrame* frames, float& speed) {
+                                             const ColliderFrame* frames, float& speed, float* c_stash,
+                                             int stride) {
   const int64_t cap = P.cap;
   const float xo[3] = {P.x[p], P.x[cap + p], P.x[2 * cap + p]};
   float fx[3];
@@ -693,10 +710,8 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) nC.a[a][b] = c4dx * (M[a][b] - (fx[b] - 1.0f) * nv[a]);
-  // the record's C part now (only C22 stays live: it shares a float4 with the stress written last)
-  P.rec[4 * p] = make_float4(nC.a[0][0], nC.a[0][1], nC.a[0][2], nC.a[1][0]);
-  P.rec[4 * p + 1] = make_float4(nC.a[1][1], nC.a[1][2], nC.a[2][0], nC.a[2][1]);
-  const float c22 = nC.a[2][2];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) c_stash[e * stride] = nC.a[e / 3][e % 3];
   float nx[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -728,17 +743,21 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
   float sig[3];
   svd3(Fn, U, sig, V);
   float t[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if (needs_fp64(sc.precision, sig, m.model == kSnow ? P.Jp[p] : 1.0f)) {
+  const bool defer = needs_fp64(sc.precision, sig, m.model == kSnow ? P.Jp[p] : 1.0f);
+  if (defer) {
     // deferred to the FP64 pass (k_fp64_fixup), which reads Fn back from F and writes the stress
     store_m3(P.F, cap, p, Fn);
-    P.rec[4 * p + 2] = make_float4(c22, 0.0f, 0.0f, 0.0f);
-    P.rec[4 * p + 3] = make_float4(0.0f, 0.0f, 0.0f, m.p_mass);
-    return true;
+  } else {
+    material_update(m, Fn, U, sig, V, P, t, p);
   }
-  material_update(m, Fn, U, sig, V, P, t, p);
-  P.rec[4 * p + 2] = make_float4(c22, t[0], t[1], t[2]);
+  float c[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) c[e] = c_stash[e * stride];
+  P.rec[4 * p] = make_float4(c[0], c[1], c[2], c[3]);
+  P.rec[4 * p + 1] = make_float4(c[4], c[5], c[6], c[7]);
+  P.rec[4 * p + 2] = make_float4(c[8], t[0], t[1], t[2]);
   P.rec[4 * p + 3] = make_float4(t[3], t[4], t[5], m.p_mass);
-  return false;
+  return defer;
 }
 
 // largest speed of the workgroup into block_max[blockIdx.x]; NaN (which fmaxf would drop) counts as +inf,
@@ -762,10 +781,11 @@ __global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restri
                                              ColliderSet cs, ColliderFrames fr, FixList fix) {
   const ColliderFrame* frames = fr.f;
   cs = with_records(cs, fr);
+  __shared__ float s_c[9][256];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float speed = 0.0f;
   bool defer = false;
-  if (p < n) defer = g2p_particle(P, mats, g, sc, dt, vel, p, cs, frames, speed);
+  if (p < n) defer = g2p_particle(P, mats, g, sc, dt, vel, p, cs, frames, speed, &s_c[0][threadIdx.x], 256);
   if (sc.precision != kPrecisionFp32) wave_append(fix, defer, (uint32_t)p);
   block_max_speed(speed, block_max);
 }
