@@ -23,8 +23,8 @@ constexpr int kWin = 8;
 constexpr int kWinNodes = kWin * kWin * kWin;
 // The staged part of a window: its core, window indices 1..6 per axis (nodes 4B .. 4B + 5 of brick B).
 // A particle binned in brick B has its stencil base in the brick's cells 4B .. 4B + 3, so its 3 nodes
-// per axis lie in the core; only a drifted particle scattered by its holder could reach the outer planes,
-// and such a particle goes to the HBM accumulator instead (p2g_scatter_atomic).
+// per axis lie in the core; only drifted particles scattered by their holder reach the outer shell
+// (indices 0 and 7), whose nonzero nodes P2G adds to the HBM accumulator at the end of the brick.
 constexpr int kCore = 6;
 constexpr int kStageNodes = kCore * kCore * kCore;
 
@@ -102,15 +102,15 @@ __device__ __forceinline__ void make_p2g_particle(const P2GRaw& r, const GridGeo
 }
 
 // Scatter of a particle whose stencil is not its sorted cell's (it drifted since the last sort): through
-// atomics into the wave's window when the stencil fits the window, else straight to HBM.
-__device__ __forceinline__ void p2g_scatter_atomic(const P2GParticle& q, const GridGeom& g, const int (&wo)[3],
+// atomics into the wave's window when the stencil fits the window, else straight to HBM. Returns whether
+// it wrote to the window's outer shell.
+__device__ __forceinline__ bool p2g_scatter_atomic(const P2GParticle& q, const GridGeom& g, const int (&wo)[3],
                                                 float (*win)[kWinNodes], float4* __restrict__ accum,
                                                 uint32_t* __restrict__ dirty, const FixedAccum& fa) {
   const int l0 = q.base[0] - wo[0], l1 = q.base[1] - wo[1], l2 = q.base[2] - wo[2];
-  // in the window when the whole stencil lies in its staged core (window indices 1 .. kCore); fixed
-  // point: straight to the integer grid (the window's float atomics would not add in a fixed order)
-  const bool in_window = fa.fx == nullptr && l0 >= 1 && l1 >= 1 && l2 >= 1 && l0 <= kCore - 2 &&
-                         l1 <= kCore - 2 && l2 <= kCore - 2;
+  // fixed point: straight to the integer grid (the window's float atomics would not add in a fixed order)
+  const bool in_window = fa.fx == nullptr && l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 <= kWin - 3 &&
+                         l1 <= kWin - 3 && l2 <= kWin - 3;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       const float wij = q.wx[0][i] * q.wx[1][j];
@@ -148,7 +148,9 @@ __device__ __forceinline__ void p2g_scatter_atomic(const P2GParticle& q, const G
       for (int j = 0; j < 2; ++j)
         for (int kk = 0; kk < 2; ++kk)
           mark_brick(g, dirty, (q.base[0] + 2 * i) >> 2, (q.base[1] + 2 * j) >> 2, (q.base[2] + 2 * kk) >> 2);
+    return false;
   }
+  return l0 == 0 || l1 == 0 || l2 == 0 || l0 == kWin - 3 || l1 == kWin - 3 || l2 == kWin - 3;
 }
 
 // ---- drift routing --------------------------------------------------------------------------------
@@ -282,6 +284,7 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
       for (int kk = 0; kk < 3; ++kk)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[i][j][kk][c] = 0.0f;
+  bool shell = false;  // this lane scattered into the window's outer shell
   // the next particle's inputs load while the current one is scattered
   P2GRaw next;
   bool next_fwd = false;
@@ -322,7 +325,7 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
         }
       }
     } else if (!cur_skip) {
-      p2g_scatter_atomic(q, g, wo, win, accum, dirty, fa);
+      shell |= p2g_scatter_atomic(q, g, wo, win, accum, dirty, fa);
     }
   }
   wave_lds_fence();
@@ -346,7 +349,9 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
       }
   // the brick's window: this wave's, or the sum of the workgroup's (each thread then takes every
   // (64 kSplit)-th node)
-  if (kSplit > 1) __syncthreads();
+  if (kSplit > 1) __syncthreads();  // every wave's window is complete
+  // whether any lane of the brick's workgroup wrote the outer shell (uniform over the workgroup)
+  const bool any_shell = kSplit > 1 ? __syncthreads_or(shell) != 0 : __ballot(shell) != 0;
   const int tid = half * 64 + lane;
   const auto node_value = [&](int i, int c) {
     float v = wins[0][c][i];
@@ -363,11 +368,31 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
       const int i = ((x + 1) * kWin + (y + 1)) * kWin + (z + 1);
       out[c] = make_float4(node_value(i, 0), node_value(i, 1), node_value(i, 2), node_value(i, 3));
     }
+    // the outer shell holds only what drifted particles scattered there: its nonzero nodes go to the HBM
+    // accumulator (four atomics per node, with the node's brick marked) instead of being staged
+    for (int i = tid; any_shell && i < kWinNodes; i += 64 * kSplit) {
+      const int x = i >> 6, y = (i >> 3) & 7, z = i & 7;
+      if (x != 0 && x != kWin - 1 && y != 0 && y != kWin - 1 && z != 0 && z != kWin - 1) continue;
+      const float vx = node_value(i, 0), vy = node_value(i, 1), vz = node_value(i, 2), vm = node_value(i, 3);
+      if (vx == 0.0f && vy == 0.0f && vz == 0.0f && vm == 0.0f) continue;
+      const int gi = wo[0] + x, gj = wo[1] + y, gk = wo[2] + z;
+      if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
+      float4* node = accum + node_addr(g, gi, gj, gk);
+      if (vx != 0.0f) atomicAdd(&node->x, vx);
+      if (vy != 0.0f) atomicAdd(&node->y, vy);
+      if (vz != 0.0f) atomicAdd(&node->z, vz);
+      if (vm != 0.0f) atomicAdd(&node->w, vm);
+      mark_brick(g, dirty, gi >> 2, gj >> 2, gk >> 2);
+    }
   } else for (int i = tid; i < kWinNodes; i += 64 * kSplit) {
-    const int gi = wo[0] + (i >> 6), gj = wo[1] + ((i >> 3) & 7), gk = wo[2] + (i & 7);
+    const int x = i >> 6, y = (i >> 3) & 7, z = i & 7;
+    const int gi = wo[0] + x, gj = wo[1] + y, gk = wo[2] + z;
     if (gi < 0 || gj < 0 || gk < 0 || gi >= g.dims[0] || gj >= g.dims[1] || gk >= g.dims[2]) continue;
     const float vx = node_value(i, 0), vy = node_value(i, 1), vz = node_value(i, 2), vm = node_value(i, 3);
     if (vx == 0.0f && vy == 0.0f && vz == 0.0f && vm == 0.0f) continue;
+    // a shell node may lie in the brick below (the core's bricks are marked below for every node)
+    if (x == 0 || x == kWin - 1 || y == 0 || y == kWin - 1 || z == 0 || z == kWin - 1)
+      mark_brick(g, dirty, gi >> 2, gj >> 2, gk >> 2);
     if (fa.fx) {
       const int nd = node_addr(g, gi, gj, gk);
       if (vx != 0.0f) fixed_add(fa, nd, 0, vx);
