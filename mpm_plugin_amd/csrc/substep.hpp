@@ -481,6 +481,44 @@ __device__ __forceinline__ float4 staged_sum(const GridGeom& g, int i, int j, in
   return a;
 }
 
+// staged_sum for every node of one brick (bx, by, bz), as the grid update needs it: the cores that can cover
+// a node of the brick are those of the 8 bricks {bx - 1, bx} x {by - 1, by} x {bz - 1, bz}, the same for every
+// node, so their slots are loaded once per wave (wave-uniform) and a node's (up to 8) stage loads are issued
+// together instead of one slot-then-stage pair after another. Node 4B + r needs brick B - 1 along an axis
+// only when r <= 1. The cores are added in the order staged_sum adds them (bricks in increasing order, x
+// outermost), so the sums are the same.
+__device__ __forceinline__ float4 staged_sum_brick(const GridGeom& g, int bx, int by, int bz, int i, int j, int k,
+                                                   float4 a, const float4* __restrict__ stage,
+                                                   const int32_t* __restrict__ slot) {
+  int32_t s[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int cx = bx - 1 + (q >> 2), cy = by - 1 + ((q >> 1) & 1), cz = bz - 1 + (q & 1);
+    const bool in = cx >= 0 && cy >= 0 && cz >= 0 && cx < g.bdims[0] && cy < g.bdims[1] && cz < g.bdims[2];
+    s[q] = in ? slot[(cx * g.bdims[1] + cy) * g.bdims[2] + cz] : -1;
+  }
+  const bool lo[3] = {i - 4 * bx <= 1, j - 4 * by <= 1, k - 4 * bz <= 1};
+  float4 w[8];
+  bool use[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int dx = q >> 2, dy = (q >> 1) & 1, dz = q & 1;
+    use[q] = s[q] >= 0 && (dx == 1 || lo[0]) && (dy == 1 || lo[1]) && (dz == 1 || lo[2]);
+    const int cx = bx - 1 + dx, cy = by - 1 + dy, cz = bz - 1 + dz;
+    const int li = ((i - 4 * cx) * kCore + (j - 4 * cy)) * kCore + (k - 4 * cz);
+    w[q] = use[q] ? stage[(int64_t)s[q] * kStageNodes + li] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (use[q]) {
+      a.x += w[q].x;
+      a.y += w[q].y;
+      a.z += w[q].z;
+      a.w += w[q].w;
+    }
+  return a;
+}
+
 // the P2G result of every node (accumulator plus staged windows) as float4, for the grid download
 __global__ void __launch_bounds__(256) k_p2g_resolve(GridGeom g, const float4* __restrict__ accum, int n_nodes,
                                                      const float4* __restrict__ stage,
@@ -501,6 +539,7 @@ __device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& 
                                            const ColliderSet& cs, const ColliderFrame* frames, int lane,
                                            const FixedAccum& fa, const float4* __restrict__ stage,
                                            const int32_t* __restrict__ slot) {
+  brick = __builtin_amdgcn_readfirstlane(brick);  // one brick per wave
   const int addr = brick * 64 + lane;
   int i, j, k;
   node_ijk(g, addr, i, j, k);
@@ -514,7 +553,10 @@ __device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& 
     a = accum[addr];
   }
   const float4 drift = a;
-  if (stage && !fa.fx) a = staged_sum(g, i, j, k, a, stage, slot);
+  if (stage && !fa.fx) {
+    const int bz = brick % g.bdims[2], by = (brick / g.bdims[2]) % g.bdims[1], bx = brick / (g.bdims[2] * g.bdims[1]);
+    a = staged_sum_brick(g, bx, by, bz, i, j, k, a, stage, slot);
+  }
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (i < g.dims[0] && j < g.dims[1] && k < g.dims[2]) {
     out = update_node(g, sc, dt, i, j, k, a);
