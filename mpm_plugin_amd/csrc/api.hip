@@ -690,7 +690,8 @@ void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
                                                                   frames_at(c, s, inv_frame_dt), c->n_fix);
     check_launch("k_grid_dense");
   } else {
-    const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
+    const int64_t waves = (c->n_bricks + kGridBricksPerWave - 1) / kGridBricksPerWave;
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
     k_grid_bricks<<<blocks, 256, 0, c->stream>>>(c->geom, c->sc, dt, c->n_bricks, c->dirty, c->live, c->accum,
                                                  c->vel, c->collider_set(), frames_at(c, s, inv_frame_dt),
                                                  c->fixed(), c->flush_stage(), c->slot, c->n_fix);

@@ -568,6 +568,8 @@ __device__ __forceinline__ void grid_brick(const GridGeom& g, const StepConsts& 
 }
 
 // one wave per brick; non-dirty bricks are skipped (and flagged stale for the grid download)
+constexpr int kGridBricksPerWave = 8;
+
 __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, float dt, int n_bricks,
                                                      uint32_t* __restrict__ dirty, uint8_t* __restrict__ live,
                                                      float4* __restrict__ accum, float4* __restrict__ vel,
@@ -579,14 +581,26 @@ __global__ void __launch_bounds__(256) k_grid_bricks(GridGeom g, StepConsts sc, 
   cs = with_records(cs, fr);
   // the FP64 work list of the G2P that follows starts empty
   if (n_fix && blockIdx.x == 0 && threadIdx.x == 0) *n_fix = 0u;
-  const int brick = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (brick >= n_bricks) return;
+  // wave w examines the kGridBricksPerWave bricks w, w + W, w + 2W, ... (W waves in the launch): a few
+  // thousand waves instead of one per brick of the grid, most of which would find their brick clean, and
+  // the dirty bricks (contiguous bodies in brick order) still spread over the waves
+  const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   const int lane = threadIdx.x & 63;
-  const bool d = dirty[brick] != 0u;
-  if (lane == 0) live[brick] = d ? 1 : 0;
-  if (!d) return;
-  grid_brick(g, sc, dt, brick, accum, vel, cs, frames, lane, fa, stage, slot);
-  if (lane == 0) dirty[brick] = 0u;
+  const int mine = w + lane * waves;
+  bool d = false;
+  if (lane < kGridBricksPerWave && mine < n_bricks) {
+    d = dirty[mine] != 0u;
+    live[mine] = d ? 1 : 0;
+  }
+  uint64_t todo = __ballot(d);
+  while (todo != 0ull) {
+    const int l = __builtin_ctzll(todo);
+    todo &= todo - 1ull;
+    const int brick = w + l * waves;
+    grid_brick(g, sc, dt, brick, accum, vel, cs, frames, lane, fa, stage, slot);
+    if (lane == 0) dirty[brick] = 0u;
+  }
 }
 
 // dense variant for the reference-order P2G (which flags no bricks): every node of the grid
