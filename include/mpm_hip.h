@@ -168,9 +168,10 @@ int32_t mpm_debug_sort(mpm_ctx* ctx);
                                     n takes the particles h, h + n, ... of each cell (more waves in flight
                                     for scenes with few bricks) */
 #define MPM_OPT_P2G_FLUSH 5      /* binned P2G: how a brick's window reaches the grid update. MPM_P2G_FLUSH_STAGED
-                                    (default): stored whole into the brick's staging slot, and the grid update sums
-                                    the (up to 8) windows covering each node in a fixed order; MPM_P2G_FLUSH_ATOMIC:
-                                    its non-zero nodes added into the HBM accumulator with global float atomics */
+                                    (default): its 6^3-node core stored into the brick's staging slot, and the grid
+                                    update sums the (1 to 8) cores covering each node in a fixed order;
+                                    MPM_P2G_FLUSH_ATOMIC: its non-zero nodes added into the HBM accumulator with
+                                    global float atomics */
 #define MPM_P2G_FLUSH_STAGED 0
 #define MPM_P2G_FLUSH_ATOMIC 1
 #define MPM_P2G_BINNED 0         /* per-brick LDS window, global FP32 atomics for the flush */

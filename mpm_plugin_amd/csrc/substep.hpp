@@ -73,32 +73,66 @@ __device__ __forceinline__ void load_p2g_raw(const Particles& P, int64_t p, P2GR
   for (int e = 0; e < 4; ++e) r.r[e] = P.rec[4 * p + e];
 }
 
-__device__ __forceinline__ void make_p2g_particle(const P2GRaw& r, const GridGeom& g, float k, P2GParticle& q) {
+// the parts of a particle's P2G inputs that do not depend on where it lies in its stencil: the affine
+// momentum matrix A, m v and m
+__device__ __forceinline__ void p2g_affine(const P2GRaw& r, float k, float (&A)[3][3], float (&mv)[3], float& pm) {
   const float C[9] = {r.r[0].x, r.r[0].y, r.r[0].z, r.r[0].w, r.r[1].x, r.r[1].y, r.r[1].z, r.r[1].w, r.r[2].x};
   const float t00 = r.r[2].y, t11 = r.r[2].z, t22 = r.r[2].w, t01 = r.r[3].x, t02 = r.r[3].y, t12 = r.r[3].z;
-  const float pm = r.r[3].w;
-  q.A[0][0] = k * t00 + pm * C[0];
-  q.A[0][1] = k * t01 + pm * C[1];
-  q.A[0][2] = k * t02 + pm * C[2];
-  q.A[1][0] = k * t01 + pm * C[3];
-  q.A[1][1] = k * t11 + pm * C[4];
-  q.A[1][2] = k * t12 + pm * C[5];
-  q.A[2][0] = k * t02 + pm * C[6];
-  q.A[2][1] = k * t12 + pm * C[7];
-  q.A[2][2] = k * t22 + pm * C[8];
-  q.pm = pm;
-  q.mv[0] = pm * r.v[0];
-  q.mv[1] = pm * r.v[1];
-  q.mv[2] = pm * r.v[2];
-  float fx[3];
-  stencil_base(g, r.x, q.base, fx);
+  pm = r.r[3].w;
+  A[0][0] = k * t00 + pm * C[0];
+  A[0][1] = k * t01 + pm * C[1];
+  A[0][2] = k * t02 + pm * C[2];
+  A[1][0] = k * t01 + pm * C[3];
+  A[1][1] = k * t11 + pm * C[4];
+  A[1][2] = k * t12 + pm * C[5];
+  A[2][0] = k * t02 + pm * C[6];
+  A[2][1] = k * t12 + pm * C[7];
+  A[2][2] = k * t22 + pm * C[8];
+  mv[0] = pm * r.v[0];
+  mv[1] = pm * r.v[1];
+  mv[2] = pm * r.v[2];
+}
+
+// per-axis stencil weights and node offsets (x_node - x_p) of a particle at fractional offset fx
+__device__ __forceinline__ void p2g_weights(const GridGeom& g, const float (&fx)[3], float (&wx)[3][3],
+                                            float (&dp)[3][3]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      q.wx[a][i] = bspline_w(fx[a], i);
-      q.dp[a][i] = ((float)i - fx[a]) * g.dx;
+      wx[a][i] = bspline_w(fx[a], i);
+      dp[a][i] = ((float)i - fx[a]) * g.dx;
     }
+}
+
+__device__ __forceinline__ void make_p2g_particle(const P2GRaw& r, const GridGeom& g, float k, P2GParticle& q) {
+  p2g_affine(r, k, q.A, q.mv, q.pm);
+  float fx[3];
+  stencil_base(g, r.x, q.base, fx);
+  p2g_weights(g, fx, q.wx, q.dp);
+}
+
+// a particle's 27 stencil contributions (momentum with its affine part, and mass) added to a lane's sums
+__device__ __forceinline__ void p2g_accumulate(float (&acc)[3][3][3][4], const float (&A)[3][3], const float (&mv)[3],
+                                               float pm, const float (&wx)[3][3], const float (&dp)[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wij = wx[0][i] * wx[1][j];
+      const float ax = mv[0] + A[0][0] * dp[0][i] + A[0][1] * dp[1][j];
+      const float ay = mv[1] + A[1][0] * dp[0][i] + A[1][1] * dp[1][j];
+      const float az = mv[2] + A[2][0] * dp[0][i] + A[2][1] * dp[1][j];
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const float wgt = wij * wx[2][kk];
+        acc[i][j][kk][0] += wgt * (ax + A[0][2] * dp[2][kk]);
+        acc[i][j][kk][1] += wgt * (ay + A[1][2] * dp[2][kk]);
+        acc[i][j][kk][2] += wgt * (az + A[2][2] * dp[2][kk]);
+        acc[i][j][kk][3] += wgt * pm;
+      }
+    }
+  }
 }
 
 // Scatter of a particle whose stencil is not its sorted cell's (it drifted since the last sort): through
@@ -306,24 +340,7 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
     P2GParticle q;
     make_p2g_particle(cur, g, k, q);
     if (q.base[0] == cb[0] && q.base[1] == cb[1] && q.base[2] == cb[2]) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          const float wij = q.wx[0][i] * q.wx[1][j];
-          const float ax = q.mv[0] + q.A[0][0] * q.dp[0][i] + q.A[0][1] * q.dp[1][j];
-          const float ay = q.mv[1] + q.A[1][0] * q.dp[0][i] + q.A[1][1] * q.dp[1][j];
-          const float az = q.mv[2] + q.A[2][0] * q.dp[0][i] + q.A[2][1] * q.dp[1][j];
-#pragma unroll
-          for (int kk = 0; kk < 3; ++kk) {
-            const float wgt = wij * q.wx[2][kk];
-            acc[i][j][kk][0] += wgt * (ax + q.A[0][2] * q.dp[2][kk]);
-            acc[i][j][kk][1] += wgt * (ay + q.A[1][2] * q.dp[2][kk]);
-            acc[i][j][kk][2] += wgt * (az + q.A[2][2] * q.dp[2][kk]);
-            acc[i][j][kk][3] += wgt * q.pm;
-          }
-        }
-      }
+      p2g_accumulate(acc, q.A, q.mv, q.pm, q.wx, q.dp);
     } else if (!cur_skip) {
       shell |= p2g_scatter_atomic(q, g, wo, win, accum, dirty, fa);
     }
