@@ -1,16 +1,17 @@
 // Binned P2G and brick-sparse grid update on gfx950.
 //
-// P2G (kernels.py:92-140 semantics) runs one workgroup per occupied brick B (grid-stride over the
-// device-side list of occupied bricks, so the launch needs no host-side count). Particles were sorted
-// by brick, so brick B's particles are the contiguous slots [offsets[B], offsets[B+1]). Their stencils
-// lie in the 8x8x8-node window starting at node 4B - 1, which leaves one node of margin for particles
-// that drifted since the sort; the window (momentum xyz + mass, 8 KiB as four planes) is accumulated
-// with LDS float atomics and flushed to the bricked HBM grid with one global atomic per non-zero node
-// component. A particle whose stencil left the window (drift of more than one node) scatters straight
-// to HBM with global atomics. Every brick a workgroup may have written is flagged dirty.
+// P2G (kernels.py:92-140 semantics) walks the device-side list of occupied bricks (grid-stride, so the
+// launch needs no host-side count). Particles were sorted by brick and cell, so each lane of a wave owns
+// one of the brick's 64 cells and sums its particles' 27 stencil contributions in registers; the 64 lanes
+// then add their sums into the wave's 8x8x8-node LDS window with plain read-add-writes (no two lanes of
+// one instruction touch the same node). The window's 6^3 core goes to the brick's staging slot; the outer
+// shell, which only particles that drifted since the sort reach, goes to the HBM accumulator with global
+// atomics. A drifted particle is summed by the lane of its current cell when drift routing handed it
+// there, else scattered with atomics by the lane that holds it.
 //
-// The grid update (kernels.py:155-180) runs one wave per brick (64 nodes = 64 lanes) and only on dirty
-// bricks: momentum -> velocity, gravity, walls, then contact with the mesh colliders; it clears the
+// The grid update (kernels.py:155-180) gives each wave a strided set of bricks and updates the dirty ones,
+// one lane per node: the node's P2G result is the sum of the staged cores that cover it plus its atomic
+// part; momentum -> velocity, gravity, walls, then contact with the mesh colliders. It clears the
 // accumulator and the dirty flag behind itself and records which bricks hold a fresh velocity.
 #pragma once
 
