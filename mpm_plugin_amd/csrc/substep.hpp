@@ -310,6 +310,16 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
     if (dl.cnt && half == 0) dl.cnt[slot * 64u + (uint32_t)lane] = 0u;
   }
   const uint32_t total = own_n + n_in;
+  // The routed particles' indices are held in registers, so no particle's loads wait on an index load
+  // inside the loop below (a wait for one load is a wait for every load issued before it, the prefetch
+  // of the next particle included).
+  uint32_t inl[kInPerCell];
+#pragma unroll
+  for (int j = 0; j < kInPerCell; ++j) inl[j] = (uint32_t)j < n_in ? in_list[j] : 0u;
+  const auto index_of = [&](uint32_t t) -> uint32_t {
+    const uint32_t j = t - own_n;
+    return t < own_n ? p0 + t : (j == 0 ? inl[0] : j == 1 ? inl[1] : j == 2 ? inl[2] : inl[3]);
+  };
   float acc[3][3][3][4];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -320,23 +330,25 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[i][j][kk][c] = 0.0f;
   bool shell = false;  // this lane scattered into the window's outer shell
-  // the next particle's inputs load while the current one is scattered
+  // The next particle's inputs load while the current one is scattered. Its drift flag is loaded as a raw
+  // byte and tested only once it is the current particle: testing it at once would wait for the whole
+  // prefetch.
   P2GRaw next;
-  bool next_fwd = false;
+  uint32_t next_fwd = 0u;
   const uint32_t t0 = (uint32_t)half;
   if (t0 < total) {
-    const uint32_t pn = t0 < own_n ? p0 + t0 : in_list[t0 - own_n];
+    const uint32_t pn = index_of(t0);
     load_p2g_raw(P, (int64_t)pn, next);
-    if (dl.cnt && t0 < own_n) next_fwd = dl.fwd[pn] != 0;
+    if (dl.cnt) next_fwd = dl.fwd[pn];
   }
   for (uint32_t t = t0; t < total; t += kSplit) {
     const P2GRaw cur = next;
-    const bool cur_skip = t < own_n && next_fwd;  // routed to its current cell's lane
+    const bool cur_skip = t < own_n && next_fwd != 0u;  // routed to its current cell's lane
     const uint32_t tn = t + kSplit;
     if (tn < total) {
-      const uint32_t pn = tn < own_n ? p0 + tn : in_list[tn - own_n];
+      const uint32_t pn = index_of(tn);
       load_p2g_raw(P, (int64_t)pn, next);
-      next_fwd = dl.cnt && tn < own_n && dl.fwd[pn] != 0;
+      if (dl.cnt) next_fwd = dl.fwd[pn];
     }
     P2GParticle q;
     make_p2g_particle(cur, g, k, q);
