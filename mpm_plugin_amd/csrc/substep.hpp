@@ -236,6 +236,14 @@ __global__ void __launch_bounds__(256) k_drift(Particles P, GridGeom g, int n, c
   if (p < n) drift_route(P, g, p, keys, slot, in_cnt, in_lst, fwd);
 }
 
+// A value of another lane of the same 16-lane row (DPP row shift; kRowShr4: lane l - 4, kRowShl8: l + 8, ...);
+// zero when that lane lies outside the row. The whole wave must be active.
+constexpr int kRowShl4 = 0x104, kRowShl8 = 0x108, kRowShl12 = 0x10c, kRowShr4 = 0x114, kRowShr8 = 0x118;
+template <int kCtrl>
+__device__ __forceinline__ float row_shift(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, true));
+}
+
 // orders this wave's LDS accesses across lanes (a wave's LDS operations complete in issue order)
 __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -359,24 +367,35 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
     }
   }
   wave_lds_fence();
-  // Node (cx + 1 + i, ...) of one lane is node (cx' + 1 + i', ...) of a neighbouring lane at another
-  // offset, so the 27 read-add-writes must stay in program order across lanes: the fence after each
-  // offset keeps the compiler from batching the reads of later offsets ahead of earlier writes (a
-  // wave's LDS operations then complete in issue order). Lanes without particles add nothing.
-  const bool any = total > t0;
+  // The lane sums reach the window in two steps. Along y they are added across lanes first: lane
+  // 16 cx + 4 cy + cz, so a 16-lane row is one cx and a row shift by 4 lanes moves one cell along y (a
+  // lane whose source lies outside its row reads zero). For each (i, k), window node y index cy + 1
+  // collects acc[i][0][k] of cell cy, acc[i][1][k] of cell cy - 1 and acc[i][2][k] of cell cy - 2; node
+  // cy + 5 (cy < 2) collects acc[i][1][k] of cell cy + 3 and acc[i][2][k] of cell cy + 2. Those nine (i, k)
+  // sums are then added into the window with read-add-writes: within one (i, k) no two lanes touch the same
+  // node; across them they may, so the fence after each keeps the nine in program order (a wave's LDS
+  // operations complete in issue order). Every lane takes part (an empty cell's lane still carries its
+  // neighbours' sums).
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+    for (int kk = 0; kk < 3; ++kk) {
+      float lo[4], hi[4];
 #pragma unroll
-      for (int kk = 0; kk < 3; ++kk) {
-        const int li = ((cx + 1 + i) * kWin + (cy + 1 + j)) * kWin + (cz + 1 + kk);
-        if (any) {
-#pragma unroll
-          for (int c = 0; c < 4; ++c) win[c][li] += acc[i][j][kk][c];
-        }
-        wave_lds_fence();
+      for (int c = 0; c < 4; ++c) {
+        const float a0 = acc[i][0][kk][c], a1 = acc[i][1][kk][c], a2 = acc[i][2][kk][c];
+        lo[c] = (a0 + row_shift<kRowShr4>(a1)) + row_shift<kRowShr8>(a2);
+        hi[c] = row_shift<kRowShl12>(a1) + row_shift<kRowShl8>(a2);
       }
+      const int li = ((cx + 1 + i) * kWin + (cy + 1)) * kWin + (cz + 1 + kk);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) win[c][li] += lo[c];
+      if (cy < 2) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) win[c][li + 4 * kWin] += hi[c];
+      }
+      wave_lds_fence();
+    }
   // the brick's window: this wave's, or the sum of the workgroup's (each thread then takes every
   // (64 kSplit)-th node)
   if (kSplit > 1) __syncthreads();  // every wave's window is complete
