@@ -14,7 +14,7 @@ from collections.abc import Sequence
 
 import numpy as np
 
-from mpm_core.geometry import decompose_transform, mesh_is_closed
+from mpm_core.geometry import decompose_transform_sv, mesh_is_closed
 from mpm_core.materials import MaterialParams
 from mpm_core.scene import Domain
 from mpm_core.solver import AdvanceStats, ColliderGeometry, ParticleData, Solver
@@ -92,7 +92,8 @@ def material_table(materials: Sequence[MaterialParams], domain: Domain, particle
 
 def _pose_parts(m: np.ndarray):
     """decompose_transform(m) and the smallest singular value of its linear part (the margin scale)."""
-    return decompose_transform(m), float(np.linalg.svd(np.asarray(m)[:3, :3], compute_uv=False).min())
+    t, q, S, smallest = decompose_transform_sv(m)
+    return (t, q, S), smallest
 
 
 class _PoseCache:
@@ -114,19 +115,19 @@ class _PoseCache:
         return hit
 
 
+_POSE_FLOATS = C.sizeof(N.ColliderPose) // 4  # t0 3, t1 3, q0 4, q1 4, S0 9, S1 9, margin 1
+
+
 def collider_poses(start: np.ndarray, end: np.ndarray, dx: float, parts=_pose_parts):
     """Decomposed start/end poses and AABB margins (solver.py:154-172, 258-260)."""
     poses = (N.ColliderPose * max(1, len(start)))()
+    rows = np.frombuffer(poses, np.float32).reshape(-1, _POSE_FLOATS)  # the structs' fields, in place
     for i, (a, b) in enumerate(zip(start, end)):
         ((t0, q0, S0), smin0), ((t1, q1, S1), smin1) = parts(a), parts(b)
         if np.dot(q0, q1) < 0:
             q1 = -q1  # slerp the short way round
-        p = poses[i]
-        p.t0[:], p.t1[:] = t0.tolist(), t1.tolist()
-        p.q0[:], p.q1[:] = q0.tolist(), q1.tolist()
-        p.S0[:], p.S1[:] = S0.ravel().tolist(), S1.ravel().tolist()
         inv_scale = max(1.0 / max(smin0, 1e-12), 1.0 / max(smin1, 1e-12))
-        p.margin = 3.0 * dx * inv_scale
+        rows[i] = np.concatenate((t0, t1, q0, q1, S0.ravel(), S1.ravel(), (3.0 * dx * inv_scale,)))
     return poses
 
 

@@ -71,14 +71,20 @@ def decompose_transform(m: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarr
     A mirrored matrix keeps R a proper rotation; the reflection goes into S through the sign of the
     smallest singular value.
     """
+    return decompose_transform_sv(m)[:3]
+
+
+def decompose_transform_sv(m: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray, float]:
+    """decompose_transform(m) and the smallest singular value of the linear part, from the same SVD."""
     A = np.asarray(m, np.float64)[:3, :3]
     U, sig, Vt = np.linalg.svd(A)
+    smallest = float(sig[2])  # descending and non-negative before the reflection
     if np.linalg.det(U @ Vt) < 0:
         U[:, 2] *= -1.0
         sig[2] *= -1.0
     R = U @ Vt
     S = Vt.T @ (sig[:, None] * Vt)
-    return np.asarray(m, np.float64)[:3, 3].copy(), _quat_from_matrix(R), S
+    return np.asarray(m, np.float64)[:3, 3].copy(), _quat_from_matrix(R), S, smallest
 
 
 def _slerp(q0: np.ndarray, q1: np.ndarray, s: float) -> np.ndarray:

@@ -146,15 +146,20 @@ class Workload:
         return len(self.positions)
 
     def collider_transforms(self, t0: float, t1: float) -> tuple[np.ndarray, np.ndarray]:
-        """(C,4,4) start/end world matrices at times t0 and t1 of a linear translation path."""
-        starts, ends = [], []
-        for c, ((a, b), geom) in enumerate(zip(self.collider_path, self.colliders)):
-            del geom
+        """(C,4,4) start/end world matrices at times t0 and t1 of a linear translation path: translate(p)
+        @ scale(s), written out (the product of those float32 matrices is exactly this one)."""
+        n = min(len(self.collider_path), len(self.colliders))
+        out = np.zeros((2, n, 4, 4), np.float32)
+        for c in range(n):
+            a, b = self.collider_path[c]
             scale = self.collider_scale[c] if c < len(self.collider_scale) else 0.1
-            for t, out in ((t0, starts), (t1, ends)):
+            for k, t in enumerate((t0, t1)):
                 u = min(max(t / self.duration, 0.0), 1.0) if self.duration > 0 else 0.0
-                out.append(_translate(*(a + u * (b - a))) @ _scale(scale))
-        return np.stack(starts), np.stack(ends)
+                m = out[k, c]
+                m[0, 0] = m[1, 1] = m[2, 2] = scale
+                m[3, 3] = 1.0
+                m[:3, 3] = a + u * (b - a)
+        return out[0], out[1]
 
     def make_solver(self, backend: str = "hip", device: str | None = None):
         from mpm_core.backends import create_solver
