@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -209,10 +210,12 @@ struct mpm_ctx {
   uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
   float *d_cell_key = nullptr, *d_cell_far = nullptr;
 
-  float* h_vmax = nullptr;        // reduced maximum speed, in pinned host memory the reduction writes to
+  // The reduced maximum speed, in pinned host memory the reduction writes to: h_vmax[0] the speed, h_vmax[1]
+  // (as uint32) the sequence number of the request it answers, written after it behind a system fence.
+  float* h_vmax = nullptr;
   float* h_vmax_dev = nullptr;    // its device address
+  uint32_t vmax_seq = 0;          // the last request's sequence number
   bool vmax_pending = false;      // a reduction into h_vmax of the current state is on the stream
-  hipEvent_t vmax_done = nullptr; // recorded behind that reduction
   float* d_block_max = nullptr;   // per-workgroup maxima of the last G2P (or speed pass)
   int64_t block_max_cap = 0;
   int64_t block_max_count = 0;
@@ -700,6 +703,9 @@ void run_grid(mpm_ctx* c, float dt, float s, float inv_frame_dt) {
   c->n_fix_cleared = true;
 }
 
+// device address of the published sequence number (next to the speed in pinned host memory)
+uint32_t* speed_seq_dev(mpm_ctx* c) { return reinterpret_cast<uint32_t*>(c->h_vmax_dev + 1); }
+
 // reduce_speed: the caller wants this G2P's speed maximum next (the last substep of an advance); with the
 // FP64 pass on, that pass reduces it into h_vmax on the way and the request is marked as enqueued
 void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_speed = false) {
@@ -721,18 +727,15 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_spee
   if (fix) {
     // a persistent grid: the deferred particles are usually few
     Timed t(c, MPM_K_OTHER);
+    if (fused) ++c->vmax_seq;
     k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 256), kBlock, 0, c->stream>>>(
         c->parts(), c->d_mats, c->fix_list, c->n_fix, block_max(c), (int)grid_for(c->n),
-        fused ? c->h_vmax_dev : nullptr);
+        fused ? c->h_vmax_dev : nullptr, speed_seq_dev(c), c->vmax_seq);
     check_launch("k_fp64_fixup");
   }
   c->block_max_count = grid_for(c->n);
   c->vmax_valid = true;
-  c->vmax_pending = false;  // a reduction enqueued before this G2P is of the old state
-  if (fused) {
-    HIP_CHECK(hipEventRecord(c->vmax_done, c->stream));
-    c->vmax_pending = true;
-  }
+  c->vmax_pending = fused;  // (a reduction enqueued before this G2P is of the old state)
 }
 
 void substep(mpm_ctx* c, double dt, double s, double frame_dt, const DriftLists* pre = nullptr,
@@ -748,14 +751,15 @@ void substep(mpm_ctx* c, double dt, double s, double frame_dt, const DriftLists*
 // max |v| over the particles (kernels.py:266-269): the last G2P left per-workgroup maxima; after any
 // state change a speed pass recomputes them
 // the speed reduction without a host wait: writes max |v| to `out` (device memory) on the context's stream
-void reduce_speed_async(mpm_ctx* c, float* out) {
+// seq_out: publish the maximum to the host with that sequence number (see reduce_block_maxima)
+void reduce_speed_async(mpm_ctx* c, float* out, uint32_t* seq_out = nullptr, uint32_t seq = 0) {
   if (!c->vmax_valid) {
     k_speed_partials<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), (int)c->n, block_max(c));
     check_launch("k_speed_partials");
     c->block_max_count = grid_for(c->n);
     c->vmax_valid = true;
   }
-  k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->d_block_max, (int)c->block_max_count, out);
+  k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->d_block_max, (int)c->block_max_count, out, seq_out, seq);
   check_launch("k_reduce_max");
 }
 
@@ -766,24 +770,31 @@ void request_speed(mpm_ctx* c) {
   if (c->n == 0 || c->vmax_pending) return;
   {
     Timed t(c, MPM_K_OTHER);
-    reduce_speed_async(c, c->h_vmax_dev);
+    ++c->vmax_seq;
+    reduce_speed_async(c, c->h_vmax_dev, speed_seq_dev(c), c->vmax_seq);
   }
-  HIP_CHECK(hipEventRecord(c->vmax_done, c->stream));
   c->vmax_pending = true;
 }
 
-// Waits for the requested reduction only (work enqueued behind it keeps running) by polling its event:
-// the step between two advances is on the critical path, and a blocking wait adds its wake-up latency
-// to every one of them.
+// Waits for the requested reduction only (work enqueued behind it keeps running) by polling the sequence
+// number the reduction publishes with it: the step between two advances is on the critical path, and
+// neither a blocking wait (its wake-up latency) nor an event behind the kernel (a barrier packet between
+// it and the next kernel) belongs there. Should the stream finish without the number appearing, the wait
+// fails rather than spinning on.
 float max_speed(mpm_ctx* c) {
   request_speed(c);
   if (c->n > 0) {
-    hipError_t e;
-    while ((e = hipEventQuery(c->vmax_done)) == hipErrorNotReady) {
+    const volatile uint32_t* seq = reinterpret_cast<const volatile uint32_t*>(c->h_vmax + 1);
+    for (uint64_t spins = 1; *seq != c->vmax_seq; ++spins) {
+      if (spins % 4096 == 0) {
+        const hipError_t e = hipStreamQuery(c->stream);
+        if (e == hipSuccess && *seq != c->vmax_seq) fail(MPM_ERR_RUNTIME, "speed reduction did not publish");
+        if (e != hipSuccess && e != hipErrorNotReady) HIP_CHECK(e);
+      }
     }
-    HIP_CHECK(e);
+    std::atomic_thread_fence(std::memory_order_acquire);
   }
-  return *static_cast<volatile float*>(c->h_vmax);
+  return static_cast<volatile float*>(c->h_vmax)[0];
 }
 
 void load_state(mpm_ctx* c, int64_t n, const float* x, const float* v, const float* Cm, const float* F,
@@ -1779,10 +1790,11 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     HIP_CHECK(hipMemsetAsync(c->dirty, 0, bricks * 4, c->stream));
     HIP_CHECK(hipMemsetAsync(c->live, 0, bricks, c->stream));
     HIP_CHECK(hipMemsetAsync(c->n_active, 0, 4, c->stream));
-    HIP_CHECK(hipHostMalloc((void**)&c->h_vmax, sizeof(float), hipHostMallocDefault));
-    *c->h_vmax = 0.0f;
+    // coherent: the reduction's writes reach host memory as it makes them, not at the end of its kernel
+    HIP_CHECK(hipHostMalloc((void**)&c->h_vmax, 2 * sizeof(float), hipHostMallocCoherent));
+    c->h_vmax[0] = 0.0f;
+    reinterpret_cast<uint32_t*>(c->h_vmax)[1] = 0u;
     HIP_CHECK(hipHostGetDevicePointer((void**)&c->h_vmax_dev, c->h_vmax, 0));
-    HIP_CHECK(hipEventCreateWithFlags(&c->vmax_done, hipEventDisableTiming));
     reserve(c, 256);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -1813,7 +1825,6 @@ int32_t mpm_destroy(mpm_ctx* c) {
   c->scan_sums.release();
   release_timing(c->timing);
   if (c->h_vmax) (void)hipHostFree(c->h_vmax);
-  if (c->vmax_done) (void)hipEventDestroy(c->vmax_done);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   (void)hipSetDevice(prev);
   delete c;

@@ -736,7 +736,10 @@ __device__ __forceinline__ void material_update_fp64(const MatDev& m, Particles&
 // one workgroup of 256: the largest of count per-workgroup maxima into *out (NaN counts as +inf). Each
 // thread keeps 8 loads in flight per round (independent partial maxima): one at a time, the reduction of
 // a 1M-particle G2P's 3,990 maxima paid the L2 latency 16 times in a row on the substep's critical path.
-__device__ void reduce_block_maxima(const float* __restrict__ block_max, int count, float* __restrict__ out) {
+// With seq_out, the maximum is published to the host: after *out, a system-scope fence, then *seq_out = seq,
+// so a host that sees seq also sees the maximum (and needs no event behind the kernel).
+__device__ void reduce_block_maxima(const float* __restrict__ block_max, int count, float* __restrict__ out,
+                                    uint32_t* seq_out = nullptr, uint32_t seq = 0u) {
   constexpr int kInFlight = 8;
   float part[kInFlight];
 #pragma unroll
@@ -756,7 +759,13 @@ __device__ void reduce_block_maxima(const float* __restrict__ block_max, int cou
   const float wm = wave_max(m);
   if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
   __syncthreads();
-  if (threadIdx.x == 0) *out = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+  if (threadIdx.x == 0) {
+    *out = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+    if (seq_out) {
+      __threadfence_system();
+      *static_cast<volatile uint32_t*>(seq_out) = seq;
+    }
+  }
   __syncthreads();
 }
 
@@ -767,8 +776,9 @@ __global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* _
                                                     const uint32_t* __restrict__ list,
                                                     const uint32_t* __restrict__ count,
                                                     const float* __restrict__ block_max, int block_count,
-                                                    float* __restrict__ vmax_out) {
-  if (vmax_out && blockIdx.x == 0) reduce_block_maxima(block_max, block_count, vmax_out);
+                                                    float* __restrict__ vmax_out, uint32_t* seq_out,
+                                                    uint32_t seq) {
+  if (vmax_out && blockIdx.x == 0) reduce_block_maxima(block_max, block_count, vmax_out, seq_out, seq);
   const uint32_t n = *count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t p = list[i];
@@ -945,8 +955,8 @@ __global__ void __launch_bounds__(256) k_speed_partials(Particles P, int n, floa
 
 // one workgroup: the maximum of the per-workgroup maxima
 __global__ void __launch_bounds__(256) k_reduce_max(const float* __restrict__ block_max, int count,
-                                                    float* __restrict__ out) {
-  reduce_block_maxima(block_max, count, out);
+                                                    float* __restrict__ out, uint32_t* seq_out, uint32_t seq) {
+  reduce_block_maxima(block_max, count, out, seq_out, seq);
 }
 
 }  // namespace mpm
