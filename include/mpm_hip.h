@@ -108,8 +108,16 @@ int32_t mpm_get_state(mpm_ctx* ctx, float* x, float* v, float* C, float* F, floa
 /* replaces WarpSolver.set_colliders (solver.py:142-152): meshes concatenated, local space */
 int32_t mpm_set_colliders(mpm_ctx* ctx, int32_t n, const int32_t* n_verts, const int32_t* n_tris,
                           const float* verts, const int32_t* tris, const int32_t* closed, const float* friction);
-/* replaces WarpSolver.set_collider_transforms (solver.py:154-172) */
+/* replaces WarpSolver.set_collider_transforms (solver.py:154-172) with poses decomposed by the caller */
 int32_t mpm_set_collider_poses(mpm_ctx* ctx, int32_t n, const mpm_collider_pose* poses);
+/* replaces WarpSolver.set_collider_transforms (solver.py:154-172) with _decompose (:258-260,
+ * geometry.py:30-49) done here in FP64: start/end are n row-major 4x4 float32 world matrices. A count other
+ * than the colliders' is a ValueError (solver.py:157-158); so is a non-finite entry. */
+int32_t mpm_set_collider_transforms(mpm_ctx* ctx, int32_t n, const float* start, const float* end);
+/* the decomposition mpm_set_collider_transforms applies (geometry.py:30-49 decompose_transform): of one
+ * row-major 4x4 float32 matrix, translation t[3], rotation quaternion q[4] (x, y, z, w), symmetric stretch
+ * S[9] (row-major) and the smallest singular value of the linear part; no device work */
+int32_t mpm_decompose_transform(const float* m, double* t, double* q, double* S, double* smin);
 
 /* ---- stepping ----------------------------------------------------------------------------- */
 /* replaces WarpSolver.advance (solver.py:213-220) with _choose_dt (:176-185) and _remove_escaped (:222-240) */

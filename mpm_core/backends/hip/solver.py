@@ -14,7 +14,7 @@ from collections.abc import Sequence
 
 import numpy as np
 
-from mpm_core.geometry import decompose_transform_sv, mesh_is_closed
+from mpm_core.geometry import mesh_is_closed
 from mpm_core.materials import MaterialParams
 from mpm_core.scene import Domain
 from mpm_core.solver import AdvanceStats, ColliderGeometry, ParticleData, Solver
@@ -90,47 +90,6 @@ def material_table(materials: Sequence[MaterialParams], domain: Domain, particle
     return arr
 
 
-def _pose_parts(m: np.ndarray):
-    """decompose_transform(m) and the smallest singular value of its linear part (the margin scale)."""
-    t, q, S, smallest = decompose_transform_sv(m)
-    return (t, q, S), smallest
-
-
-class _PoseCache:
-    """_pose_parts of the last few matrices: consecutive advances share their boundary pose (the end of one
-    is the start of the next), so every pose is decomposed once."""
-
-    def __init__(self, size: int = 16):
-        self.size = size
-        self.items: dict[bytes, tuple] = {}
-
-    def __call__(self, m: np.ndarray):
-        key = np.ascontiguousarray(m, np.float32).tobytes()
-        hit = self.items.get(key)
-        if hit is None:
-            hit = _pose_parts(m)
-            if len(self.items) >= self.size:
-                self.items.pop(next(iter(self.items)))
-            self.items[key] = hit
-        return hit
-
-
-_POSE_FLOATS = C.sizeof(N.ColliderPose) // 4  # t0 3, t1 3, q0 4, q1 4, S0 9, S1 9, margin 1
-
-
-def collider_poses(start: np.ndarray, end: np.ndarray, dx: float, parts=_pose_parts):
-    """Decomposed start/end poses and AABB margins (solver.py:154-172, 258-260)."""
-    poses = (N.ColliderPose * max(1, len(start)))()
-    rows = np.frombuffer(poses, np.float32).reshape(-1, _POSE_FLOATS)  # the structs' fields, in place
-    for i, (a, b) in enumerate(zip(start, end)):
-        ((t0, q0, S0), smin0), ((t1, q1, S1), smin1) = parts(a), parts(b)
-        if np.dot(q0, q1) < 0:
-            q1 = -q1  # slerp the short way round
-        inv_scale = max(1.0 / max(smin0, 1e-12), 1.0 / max(smin1, 1e-12))
-        rows[i] = np.concatenate((t0, t1, q0, q1, S0.ravel(), S1.ravel(), (3.0 * dx * inv_scale,)))
-    return poses
-
-
 class HipSolver(Solver):
     """Dense-domain MLS-MPM on one AMD GPU through libmpm_hip.so."""
 
@@ -148,7 +107,6 @@ class HipSolver(Solver):
         self._materials_desc = material_table(self.materials, domain, particles_per_cell)
         self.dims = tuple(self._domain_desc.dims)
         self._n_colliders = 0
-        self._poses = _PoseCache()
         handle = C.c_void_p()
         N.check(self._lib.mpm_create(self.ordinal, C.byref(self._domain_desc), self._materials_desc,
                                      len(self.materials), C.byref(handle)))
@@ -232,12 +190,12 @@ class HipSolver(Solver):
         self.set_collider_transforms(identity, identity)
 
     def set_collider_transforms(self, start: np.ndarray, end: np.ndarray) -> None:
-        start = np.asarray(start, np.float32).reshape(-1, 4, 4)
-        end = np.asarray(end, np.float32).reshape(-1, 4, 4)
+        start = np.ascontiguousarray(np.asarray(start, np.float32).reshape(-1, 4, 4))
+        end = np.ascontiguousarray(np.asarray(end, np.float32).reshape(-1, 4, 4))
         if len(start) != self._n_colliders or len(end) != self._n_colliders:
             raise ValueError(f"expected {self._n_colliders} collider transforms")
-        poses = collider_poses(start, end, self.dx, self._poses)
-        N.check(self._lib.mpm_set_collider_poses(self._ctx, self._n_colliders, poses))
+        # decomposed by the library in FP64 (geometry.decompose_transform's polar split, solver.py:258-260)
+        N.check(self._lib.mpm_set_collider_transforms(self._ctx, self._n_colliders, N.fptr(start), N.fptr(end)))
 
     # -- stepping --------------------------------------------------------------------------------
     def advance(self, frame_dt: float) -> AdvanceStats:

@@ -77,6 +77,9 @@ def _declare(L: C.CDLL) -> None:
         "mpm_get_state": ([ctx, fp, fp, fp, fp, fp, ip, ip], i32),
         "mpm_set_colliders": ([ctx, i32, ip, ip, fp, ip, ip, fp], i32),
         "mpm_set_collider_poses": ([ctx, i32, C.POINTER(ColliderPose)], i32),
+        "mpm_set_collider_transforms": ([ctx, i32, fp, fp], i32),
+        "mpm_decompose_transform": ([fp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)], i32),
         "mpm_advance": ([ctx, f64, C.POINTER(AdvanceStats)], i32),
         "mpm_substep": ([ctx, f64, f64, f64], i32),
         "mpm_is_occupied": ([ctx, i64, fp, u8], i32),

@@ -13,10 +13,12 @@
 //   k_p2g_cells    (one wave per occupied brick, one lane per cell, wave-private LDS window)
 //   k_grid_bricks  (dirty bricks: update, walls, colliders)
 //   k_g2p          (gather, advection, collider projection, return map, stress, max speed)
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -215,6 +217,7 @@ struct mpm_ctx {
   float* h_vmax = nullptr;
   float* h_vmax_dev = nullptr;    // its device address
   uint32_t vmax_seq = 0;          // the last request's sequence number
+  float* d_vmax = nullptr;        // device copy of the published speed
   bool vmax_pending = false;      // a reduction into h_vmax of the current state is on the stream
   float* d_block_max = nullptr;   // per-workgroup maxima of the last G2P (or speed pass)
   int64_t block_max_cap = 0;
@@ -370,6 +373,21 @@ struct Timed {
     }
   }
 };
+
+// One kernel launch of class k, timed (when the class is on) by events the dispatch records itself
+// (hipExtLaunchKernelGGL): no marker packets on the stream between the kernel and its neighbours, which
+// would add a few microseconds to every bracketed launch.
+template <typename... Args, typename F = void (*)(Args...)>
+void launch_timed(Timing& timing, hipStream_t stream, bool capturing, int k, F kernel, dim3 grid, dim3 block,
+                  Args... args) {
+  if (((timing.mask >> k) & 1u) && !capturing) {
+    hipEvent_t a = pool_event(timing), b = pool_event(timing);
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, a, b, 0, args...);
+    timing.pending.push_back({k, a, b});
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+  }
+}
 
 void collect_timing(Timing& t, hipStream_t stream) {
   if (t.pending.empty()) return;
@@ -641,7 +659,8 @@ DriftLists bin_for_p2g(mpm_ctx* c) {
 }
 
 // P2G; `pre` is the binning bin_for_p2g already enqueued for it (null: bin now)
-void run_p2g(mpm_ctx* c, float dt, const DriftLists* pre = nullptr) {
+// cfl.vmax: derive dt on the device (the advance's first substep, launched before the host knows dt)
+void run_p2g(mpm_ctx* c, float dt, const DriftLists* pre = nullptr, CflStep cfl = CflStep{}) {
   if (c->p2g_mode == MPM_P2G_ATOMIC) {
     if (c->n == 0) return;
     Timed t(c, MPM_K_P2G);
@@ -654,25 +673,17 @@ void run_p2g(mpm_ctx* c, float dt, const DriftLists* pre = nullptr) {
   if (c->n == 0) return;
   // one wave per workgroup, so a brick starts as soon as any wave slot frees; grid-stride over the
   // device-side brick list (costliest first)
-  Timed t(c, MPM_K_P2G);
   int64_t blocks = c->p2g_blocks > 0 ? c->p2g_blocks : 16384;
   blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, std::min<int64_t>(c->n_bricks, c->n)));
-  if (c->p2g_waves == 4) {
-    k_p2g_split<4><<<(unsigned)blocks, 256, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
-                                                            c->active, c->n_active, c->accum, c->dirty, c->fixed(),
-                                                            c->flush_stage(), dl);
-    check_launch("k_p2g_split");
-  } else if (c->p2g_waves == 2) {
-    k_p2g_split<2><<<(unsigned)blocks, 128, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
-                                                            c->active, c->n_active, c->accum, c->dirty, c->fixed(),
-                                                            c->flush_stage(), dl);
-    check_launch("k_p2g_split");
-  } else {
-    k_p2g_cells<1><<<(unsigned)blocks, 64, 0, c->stream>>>(c->parts(), c->geom, c->sc.p_vol, dt, c->offsets,
-                                                         c->active, c->n_active, c->accum, c->dirty, c->fixed(),
-                                                         c->flush_stage(), dl);
-    check_launch("k_p2g_cells");
-  }
+  const auto go = [&](auto kernel, unsigned threads) {
+    launch_timed(c->timing, c->stream, c->capturing, MPM_K_P2G, kernel, dim3((unsigned)blocks), dim3(threads),
+                 c->parts(), c->geom, c->sc.p_vol, dt, (const uint32_t*)c->offsets, (const uint32_t*)c->active,
+                 (const uint32_t*)c->n_active, c->accum, c->dirty, c->fixed(), c->flush_stage(), dl, cfl);
+  };
+  if (c->p2g_waves == 4) go(k_p2g_split<4>, 256);
+  else if (c->p2g_waves == 2) go(k_p2g_split<2>, 128);
+  else go(k_p2g_cells<1>, 64);
+  check_launch("k_p2g");
 }
 
 // the colliders' frames at frame fraction s (the same function the batched plan runs on the device)
@@ -716,13 +727,10 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_spee
     check_launch("k_zero_u32");
   }
   c->n_fix_cleared = false;
-  {
-    Timed t(c, MPM_K_G2P);
-    k_g2p<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->d_mats, c->geom, c->sc, (int)c->n, dt, c->vel,
-                                                    block_max(c), c->collider_set(), frames_at(c, s, inv_frame_dt),
-                                                    FixList{c->fix_list, c->n_fix});
-    check_launch("k_g2p");
-  }
+  launch_timed(c->timing, c->stream, c->capturing, MPM_K_G2P, k_g2p, dim3((unsigned)grid_for(c->n)), dim3(kBlock),
+               c->parts(), (const MatDev*)c->d_mats, c->geom, c->sc, (int)c->n, dt, (const float4*)c->vel,
+               block_max(c), c->collider_set(), frames_at(c, s, inv_frame_dt), FixList{c->fix_list, c->n_fix});
+  check_launch("k_g2p");
   const bool fused = fix && reduce_speed;
   if (fix) {
     // a persistent grid: the deferred particles are usually few
@@ -730,7 +738,7 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_spee
     if (fused) ++c->vmax_seq;
     k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 256), kBlock, 0, c->stream>>>(
         c->parts(), c->d_mats, c->fix_list, c->n_fix, block_max(c), (int)grid_for(c->n),
-        fused ? c->h_vmax_dev : nullptr, speed_seq_dev(c), c->vmax_seq);
+        fused ? c->h_vmax_dev : nullptr, speed_seq_dev(c), c->vmax_seq, c->d_vmax);
     check_launch("k_fp64_fixup");
   }
   c->block_max_count = grid_for(c->n);
@@ -738,10 +746,11 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_spee
   c->vmax_pending = fused;  // (a reduction enqueued before this G2P is of the old state)
 }
 
+// p2g_done: this substep's P2G is already on the stream (the advance's first, launched early)
 void substep(mpm_ctx* c, double dt, double s, double frame_dt, const DriftLists* pre = nullptr,
-             bool reduce_speed = false) {
+             bool reduce_speed = false, bool p2g_done = false) {
   const float dtf = (float)dt, sf = (float)s, inv_fdt = (float)(1.0 / frame_dt);
-  run_p2g(c, dtf, pre);
+  if (!p2g_done) run_p2g(c, dtf, pre);
   run_grid(c, dtf, sf, inv_fdt);
   run_g2p(c, dtf, sf, inv_fdt, reduce_speed);
   c->steps_since_sort += 1;
@@ -759,7 +768,8 @@ void reduce_speed_async(mpm_ctx* c, float* out, uint32_t* seq_out = nullptr, uin
     c->block_max_count = grid_for(c->n);
     c->vmax_valid = true;
   }
-  k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->d_block_max, (int)c->block_max_count, out, seq_out, seq);
+  k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->d_block_max, (int)c->block_max_count, out, seq_out, seq,
+                                            seq_out ? c->d_vmax : nullptr);
   check_launch("k_reduce_max");
 }
 
@@ -784,9 +794,12 @@ void request_speed(mpm_ctx* c) {
 float max_speed(mpm_ctx* c) {
   request_speed(c);
   if (c->n > 0) {
+    // (the stream is queried only after a second of spinning: a query puts a marker on the stream, which
+    // delays the next kernel by several microseconds)
     const volatile uint32_t* seq = reinterpret_cast<const volatile uint32_t*>(c->h_vmax + 1);
+    const auto t0 = std::chrono::steady_clock::now();
     for (uint64_t spins = 1; *seq != c->vmax_seq; ++spins) {
-      if (spins % 4096 == 0) {
+      if (spins % 65536 == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
         const hipError_t e = hipStreamQuery(c->stream);
         if (e == hipSuccess && *seq != c->vmax_seq) fail(MPM_ERR_RUNTIME, "speed reduction did not publish");
         if (e != hipSuccess && e != hipErrorNotReady) HIP_CHECK(e);
@@ -1795,6 +1808,7 @@ int32_t mpm_create(int32_t device, const mpm_domain_desc* domain, const mpm_mate
     c->h_vmax[0] = 0.0f;
     reinterpret_cast<uint32_t*>(c->h_vmax)[1] = 0u;
     HIP_CHECK(hipHostGetDevicePointer((void**)&c->h_vmax_dev, c->h_vmax, 0));
+    c->d_vmax = dev_alloc<float>(1);
     reserve(c, 256);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -1815,7 +1829,7 @@ int32_t mpm_destroy(mpm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_particles(c);
   for (void* p : {(void*)c->accum, (void*)c->vel, (void*)c->dirty, (void*)c->live, (void*)c->offsets,
-                  (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_mats, (void*)c->d_block_max, (void*)c->d_cols,
+                  (void*)c->active, (void*)c->dirty_list, (void*)c->slot, (void*)c->stage, (void*)c->in_cnt, (void*)c->in_lst, (void*)c->fx, (void*)c->n_active, (void*)c->n_fix, (void*)c->d_vmax, (void*)c->d_mats, (void*)c->d_block_max, (void*)c->d_cols,
                   (void*)c->d_tris, (void*)c->d_normals, (void*)c->d_cell_start, (void*)c->d_cell_tris,
                   (void*)c->d_cell_key, (void*)c->d_cell_far})
     dev_free(p);
@@ -1927,6 +1941,150 @@ int32_t mpm_set_colliders(mpm_ctx* c, int32_t n, const int32_t* n_verts, const i
   });
 }
 
+// ---- collider poses from world matrices (solver.py:154-172, 258-260; geometry.py:30-49) -----------------
+// M = [A | t]: translation t, rotation R and symmetric stretch S with A = R S (the polar decomposition; a
+// mirrored A keeps R a proper rotation and its reflection goes into S through the sign of the smallest
+// singular value), from an SVD of A: cyclic Jacobi on A^T A in FP64 for V and the singular values, then
+// U from A V (Gram-Schmidt, third column the cross product, so U is proper and the third singular value
+// carries the sign of det A). R = U V^T, S = V diag(sig) V^T; R becomes a quaternion by the reference's
+// branch-free conversion. The margin's scale is the smallest unsigned singular value.
+struct PoseParts {
+  double t[3], q[4], S[9], smin;
+};
+
+namespace {
+
+void sym_eigen3(double B[3][3], double V[3][3], double lam[3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) V[i][j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    const double off = B[0][1] * B[0][1] + B[0][2] * B[0][2] + B[1][2] * B[1][2];
+    const double diag = B[0][0] * B[0][0] + B[1][1] * B[1][1] + B[2][2] * B[2][2];
+    if (off <= 1e-36 * diag || off == 0.0) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (B[p][q] == 0.0) continue;
+        const double theta = (B[q][q] - B[p][p]) / (2.0 * B[p][q]);
+        const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / std::sqrt(tt * tt + 1.0), sn = tt * cs;
+        for (int k = 0; k < 3; ++k) {  // B <- B J (columns p, q)
+          const double bp = B[k][p], bq = B[k][q];
+          B[k][p] = cs * bp - sn * bq;
+          B[k][q] = sn * bp + cs * bq;
+        }
+        for (int k = 0; k < 3; ++k) {  // B <- J^T B (rows p, q)
+          const double bp = B[p][k], bq = B[q][k];
+          B[p][k] = cs * bp - sn * bq;
+          B[q][k] = sn * bp + cs * bq;
+        }
+        for (int k = 0; k < 3; ++k) {  // V <- V J
+          const double vp = V[k][p], vq = V[k][q];
+          V[k][p] = cs * vp - sn * vq;
+          V[k][q] = sn * vp + cs * vq;
+        }
+      }
+  }
+  for (int i = 0; i < 3; ++i) lam[i] = B[i][i];
+}
+
+// m: row-major 4x4 world matrix
+PoseParts decompose_pose(const float* m) {
+  PoseParts out{};
+  double A[3][3];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) A[i][j] = (double)m[4 * i + j];
+    out.t[i] = (double)m[4 * i + 3];
+  }
+  double B[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) B[i][j] = A[0][i] * A[0][j] + A[1][i] * A[1][j] + A[2][i] * A[2][j];
+  double V[3][3], lam[3];
+  sym_eigen3(B, V, lam);
+  int order[3] = {0, 1, 2};  // descending eigenvalues
+  std::sort(order, order + 3, [&](int a, int b) { return lam[a] > lam[b]; });
+  double v[3][3];  // v[i]: the i-th right singular vector
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) v[i][k] = V[k][order[i]];
+  const auto cross = [](const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  const auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  double c3[3];
+  cross(v[0], v[1], c3);
+  if (dot(c3, v[2]) < 0.0)
+    for (int k = 0; k < 3; ++k) v[2][k] = -v[2][k];  // det V = +1
+  double Av[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) Av[i][k] = A[k][0] * v[i][0] + A[k][1] * v[i][1] + A[k][2] * v[i][2];
+  double u[3][3];
+  const auto unit_or = [&](double* w, const double* fallback) {
+    const double nrm = std::sqrt(dot(w, w));
+    if (nrm > 0.0) {
+      for (int k = 0; k < 3; ++k) w[k] /= nrm;
+    } else {
+      for (int k = 0; k < 3; ++k) w[k] = fallback[k];
+    }
+  };
+  for (int k = 0; k < 3; ++k) u[0][k] = Av[0][k];
+  unit_or(u[0], v[0]);
+  const double p01 = dot(u[0], Av[1]);
+  for (int k = 0; k < 3; ++k) u[1][k] = Av[1][k] - p01 * u[0][k];
+  {
+    // a rank-1 A leaves u1 free: any unit vector orthogonal to u0
+    double e[3] = {0.0, 0.0, 0.0};
+    e[std::fabs(u[0][0]) < 0.9 ? 0 : 1] = 1.0;
+    double w[3];
+    cross(u[0], e, w);
+    unit_or(w, e);
+    unit_or(u[1], w);
+  }
+  cross(u[0], u[1], u[2]);
+  const double sig[3] = {dot(u[0], Av[0]), dot(u[1], Av[1]), dot(u[2], Av[2])};
+  out.smin = std::sqrt(std::max(lam[order[2]], 0.0));
+  double R[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      R[i][j] = u[0][i] * v[0][j] + u[1][i] * v[1][j] + u[2][i] * v[2][j];
+      out.S[3 * i + j] = sig[0] * v[0][i] * v[0][j] + sig[1] * v[1][i] * v[1][j] + sig[2] * v[2][i] * v[2][j];
+    }
+  const double d0 = R[0][0], d1 = R[1][1], d2 = R[2][2];
+  double q[4] = {std::copysign(0.5 * std::sqrt(std::max(0.0, 1.0 + d0 - d1 - d2)), R[2][1] - R[1][2]),
+                 std::copysign(0.5 * std::sqrt(std::max(0.0, 1.0 - d0 + d1 - d2)), R[0][2] - R[2][0]),
+                 std::copysign(0.5 * std::sqrt(std::max(0.0, 1.0 - d0 - d1 + d2)), R[1][0] - R[0][1]),
+                 0.5 * std::sqrt(std::max(0.0, 1.0 + d0 + d1 + d2))};
+  const double qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  for (int k = 0; k < 4; ++k) out.q[k] = q[k] / qn;
+  return out;
+}
+
+// the decomposed start/end poses of one collider and its AABB margin (solver.py:160-172, 258-260)
+mpm_collider_pose pose_from_matrices(const float* start, const float* end, double dx) {
+  const PoseParts a = decompose_pose(start);
+  PoseParts b = decompose_pose(end);
+  if (a.q[0] * b.q[0] + a.q[1] * b.q[1] + a.q[2] * b.q[2] + a.q[3] * b.q[3] < 0.0)
+    for (double& x : b.q) x = -x;  // slerp the short way round
+  mpm_collider_pose p{};
+  for (int k = 0; k < 3; ++k) {
+    p.t0[k] = (float)a.t[k];
+    p.t1[k] = (float)b.t[k];
+  }
+  for (int k = 0; k < 4; ++k) {
+    p.q0[k] = (float)a.q[k];
+    p.q1[k] = (float)b.q[k];
+  }
+  for (int k = 0; k < 9; ++k) {
+    p.S0[k] = (float)a.S[k];
+    p.S1[k] = (float)b.S[k];
+  }
+  const double inv_scale = std::max(1.0 / std::max(a.smin, 1e-12), 1.0 / std::max(b.smin, 1e-12));
+  p.margin = (float)(3.0 * dx * inv_scale);
+  return p;
+}
+
+}  // namespace
+
 int32_t mpm_set_collider_poses(mpm_ctx* c, int32_t n, const mpm_collider_pose* poses) {
   if (!c) return MPM_ERR_VALUE;
   return run(c, [&] {
@@ -1958,6 +2116,33 @@ int32_t mpm_set_collider_poses(mpm_ctx* c, int32_t n, const mpm_collider_pose* p
   });
 }
 
+int32_t mpm_set_collider_transforms(mpm_ctx* c, int32_t n, const float* start, const float* end) {
+  if (!c) return MPM_ERR_VALUE;
+  std::vector<mpm_collider_pose> poses;
+  const int32_t rc = run(c, [&] {
+    if (n != (int32_t)c->colliders.size())
+      fail(MPM_ERR_VALUE, "expected " + std::to_string(c->colliders.size()) + " collider transforms");
+    if (n > 0 && (!start || !end)) fail(MPM_ERR_VALUE, "null transform array");
+    for (int32_t i = 0; i < n; ++i)
+      for (int k = 0; k < 16; ++k)
+        if (!std::isfinite(start[16 * i + k]) || !std::isfinite(end[16 * i + k]))
+          fail(MPM_ERR_VALUE, "non-finite collider transform");
+    for (int32_t i = 0; i < n; ++i) poses.push_back(pose_from_matrices(start + 16 * i, end + 16 * i, c->dom.dx_f64));
+  });
+  if (rc != MPM_OK) return rc;
+  return mpm_set_collider_poses(c, n, poses.data());
+}
+
+int32_t mpm_decompose_transform(const float* m, double* t, double* q, double* S, double* smin) {
+  if (!m || !t || !q || !S || !smin) return MPM_ERR_VALUE;
+  const PoseParts p = decompose_pose(m);
+  std::memcpy(t, p.t, sizeof p.t);
+  std::memcpy(q, p.q, sizeof p.q);
+  std::memcpy(S, p.S, sizeof p.S);
+  *smin = p.smin;
+  return MPM_OK;
+}
+
 int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
   if (!c || !out) return MPM_ERR_VALUE;
   return run(c, [&] {
@@ -1970,20 +2155,24 @@ int32_t mpm_advance(mpm_ctx* c, double frame_dt, mpm_advance_stats* out) {
     // valid frame_dt always gives at least one substep, which consumes the binning enqueued below
     if (frame_dt == 0.0 || !std::isfinite(frame_dt)) fail(MPM_ERR_VALUE, "frame_dt must be finite and nonzero");
     c->frame_dt = frame_dt;
-    // The first substep's binning needs the positions but not dt: it goes on the stream behind the speed
-    // reduction, so the GPU works through it while the host waits for the speed and picks dt.
-    request_speed(c);
-    const DriftLists pre = c->p2g_mode == MPM_P2G_ATOMIC ? DriftLists{} : bin_for_p2g(c);
-    double vmax = (double)max_speed(c);
-    if (!std::isfinite(vmax)) vmax = 0.0;  // already blown up: keep stepping so the NaN reaches x
     double wave = 0.0;
     for (size_t i = 0; i < c->mats.size(); ++i)
       if (c->present[i]) wave = std::max(wave, c->mats[i].wave_speed);
+    // The first substep's binning needs the positions but not dt, and its P2G only dt, which it derives
+    // on the device from the reduced speed exactly as below (cfl_dt): both go on the stream behind the
+    // speed reduction, so the GPU works through them while the host waits for the speed and picks dt.
+    request_speed(c);
+    const DriftLists pre = c->p2g_mode == MPM_P2G_ATOMIC ? DriftLists{} : bin_for_p2g(c);
+    const bool early_p2g = c->p2g_mode != MPM_P2G_ATOMIC;
+    if (early_p2g) run_p2g(c, 0.0f, &pre, CflStep{c->d_vmax, wave, c->dom.dx_f64, frame_dt});
+    double vmax = (double)max_speed(c);
+    if (!std::isfinite(vmax)) vmax = 0.0;  // already blown up: keep stepping so the NaN reaches x
     const double cfl_dt = std::min(0.4 * c->dom.dx_f64 / (wave + vmax), frame_dt);
     const int64_t steps = (int64_t)std::ceil(frame_dt / cfl_dt - 1e-9);
     const double dt = frame_dt / (double)steps;
     for (int64_t k = 0; k < steps; ++k)
-      substep(c, dt, (double)(k + 1) / (double)steps, frame_dt, k == 0 ? &pre : nullptr, k == steps - 1);
+      substep(c, dt, (double)(k + 1) / (double)steps, frame_dt, k == 0 ? &pre : nullptr, k == steps - 1,
+              k == 0 && early_p2g);
     // the removal count needs the positions (a host wait, only with open sides); otherwise the substeps
     // stay in flight and the host prepares the next call meanwhile (every read of the state waits for them)
     const int64_t removed = remove_escaped(c);

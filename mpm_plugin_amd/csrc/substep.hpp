@@ -462,6 +462,24 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
   else wave_lds_fence();
 }
 
+// The CFL step of an advance (solver.py:176-185) from the reduced speed, in FP64 with the host's operations
+// in the host's order (no contraction), so it equals the dt the host derives from the same speed: the
+// advance's first P2G derives it itself and can start before the host has read the speed.
+struct CflStep {
+  const float* vmax;  // the reduced speed (device copy); null: use the dt argument
+  double wave, dx, frame_dt;
+};
+
+__device__ __forceinline__ float cfl_dt(const CflStep& c) {
+#pragma clang fp contract(off)
+  double vmax = (double)*c.vmax;
+  if (!isfinite(vmax)) vmax = 0.0;
+  const double a = 0.4 * c.dx / (c.wave + vmax);
+  const double cfl = c.frame_dt < a ? c.frame_dt : a;
+  const double steps = ceil(c.frame_dt / cfl - 1e-9);
+  return (float)(c.frame_dt / steps);
+}
+
 // P2G over the device-side list of occupied bricks: kWaves waves (bricks at a time) per workgroup,
 // grid-stride, so the launch needs no host-side count
 template <int kWaves>
@@ -470,10 +488,12 @@ __global__ void __launch_bounds__(64 * kWaves) k_p2g_cells(Particles P, GridGeom
                                                    const uint32_t* __restrict__ active,
                                                    const uint32_t* __restrict__ n_active_ptr,
                                                    float4* __restrict__ accum, uint32_t* __restrict__ dirty,
-                                                   FixedAccum fa, float4* __restrict__ stage, DriftLists dl) {
+                                                   FixedAccum fa, float4* __restrict__ stage, DriftLists dl,
+                                                   CflStep cfl) {
   __shared__ float s_win[kWaves][4][kWinNodes];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t n_active = *n_active_ptr;
+  if (cfl.vmax) dt = cfl_dt(cfl);
   const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
   for (uint32_t w = blockIdx.x * kWaves + wave; w < n_active; w += gridDim.x * kWaves)
     p2g_brick<1>(P, g, k, cell_off, active[w], &s_win[wave], accum, dirty, lane, fa, stage, w, dl);
@@ -488,10 +508,12 @@ __global__ void __launch_bounds__(64 * kSplit) k_p2g_split(Particles P, GridGeom
                                                            const uint32_t* __restrict__ active,
                                                            const uint32_t* __restrict__ n_active_ptr,
                                                            float4* __restrict__ accum, uint32_t* __restrict__ dirty,
-                                                           FixedAccum fa, float4* __restrict__ stage, DriftLists dl) {
+                                                           FixedAccum fa, float4* __restrict__ stage, DriftLists dl,
+                                                           CflStep cfl) {
   __shared__ float s_win[kSplit][4][kWinNodes];
   const int half = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t n_active = *n_active_ptr;
+  if (cfl.vmax) dt = cfl_dt(cfl);  // the advance's first P2G: its CFL step derived here
   const float k = -dt * p_vol * 4.0f * g.inv_dx * g.inv_dx;
   for (uint32_t w = blockIdx.x; w < n_active; w += gridDim.x)
     p2g_brick<kSplit>(P, g, k, cell_off, active[w], s_win, accum, dirty, lane, fa, stage, w, dl, half);
@@ -738,8 +760,9 @@ __device__ __forceinline__ void material_update_fp64(const MatDev& m, Particles&
 // a 1M-particle G2P's 3,990 maxima paid the L2 latency 16 times in a row on the substep's critical path.
 // With seq_out, the maximum is published to the host: after *out, a system-scope fence, then *seq_out = seq,
 // so a host that sees seq also sees the maximum (and needs no event behind the kernel).
+// dev_out (optional): a device copy of the maximum, for a kernel that derives the next CFL step from it.
 __device__ void reduce_block_maxima(const float* __restrict__ block_max, int count, float* __restrict__ out,
-                                    uint32_t* seq_out = nullptr, uint32_t seq = 0u) {
+                                    uint32_t* seq_out = nullptr, uint32_t seq = 0u, float* dev_out = nullptr) {
   constexpr int kInFlight = 8;
   float part[kInFlight];
 #pragma unroll
@@ -760,7 +783,9 @@ __device__ void reduce_block_maxima(const float* __restrict__ block_max, int cou
   if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = wm;
   __syncthreads();
   if (threadIdx.x == 0) {
-    *out = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+    const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+    *out = m;
+    if (dev_out) *dev_out = m;
     if (seq_out) {
       __threadfence_system();
       *static_cast<volatile uint32_t*>(seq_out) = seq;
@@ -777,8 +802,8 @@ __global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* _
                                                     const uint32_t* __restrict__ count,
                                                     const float* __restrict__ block_max, int block_count,
                                                     float* __restrict__ vmax_out, uint32_t* seq_out,
-                                                    uint32_t seq) {
-  if (vmax_out && blockIdx.x == 0) reduce_block_maxima(block_max, block_count, vmax_out, seq_out, seq);
+                                                    uint32_t seq, float* vmax_dev) {
+  if (vmax_out && blockIdx.x == 0) reduce_block_maxima(block_max, block_count, vmax_out, seq_out, seq, vmax_dev);
   const uint32_t n = *count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t p = list[i];
@@ -955,8 +980,9 @@ __global__ void __launch_bounds__(256) k_speed_partials(Particles P, int n, floa
 
 // one workgroup: the maximum of the per-workgroup maxima
 __global__ void __launch_bounds__(256) k_reduce_max(const float* __restrict__ block_max, int count,
-                                                    float* __restrict__ out, uint32_t* seq_out, uint32_t seq) {
-  reduce_block_maxima(block_max, count, out, seq_out, seq);
+                                                    float* __restrict__ out, uint32_t* seq_out, uint32_t seq,
+                                                    float* vmax_dev) {
+  reduce_block_maxima(block_max, count, out, seq_out, seq, vmax_dev);
 }
 
 }  // namespace mpm
