@@ -228,6 +228,7 @@ struct mpm_ctx {
   Timing timing;
   double frame_dt = 1.0;
   bool capturing = false;  // inside a graph capture: no events, no allocations, no host waits
+  bool drift_ahead = false;  // the last G2P's follow-up kernel routed the drift for the next substep
 
   Particles parts() const { return A.view(); }
   FixedAccum fixed() const {
@@ -642,19 +643,35 @@ const uint32_t* ctx_sorted_keys(const mpm_ctx* c) {
 // ---- substep phases --------------------------------------------------------------------------
 // The binning the next binned P2G reads: a sort every sort_interval substeps, else (BINNED) the drift
 // lists. It reads positions only, so it can be enqueued before the substep's dt is known.
+// Drift lists routed ahead (by the kernel after the last G2P) for a substep that then does not read them
+// (the state changed, or the batched engine takes the scene over): their counts would add to the next
+// routing, which P2G clears only after reading it, so they are cleared here.
+void discard_drift_ahead(mpm_ctx* c) {
+  if (!c->drift_ahead) return;
+  if (c->in_cnt) HIP_CHECK(hipMemsetAsync(c->in_cnt, 0, (size_t)c->stage_cap * 64 * sizeof(uint32_t), c->stream));
+  c->drift_ahead = false;
+}
+
 DriftLists bin_for_p2g(mpm_ctx* c) {
   const bool sorting = !c->sorted || c->steps_since_sort >= c->sort_interval;
-  if (sorting) sort_particles(c);
+  if (sorting) {
+    discard_drift_ahead(c);
+    sort_particles(c);
+  }
   // between sorts, drifted particles go to their current cell's lane (not in fixed point: the lists'
-  // order is that of atomics, and the fixed-point sums must not depend on it)
+  // order is that of atomics, and the fixed-point sums must not depend on it); usually the kernel after
+  // the last G2P has routed them already
   DriftLists dl{};
   if (!sorting && c->n > 0 && c->p2g_mode == MPM_P2G_BINNED) {
-    Timed t(c, MPM_K_OTHER);
-    k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)c->n, ctx_sorted_keys(c), c->slot,
-                                                      c->in_cnt, c->in_lst, c->fwd);
-    check_launch("k_drift");
+    if (!c->drift_ahead) {
+      Timed t(c, MPM_K_OTHER);
+      k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)c->n, ctx_sorted_keys(c),
+                                                        c->slot, c->in_cnt, c->in_lst, c->fwd);
+      check_launch("k_drift");
+    }
     dl = DriftLists{c->in_cnt, c->in_lst, c->fwd};
   }
+  c->drift_ahead = false;
   return dl;
 }
 
@@ -732,14 +749,22 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_spee
                block_max(c), c->collider_set(), frames_at(c, s, inv_frame_dt), FixList{c->fix_list, c->n_fix});
   check_launch("k_g2p");
   const bool fused = fix && reduce_speed;
-  if (fix) {
+  // Whether the next substep will route drift (no sort before it, state unchanged): then the kernel after
+  // this G2P routes it, while the positions G2P wrote are in the cache, and the substep needs no k_drift.
+  const bool drift_next = c->p2g_mode == MPM_P2G_BINNED && c->sorted && c->steps_since_sort + 1 < c->sort_interval;
+  if (fix || drift_next) {
     // a persistent grid: the deferred particles are usually few
     Timed t(c, MPM_K_OTHER);
     if (fused) ++c->vmax_seq;
-    k_fp64_fixup<<<(unsigned)std::min<int64_t>(grid_for(c->n), 256), kBlock, 0, c->stream>>>(
-        c->parts(), c->d_mats, c->fix_list, c->n_fix, block_max(c), (int)grid_for(c->n),
-        fused ? c->h_vmax_dev : nullptr, speed_seq_dev(c), c->vmax_seq, c->d_vmax);
+    const DriftAhead da = drift_next ? DriftAhead{c->geom, (int)c->n, ctx_sorted_keys(c), c->slot, c->in_cnt,
+                                                  c->in_lst, c->fwd}
+                                     : DriftAhead{};
+    const int64_t blocks = drift_next ? std::min<int64_t>(grid_for(c->n), 2048) : std::min<int64_t>(grid_for(c->n), 256);
+    k_fp64_fixup<<<(unsigned)blocks, kBlock, 0, c->stream>>>(
+        c->parts(), c->d_mats, c->fix_list, fix ? c->n_fix : nullptr, block_max(c), (int)grid_for(c->n),
+        fused ? c->h_vmax_dev : nullptr, speed_seq_dev(c), c->vmax_seq, c->d_vmax, da);
     check_launch("k_fp64_fixup");
+    c->drift_ahead = drift_next;
   }
   c->block_max_count = grid_for(c->n);
   c->vmax_valid = true;
@@ -1446,6 +1471,7 @@ void batch_begin(mpm_batch* b, const double* frame_dt) {
     st.steps = (int32_t)steps;
     st.since_sort = (int32_t)std::min<int64_t>(c->steps_since_sort, 1 << 30);
     st.sorted = c->sorted ? 1 : 0;
+    discard_drift_ahead(c);  // the batch routes drift itself (its stream orders after the context's)
     c->busy = true;
   }
 }

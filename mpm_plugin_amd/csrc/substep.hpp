@@ -244,6 +244,16 @@ __device__ __forceinline__ float row_shift(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, true));
 }
 
+// The drift routing of the next substep done by the kernel after G2P (k_fp64_fixup): keys == null for none.
+struct DriftAhead {
+  GridGeom g;
+  int n;
+  const uint32_t* keys;
+  const int32_t* slot;
+  uint32_t *in_cnt, *in_lst;
+  uint8_t* fwd;
+};
+
 // orders this wave's LDS accesses across lanes (a wave's LDS operations complete in issue order)
 __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -802,13 +812,17 @@ __global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* _
                                                     const uint32_t* __restrict__ count,
                                                     const float* __restrict__ block_max, int block_count,
                                                     float* __restrict__ vmax_out, uint32_t* seq_out,
-                                                    uint32_t seq, float* vmax_dev) {
+                                                    uint32_t seq, float* vmax_dev, DriftAhead da) {
   if (vmax_out && blockIdx.x == 0) reduce_block_maxima(block_max, block_count, vmax_out, seq_out, seq, vmax_dev);
-  const uint32_t n = *count;
+  const uint32_t n = count ? *count : 0u;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t p = list[i];
     material_update_fp64(mats[P.mat[p]], P, p);
   }
+  // the next substep's drift routing (it reads only the positions G2P just wrote)
+  if (da.keys)
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < da.n; p += gridDim.x * blockDim.x)
+      drift_route(P, da.g, p, da.keys, da.slot, da.in_cnt, da.in_lst, da.fwd);
 }
 
 // c_stash: this thread's 9 LDS words, `stride` apart (one column per thread of the workgroup: no bank
