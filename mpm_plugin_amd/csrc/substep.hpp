@@ -408,8 +408,10 @@ __device__ __forceinline__ void p2g_brick(const Particles& P, const GridGeom& g,
     }
   // the brick's window: this wave's, or the sum of the workgroup's (each thread then takes every
   // (64 kSplit)-th node)
-  if (kSplit > 1) __syncthreads();  // every wave's window is complete
-  // whether any lane of the brick's workgroup wrote the outer shell (uniform over the workgroup)
+  // Whether any lane of the brick's workgroup wrote the outer shell (uniform over the workgroup). With
+  // several waves the vote is also the barrier after which every wave's window is complete (each wave's
+  // LDS writes are made visible to the workgroup first): one barrier instead of two (1 us per launch).
+  if (kSplit > 1) __threadfence_block();
   const bool any_shell = kSplit > 1 ? __syncthreads_or(shell) != 0 : __ballot(shell) != 0;
   const int tid = half * 64 + lane;
   const auto node_value = [&](int i, int c) {
