@@ -11,8 +11,8 @@ For each substep k of an FP64 oracle trajectory, the GPU is loaded with the orac
 substep in each mode; the error is ||X - X64|| / ||X64|| for the grid momentum after P2G and for the
 particle F after the substep. Modes: FP32 (every SVD and return map in FP32), the FP64 fallback (FP64 only
 for sigma_min/sigma_max < 1e-2 or J = Jp det F < 0.25), FP64 (every SVD and return map), and the
-fixed-point P2G (with the fallback). Set MPM_PRECISION_REPORT=path to write the errors as JSON and MPM_PRECISION_N to
-change the particle count per state (default 60,000; 500,000 for the full-size report).
+fixed-point P2G (with the fallback). The states have the configuration's full size, about 500,000 particles
+each (MPM_PRECISION_N changes it); set MPM_PRECISION_REPORT=path to write the errors as JSON.
 """
 
 from __future__ import annotations
@@ -31,7 +31,7 @@ from mpm_plugin_amd import workloads
 pytestmark = pytest.mark.gpu
 
 RES = 128
-N_STATE = int(os.environ.get("MPM_PRECISION_N", "60000"))
+N_STATE = int(os.environ.get("MPM_PRECISION_N", "500000"))
 SUBSTEPS = 3
 DT = 5.0e-5
 SNOW = MaterialParams("snow", "snow", 1.4e5, 0.2, 400.0, hardening=10.0, critical_compression=2.5e-2,
