@@ -550,15 +550,36 @@ void state_changed(mpm_ctx* c) {
 }
 
 // ---- device-wide exclusive scan --------------------------------------------------------------
+// The scan's buffer for `tiles` tiles: word 0 is k_scan_tiles' ticket, the tile totals follow. The ticket
+// is cleared on `stream` when the buffer is (re)allocated, and the kernel leaves it at zero. A buffer that
+// a graph capture would have to grow is an error (the batch sizes its buffer before it captures).
+uint32_t* scan_buffer(Scratch& s, int64_t tiles, hipStream_t stream) {
+  const size_t need = (size_t)(tiles + 1) * sizeof(uint32_t);
+  if (need > s.bytes) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    HIP_CHECK(hipStreamIsCapturing(stream, &st));
+    if (st != hipStreamCaptureStatusNone) fail(MPM_ERR_RUNTIME, "scan buffer grown during a graph capture");
+    s.get(need);
+    HIP_CHECK(hipMemsetAsync(s.ptr, 0, sizeof(uint32_t), stream));
+  }
+  return static_cast<uint32_t*>(s.ptr);
+}
+
+// the two-part scan of n words (k_scan_tiles): tile-local scans into out; returns the scanned tile totals
+const uint32_t* scan_u32_tiles(Scratch& scan_sums, hipStream_t stream, const uint32_t* in, uint32_t* out,
+                               int64_t n) {
+  const int64_t tiles = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
+  if (tiles > kScanTile) fail(MPM_ERR_VALUE, "scan too large");
+  uint32_t* buf = scan_buffer(scan_sums, tiles, stream);
+  k_scan_tiles<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, (int)n, out, buf + 1, buf);
+  check_launch("k_scan_tiles");
+  return buf + 1;
+}
+
 void scan_u32(Scratch& scan_sums, hipStream_t stream, const uint32_t* in, uint32_t* out, int64_t n) {
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-  if (tiles > kScanTile) fail(MPM_ERR_VALUE, "scan too large");
-  uint32_t* sums = static_cast<uint32_t*>(scan_sums.get((tiles + 1) * sizeof(uint32_t)));
-  k_scan_tiles<<<(unsigned)tiles, kScanThreads, 0, stream>>>(in, (int)n, out, sums);
-  check_launch("k_scan_tiles");
+  const uint32_t* sums = scan_u32_tiles(scan_sums, stream, in, out, n);
   if (tiles > 1) {
-    k_scan_top<<<1, kScanThreads, 0, stream>>>(sums, (int)tiles);
-    check_launch("k_scan_top");
     k_scan_add<<<(unsigned)tiles, kScanThreads, 0, stream>>>(out, (int)n, sums);
     check_launch("k_scan_add");
   }
@@ -620,8 +641,9 @@ void sort_particles(mpm_ctx* c) {
     uint32_t* vout = (pass % 2 == 0) ? c->vals1 : c->vals2;
     k_radix_hist<<<tiles, kSortThreads, 0, c->stream>>>(kin, (int)n, 8 * pass, tiles, hist);
     check_launch("k_radix_hist");
-    scan_u32(c->scan_sums, c->stream, hist, hist, (int64_t)256 * tiles);
-    k_radix_scatter<<<tiles, kSortThreads, 0, c->stream>>>(kin, vin, (int)n, 8 * pass, tiles, hist, kout, vout);
+    const uint32_t* sums = scan_u32_tiles(c->scan_sums, c->stream, hist, hist, (int64_t)256 * tiles);
+    k_radix_scatter<<<tiles, kSortThreads, 0, c->stream>>>(kin, vin, (int)n, 8 * pass, tiles, hist, sums, kout,
+                                                           vout);
     check_launch("k_radix_scatter");
     kin = kout;
     vin = vout;
@@ -665,8 +687,9 @@ DriftLists bin_for_p2g(mpm_ctx* c) {
   if (!sorting && c->n > 0 && c->p2g_mode == MPM_P2G_BINNED) {
     if (!c->drift_ahead) {
       Timed t(c, MPM_K_OTHER);
-      k_drift<<<grid_for(c->n), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)c->n, ctx_sorted_keys(c),
-                                                        c->slot, c->in_cnt, c->in_lst, c->fwd);
+      k_drift<<<grid_for((c->n + 1) / 2), kBlock, 0, c->stream>>>(c->parts(), c->geom, (int)c->n,
+                                                                  ctx_sorted_keys(c), c->slot, c->in_cnt,
+                                                                  c->in_lst, c->fwd);
       check_launch("k_drift");
     }
     dl = DriftLists{c->in_cnt, c->in_lst, c->fwd};
@@ -759,7 +782,7 @@ void run_g2p(mpm_ctx* c, float dt, float s, float inv_frame_dt, bool reduce_spee
     const DriftAhead da = drift_next ? DriftAhead{c->geom, (int)c->n, ctx_sorted_keys(c), c->slot, c->in_cnt,
                                                   c->in_lst, c->fwd}
                                      : DriftAhead{};
-    const int64_t blocks = drift_next ? std::min<int64_t>(grid_for(c->n), 2048) : std::min<int64_t>(grid_for(c->n), 256);
+    const int64_t blocks = drift_next ? std::min<int64_t>(grid_for((c->n + 1) / 2), 2048) : std::min<int64_t>(grid_for(c->n), 256);
     k_fp64_fixup<<<(unsigned)blocks, kBlock, 0, c->stream>>>(
         c->parts(), c->d_mats, c->fix_list, fix ? c->n_fix : nullptr, block_max(c), (int)grid_for(c->n),
         fused ? c->h_vmax_dev : nullptr, speed_seq_dev(c), c->vmax_seq, c->d_vmax, da);
@@ -1270,7 +1293,7 @@ void batch_layout(mpm_batch* b) {
   batch_drop_graphs(b);
   // sized before any capture: nothing allocates inside one
   b->hist.get(256 * tiles * sizeof(uint32_t));
-  b->scan_sums.get((ceil_div(256 * tiles, kScanTile) + 1) * sizeof(uint32_t));
+  scan_buffer(b->scan_sums, ceil_div(256 * tiles, kScanTile), b->stream);
   b->hist_tiles = tiles;
   b->max_passes = passes;
 }
@@ -1297,8 +1320,8 @@ void batch_enqueue(mpm_batch* b, bool sorting) {
     for (int pass = 0; pass < b->max_passes; ++pass) {
       k_batch_radix_hist<<<1024, kSortThreads, 0, s>>>(b->d_scenes, b->d_pre, n, pass, hist);
       check_launch("k_batch_radix_hist");
-      scan_u32(b->scan_sums, s, hist, hist, 256 * b->hist_tiles);
-      k_batch_radix_scatter<<<1024, kSortThreads, 0, s>>>(b->d_scenes, b->d_pre, n, pass, hist);
+      const uint32_t* sums = scan_u32_tiles(b->scan_sums, s, hist, hist, 256 * b->hist_tiles);
+      k_batch_radix_scatter<<<1024, kSortThreads, 0, s>>>(b->d_scenes, b->d_pre, n, pass, hist, sums);
       check_launch("k_batch_radix_scatter");
     }
     k_batch_cell_bounds<<<4096, 256, 0, s>>>(b->d_scenes, b->d_pre, n);

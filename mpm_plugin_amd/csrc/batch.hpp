@@ -120,32 +120,6 @@ __device__ __forceinline__ int scene_of(const uint32_t* __restrict__ pre, int n_
 
 __device__ __forceinline__ uint32_t ceil_div_u(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-// Called by every thread of every workgroup of a kernel at its end (no workgroup may have returned
-// early): true, in all threads of one workgroup, for the workgroup that finishes last. `done` counts
-// tickets; it starts at 0 and the last workgroup resets it for the kernel's next launch. Nothing waits: a
-// workgroup that is not last just ends. `s_flag` is a shared word used for nothing else.
-// What the last workgroup may read of the other workgroups' work: counters they updated with returning
-// agent-scope atomics (each wave uses the returned value, so the update is performed before the wave
-// reaches the barrier below, hence before its workgroup's ticket), read back with load_agent. There is
-// no device-scope fence: on gfx950 one writes back the XCD's L2, and a fence per wave of a persistent
-// grid cost hundreds of microseconds per substep. Plain stores of this kernel reach the next kernel
-// through the kernel boundary as usual.
-__device__ __forceinline__ bool last_workgroup(uint32_t* done, uint32_t* s_flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const bool last = atomicAdd(done, 1u) == gridDim.x - 1;
-    if (last) atomicExch(done, 0u);
-    *s_flag = last ? 1u : 0u;
-  }
-  __syncthreads();
-  return *s_flag != 0u;
-}
-
-// a counter updated by other workgroups' agent-scope atomics, read at the coherence point those use
-__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Whole workgroup: pre[0..n_scenes] = exclusive prefix of count(i) over the scenes (pre[n_scenes] is the
 // total). Any workgroup size that is a multiple of 64.
 template <typename Count>
@@ -297,11 +271,13 @@ __global__ void __launch_bounds__(kSortThreads) k_batch_radix_hist(const SceneDe
   }
 }
 
-// stable scatter of one radix tile. `offsets` is the exclusive scan of the whole histogram buffer, so the
-// scene's own exclusive scan is offsets[e] - offsets[first entry of its segment] (modulo 2^32, exact).
+// stable scatter of one radix tile. (offsets, tile_sums) is the two-part exclusive scan of the whole
+// histogram buffer (k_scan_tiles), so the scene's own exclusive scan is scanned(e) - scanned(first entry of
+// its segment) (modulo 2^32, exact).
 __global__ void __launch_bounds__(kSortThreads) k_batch_radix_scatter(const SceneDev* __restrict__ sd,
                                                                       const uint32_t* __restrict__ pre, int n_scenes,
-                                                                      int pass, const uint32_t* __restrict__ offsets) {
+                                                                      int pass, const uint32_t* __restrict__ offsets,
+                                                                      const uint32_t* __restrict__ tile_sums) {
   constexpr int kWaves = kSortThreads / 64;
   const uint32_t* P0 = pre + kPreRadixTiles * (n_scenes + 1);
   const uint32_t total = P0[n_scenes];
@@ -321,7 +297,7 @@ __global__ void __launch_bounds__(kSortThreads) k_batch_radix_scatter(const Scen
     uint32_t *kout, *vout;
     radix_io(S, pass, kin, vin, kout, vout);
     const uint32_t seg = 256u * P0[i];
-    running[threadIdx.x] = offsets[seg + threadIdx.x * tiles + t] - offsets[seg];
+    running[threadIdx.x] = scanned(offsets, tile_sums, seg + threadIdx.x * tiles + t) - scanned(offsets, tile_sums, seg);
     for (int j = 0; j < kSortItems; ++j) {
 #pragma unroll
       for (int w = 0; w < kWaves; ++w) wave_count[w][threadIdx.x] = 0;
@@ -462,7 +438,7 @@ __global__ void __launch_bounds__(256) k_batch_drift(const SceneDev* __restrict_
     const Particles P = S.set[st[i].cur];
     const int end = min(S.n, (int)(u - P0[i] + 1) * tile);
     for (int p = (int)(u - P0[i]) * tile + threadIdx.x; p < end; p += blockDim.x)
-      drift_route(P, S.g, p, sorted_keys(S), S.slot, S.in_cnt, S.in_lst, S.fwd);
+      drift_route_pair(P, S.g, p, -1, sorted_keys(S), S.slot, S.in_cnt, S.in_lst, S.fwd);
   }
   if (last_workgroup(tickets + kTicketDrift, &s_last))
     workgroup_prefix(pre + kPreBricks * (n_scenes + 1), n_scenes, s_warp,

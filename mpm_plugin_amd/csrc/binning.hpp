@@ -9,7 +9,8 @@
 //
 // Radix sort: 8-bit digits, least significant first. Each pass runs
 //   k_radix_hist     per-tile digit histogram, stored digit-major (hist[d * tiles + t])
-//   exclusive scan   over the digit-major histogram: the output offset of (digit d, tile t)
+//   k_scan_tiles     exclusive scan over the digit-major histogram, in two parts (tile-local scans and the
+//                    scanned tile totals) that give the output offset of (digit d, tile t) when added
 //   k_radix_scatter  per-tile stable ranking (wave64 ballots + per-wave digit counts in LDS)
 // A tile is 256 threads x kItems items, walked in item order so the ranking preserves input order.
 #pragma once
@@ -24,6 +25,12 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
+
+// word i of a two-part scan (k_scan_tiles)
+__device__ __forceinline__ uint32_t scanned(const uint32_t* __restrict__ out, const uint32_t* __restrict__ tile_sums,
+                                            uint32_t i) {
+  return out[i] + tile_sums[i / (uint32_t)kScanTile];
+}
 
 // keys[p] = cell key of particle slot p; vals[p] = p
 __global__ void __launch_bounds__(256) k_cell_keys(Particles P, GridGeom g, int n, uint32_t* __restrict__ keys,
@@ -57,6 +64,7 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* 
                                                                 const uint32_t* __restrict__ vals_in, int n,
                                                                 int shift, int tiles,
                                                                 const uint32_t* __restrict__ offsets,
+                                                                const uint32_t* __restrict__ tile_sums,
                                                                 uint32_t* __restrict__ keys_out,
                                                                 uint32_t* __restrict__ vals_out) {
   constexpr int kWaves = kSortThreads / 64;
@@ -64,7 +72,7 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* 
   __shared__ uint32_t wave_count[kWaves][256];
   const int t = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  running[threadIdx.x] = offsets[threadIdx.x * tiles + t];
+  running[threadIdx.x] = scanned(offsets, tile_sums, threadIdx.x * tiles + t);
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int j = 0; j < kSortItems; ++j) {
 #pragma unroll
@@ -126,10 +134,43 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   return r;
 }
 
+// Called by every thread of every workgroup of a kernel at its end (no workgroup may have returned
+// early): true, in all threads of one workgroup, for the workgroup that finishes last. `done` counts
+// tickets; it starts at 0 and the last workgroup resets it for the kernel's next launch. Nothing waits: a
+// workgroup that is not last just ends. `s_flag` is a shared word used for nothing else.
+// What the last workgroup may read of the other workgroups' work: counters they updated with returning
+// agent-scope atomics (each wave uses the returned value, so the update is performed before the wave
+// reaches the barrier below, hence before its workgroup's ticket), read back with load_agent. There is
+// no device-scope fence: on gfx950 one writes back the XCD's L2, and a fence per wave of a persistent
+// grid cost hundreds of microseconds per substep. Plain stores of this kernel reach the next kernel
+// through the kernel boundary as usual.
+__device__ __forceinline__ bool last_workgroup(uint32_t* done, uint32_t* s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const bool last = atomicAdd(done, 1u) == gridDim.x - 1;
+    if (last) atomicExch(done, 0u);
+    *s_flag = last ? 1u : 0u;
+  }
+  __syncthreads();
+  return *s_flag != 0u;
+}
+
+// a counter updated by other workgroups' agent-scope atomics, read at the coherence point those use
+__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan of n words in two parts: each kScanTile-word tile of `in` scanned on its own into `out`,
+// and the tiles' totals (up to kScanTile tiles), which the workgroup that finishes last scans in place, so
+// that the scan of word i is scanned(out, tile_sums, i). `ticket` (a last_workgroup counter) starts at 0
+// and is left at 0. Each total is published with a returning agent-scope exchange, which the last
+// workgroup reads back with load_agent.
 __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const uint32_t* __restrict__ in, int n,
                                                              uint32_t* __restrict__ out,
-                                                             uint32_t* __restrict__ tile_sums) {
+                                                             uint32_t* __restrict__ tile_sums,
+                                                             uint32_t* __restrict__ ticket) {
   __shared__ uint32_t s_warp[kScanThreads / 64 + 1];
+  __shared__ uint32_t s_last, s_prev;
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t sum = 0;
@@ -145,26 +186,24 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const uint32_t* __r
     if (base + k < n) out[base + k] = acc;
     acc += v[k];
   }
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
-}
-
-// single block: exclusive scan of up to kScanTile tile sums, in place
-__global__ void __launch_bounds__(kScanThreads) k_scan_top(uint32_t* __restrict__ sums, int n) {
-  __shared__ uint32_t s_warp[kScanThreads / 64 + 1];
-  const int base = threadIdx.x * kScanItems;
-  uint32_t v[kScanItems];
-  uint32_t sum = 0;
+  // the exchange's old value goes to LDS, so the wave waits for it before the ticket's barrier
+  if (threadIdx.x == 0) s_prev = atomicExch(&tile_sums[blockIdx.x], total);
+  if (!last_workgroup(ticket, &s_last)) return;
+  const int tiles = (int)gridDim.x;
+  const int tb = threadIdx.x * kScanItems;
+  uint32_t w[kScanItems];
+  uint32_t tsum = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    v[k] = (base + k < n) ? sums[base + k] : 0u;
-    sum += v[k];
+    w[k] = (tb + k < tiles) ? load_agent(&tile_sums[tb + k]) : 0u;
+    tsum += w[k];
   }
-  uint32_t total;
-  uint32_t acc = block_exclusive_scan(sum, s_warp, total);
+  uint32_t all;
+  uint32_t tacc = block_exclusive_scan(tsum, s_warp, all);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < n) sums[base + k] = acc;
-    acc += v[k];
+    if (tb + k < tiles) tile_sums[tb + k] = tacc;
+    tacc += w[k];
   }
 }
 

@@ -204,36 +204,60 @@ struct DriftLists {
   const uint8_t* fwd;  // per particle: routed to its current cell's list
 };
 
-__device__ __forceinline__ void drift_route(const Particles& P, const GridGeom& g, int p,
-                                            const uint32_t* __restrict__ keys, const int32_t* __restrict__ slot,
-                                            uint32_t* __restrict__ in_cnt, uint32_t* __restrict__ in_lst,
-                                            uint8_t* __restrict__ fwd) {
-  const float x[3] = {P.x[p], P.x[P.cap + p], P.x[2 * P.cap + p]};
-  int base[3];
-  float fx[3];
-  stencil_base(g, x, base, fx);
-  const uint32_t key = cell_key(g, base);
-  uint8_t routed = 0;
-  if (key != keys[p]) {
-    const int32_t s = slot[key >> 6];
-    if (s >= 0) {
-      const uint32_t c = (uint32_t)s * 64u + (key & 63u);
-      const uint32_t pos = atomicAdd(&in_cnt[c], 1u);
-      if (pos < (uint32_t)kInPerCell) {
-        in_lst[c * kInPerCell + pos] = (uint32_t)p;
-        routed = 1;
-      }
-    }
+// The routing of particles p0 and p1 (p1 < 0: p0 alone) against the cell keys of the last sort (`keys`, in
+// slot order). Each particle's accesses form a chain (position and old key, brick slot, list counter, list
+// entry); the two chains are issued side by side, so that they cost one chain's latency. The slot is read
+// whether or not the cell changed: every cell key lies inside the grid.
+__device__ __forceinline__ void drift_route_pair(const Particles& P, const GridGeom& g, int p0, int p1,
+                                                 const uint32_t* __restrict__ keys,
+                                                 const int32_t* __restrict__ slot, uint32_t* __restrict__ in_cnt,
+                                                 uint32_t* __restrict__ in_lst, uint8_t* __restrict__ fwd) {
+  const int pp[2] = {p0, p1 >= 0 ? p1 : p0};
+  float x[2][3];
+  uint32_t old[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    x[j][0] = P.x[pp[j]];
+    x[j][1] = P.x[P.cap + pp[j]];
+    x[j][2] = P.x[2 * P.cap + pp[j]];
+    old[j] = keys[pp[j]];
   }
-  fwd[p] = routed;
+  uint32_t key[2];
+  int32_t s[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    int base[3];
+    float fx[3];
+    stencil_base(g, x[j], base, fx);
+    key[j] = cell_key(g, base);
+    s[j] = slot[key[j] >> 6];
+  }
+  // the slot cell each particle drifted to (~0u: none), both settled before either counter update is
+  // issued: memory counters complete in issue order, so a wait for the second slot after the first update
+  // would wait for that update's return too
+  uint32_t cell[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+    cell[j] = key[j] != old[j] && s[j] >= 0 && (j == 0 || p1 >= 0) ? (uint32_t)s[j] * 64u + (key[j] & 63u) : ~0u;
+  asm volatile("" ::"v"(cell[0]), "v"(cell[1]));
+  uint32_t pos[2] = {~0u, ~0u};
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+    if (cell[j] != ~0u) pos[j] = atomicAdd(&in_cnt[cell[j]], 1u);
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+    if (pos[j] < (uint32_t)kInPerCell) in_lst[cell[j] * kInPerCell + pos[j]] = (uint32_t)pp[j];
+  fwd[p0] = pos[0] < (uint32_t)kInPerCell ? 1 : 0;
+  if (p1 >= 0) fwd[p1] = pos[1] < (uint32_t)kInPerCell ? 1 : 0;
 }
 
-// keys: the cell keys of the last sort, in slot order
+// two particles per thread, a launch's thread count apart (launched over (n + 1) / 2 threads)
 __global__ void __launch_bounds__(256) k_drift(Particles P, GridGeom g, int n, const uint32_t* __restrict__ keys,
                                                const int32_t* __restrict__ slot, uint32_t* __restrict__ in_cnt,
                                                uint32_t* __restrict__ in_lst, uint8_t* __restrict__ fwd) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) drift_route(P, g, p, keys, slot, in_cnt, in_lst, fwd);
+  const int stride = (int)(gridDim.x * blockDim.x);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += 2 * stride)
+    drift_route_pair(P, g, p, p + stride < n ? p + stride : -1, keys, slot, in_cnt, in_lst, fwd);
 }
 
 // A value of another lane of the same 16-lane row (DPP row shift; kRowShr4: lane l - 4, kRowShl8: l + 8, ...);
@@ -817,14 +841,18 @@ __global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* _
                                                     uint32_t seq, float* vmax_dev, DriftAhead da) {
   if (vmax_out && blockIdx.x == 0) reduce_block_maxima(block_max, block_count, vmax_out, seq_out, seq, vmax_dev);
   const uint32_t n = count ? *count : 0u;
+  // the next substep's drift routing (it reads only the positions G2P just wrote; the FP64 pass below
+  // changes no position), two particles a stride apart per thread
+  if (da.keys) {
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < da.n; p += 2 * stride)
+      drift_route_pair(P, da.g, p, p + stride < da.n ? p + stride : -1, da.keys, da.slot, da.in_cnt, da.in_lst,
+                       da.fwd);
+  }
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int64_t p = list[i];
     material_update_fp64(mats[P.mat[p]], P, p);
   }
-  // the next substep's drift routing (it reads only the positions G2P just wrote)
-  if (da.keys)
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < da.n; p += gridDim.x * blockDim.x)
-      drift_route(P, da.g, p, da.keys, da.slot, da.in_cnt, da.in_lst, da.fwd);
 }
 
 // c_stash: this thread's 9 LDS words, `stride` apart (one column per thread of the workgroup: no bank
