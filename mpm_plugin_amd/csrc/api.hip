@@ -6,11 +6,12 @@
 //
 // Substep on the default (binned) path:
 //   [every sort_interval substeps, or after any state change]
-//     k_cell_keys -> radix passes (k_radix_hist, scan, k_radix_scatter) -> k_brick_bounds
-//     -> k_brick_cost (cell offsets, cost class), k_brick_order (occupied bricks, costliest first)
-//     -> k_reorder
-//   [between sorts] k_drift (particles that changed cell -> incoming lists of their current cell)
-//   k_p2g_cells    (one wave per occupied brick, one lane per cell, wave-private LDS window)
+//     k_cell_keys -> radix passes (k_radix_hist, k_scan_tiles, k_radix_scatter)
+//     -> k_brick_bounds, k_brick_cost (cell offsets, cost class), k_brick_rank, k_brick_order (occupied
+//     bricks, costliest first) -> k_reorder
+//   [between sorts] drift routing (particles that changed cell -> incoming lists of their current cell), by
+//                  the kernel after the previous G2P (k_fp64_fixup), else k_drift
+//   k_p2g_split    (two waves per occupied brick, one lane per cell, LDS window)
 //   k_grid_bricks  (dirty bricks: update, walls, colliders)
 //   k_g2p          (gather, advection, collider projection, return map, stress, max speed)
 #include <hip/hip_ext.h>
@@ -621,8 +622,6 @@ void bin_offsets_and_bricks(mpm_ctx* c, const uint32_t* sorted_keys, int64_t n) 
 void sort_particles(mpm_ctx* c) {
   const int64_t n = c->n;
   Timed t(c, MPM_K_BIN);
-  k_zero_u32<<<1, 64, 0, c->stream>>>(c->n_active, 1);
-  check_launch("k_zero_u32");
   if (n == 0) {
     bin_offsets_and_bricks(c, c->keys1, 0);
     c->sorted = true;

@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const uint32_t* __r
                                                              uint32_t* __restrict__ tile_sums,
                                                              uint32_t* __restrict__ ticket) {
   __shared__ uint32_t s_warp[kScanThreads / 64 + 1];
-  __shared__ uint32_t s_last, s_prev;
+  __shared__ uint32_t s_last;
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t sum = 0;
@@ -186,8 +186,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const uint32_t* __r
     if (base + k < n) out[base + k] = acc;
     acc += v[k];
   }
-  // the exchange's old value goes to LDS, so the wave waits for it before the ticket's barrier
-  if (threadIdx.x == 0) s_prev = atomicExch(&tile_sums[blockIdx.x], total);
+  if (threadIdx.x == 0) {
+    // the old value is used (an empty asm takes it), so the exchange returns and the wave waits for it
+    // before the ticket's barrier
+    const uint32_t prev = atomicExch(&tile_sums[blockIdx.x], total);
+    asm volatile("" ::"v"(prev));
+  }
   if (!last_workgroup(ticket, &s_last)) return;
   const int tiles = (int)gridDim.x;
   const int tb = threadIdx.x * kScanItems;
