@@ -8,10 +8,12 @@ projection. Unless --steps is given, the timed steps run to the end of the workl
 substeps (2000 for the multi-material mix, whose collider reaches the sand column after about 580), after
 the warm-up and the profile pass. With --steps K there are two windows of K steps: right after the profile
 pass, and from substep 800 (collider in contact), the steps between them run untimed; the value is the
-particle-substeps of both windows over their time. P2G's mean launch time comes from a second run of the
-same substeps with only P2G bracketed. Inputs are resident in HBM before the timed region; the timed region is bracketed by
-device synchronisation. Per-kernel device time comes from HIP events recorded by the library on its
-own stream around every launch.
+particle-substeps of both windows over their time. The value's run brackets no launch with events (a
+dispatch that records events holds the next one back: about 11 us per substep with G2P bracketed); P2G's
+and G2P's mean launch times come from a second run of the same substeps with both bracketed. Inputs are
+resident in HBM before the timed region; the timed region is bracketed by device synchronisation.
+Per-kernel device time comes from HIP events recorded by the library on its own stream around the
+launches it brackets.
 
 A second measurement (skipped with --no-batch) runs the batched engine on the 256-scene mix
 (mpm_plugin_amd.workloads.scene_mix: grid edges 32-128, 5k-400k particles per scene, four material models,
@@ -456,9 +458,10 @@ def main(argv=None) -> int:
         if w.colliders and CONTACT_SUBSTEP >= first + args.steps and CONTACT_SUBSTEP + args.steps <= w.substeps:
             windows.append((CONTACT_SUBSTEP, CONTACT_SUBSTEP + args.steps))  # in collider contact as well
 
-    def timed_run(kernel: str, profile: bool):
-        """A fresh run: warm-up, [profile pass], then the windows (untimed steps between them) with only
-        `kernel` bracketed by the library's events. Returns (solver, seconds per window, kernel time, profile)."""
+    def timed_run(kernels: tuple, profile: bool):
+        """A fresh run: warm-up, [profile pass], then the windows (untimed steps between them) with only the
+        launches of `kernels` (none: no launch) bracketed by the library's events. Returns (solver, seconds
+        per window, kernel times, profile, active nodes per window)."""
         solver, step = make_run()
         for _ in range(args.warmup):
             step()
@@ -482,7 +485,7 @@ def main(argv=None) -> int:
         for a, b in windows:
             for _ in range(k, a):
                 step()
-            solver.set_timing(True, phases=[kernel])
+            solver.set_timing(bool(kernels), phases=list(kernels))
             solver.synchronize()
             t0 = time.perf_counter()
             for _ in range(a, b):
@@ -492,26 +495,27 @@ def main(argv=None) -> int:
             solver.set_timing(False)
             nodes.append(active_nodes(solver))
             k = b
-        return solver, secs, solver.kernel_times()[kernel], prof, nodes
+        return solver, secs, solver.kernel_times(), prof, nodes
 
-    # the timed steps bracket only G2P (every bracketed kernel adds its events' cost to the step)
-    solver, secs, g2p_time, times, nodes = timed_run("g2p", profile=True)
+    solver, secs, _, times, nodes = timed_run((), profile=True)
     # grid nodes with mass at the end of each window, weighted by the window's steps (the mean per launch)
     n_active = int(round(sum(m * (b - a) for m, (a, b) in zip(nodes, windows)) / sum(b - a for a, b in windows)))
     n_snow = int(sum((w.material_ids == i).sum() for i, m in enumerate(w.materials) if m.model == "snow"))
     log(f"profile pass ({PROFILE_STEPS} steps), kernel ms: "
         + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in times.items()))
-    timed = {"g2p": g2p_time}
     elapsed = sum(secs)
     total_steps = sum(b - a for a, b in windows)
     n = solver.particle_count
     solver.close()
-    # P2G's launch time over the same substeps, from a second run with only P2G bracketed
-    solver, _, timed["p2g"], _, _ = timed_run("p2g", profile=False)
+    # P2G's and G2P's launch times over the same substeps, from a second run with both bracketed
+    solver, secs_bracketed, kt, _, _ = timed_run(("g2p", "p2g"), profile=False)
+    timed = {"g2p": kt["g2p"], "p2g": kt["p2g"]}
     final = solver.get_particles() if not args.no_surface else None
     log(f"timed windows {windows}: " + ", ".join(f"{s:.3f} s" for s in secs)
-        + f"; g2p {timed['g2p'][0]:.2f} ms / {timed['g2p'][1]} launches (p2g {timed['p2g'][0]:.2f} ms / "
-          f"{timed['p2g'][1]} launches over the same substeps of a second run); active nodes {n_active}")
+        + f"; a second run of the same substeps with P2G and G2P bracketed: "
+        + ", ".join(f"{s:.3f} s" for s in secs_bracketed)
+        + f", g2p {timed['g2p'][0]:.2f} ms / {timed['g2p'][1]} launches, p2g {timed['p2g'][0]:.2f} ms / "
+          f"{timed['p2g'][1]} launches; active nodes {n_active}")
     bytes_per_launch = {
         "p2g": P2G_BYTES_PARTICLE * n + P2G_BYTES_SNOW * n_snow + P2G_BYTES_NODE * n_active,
         "g2p": G2P_BYTES_PARTICLE * n + G2P_BYTES_SNOW * n_snow + G2P_BYTES_NODE * n_active,
@@ -583,6 +587,7 @@ def main(argv=None) -> int:
                    "substeps_per_step": 1, "svd_precision": args.precision, "p2g_mode": args.p2g_mode,
                    "timed_substeps": [list(ab) for ab in windows],
                    "window_values": [round(n * (b - a) / t, 1) for (a, b), t in zip(windows, secs)],
+                   "value_with_p2g_g2p_bracketed": round(n * total_steps / sum(secs_bracketed), 1),
                    "active_nodes": n_active},
         "roofline": roofline,
         "cpu_baseline": baseline,
