@@ -62,7 +62,7 @@ BYTES_PER_OCCUPIED_BRICK = 216 * 16 + 64 * 4 + 64 * 4 * 4
 GROWTH = 1.5  # capacity grows by half when particles are added
 LATENCY_WINDOW = 32     # finished frames between two adjustments of the admission cap
 LATENCY_BACKOFF = 0.7   # the cap after a window whose p99 exceeded the target, as a share of bakes in flight
-LATENCY_HEADROOM = 0.7  # the cap grows after a window whose p99 stayed below this share of the target
+LATENCY_HEADROOM = 0.5  # the cap grows after a window whose p99 stayed below this share of the target
 LATENCY_START_CAP = 8   # bakes admitted at once before the first latency window
 
 
