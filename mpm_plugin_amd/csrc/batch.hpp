@@ -398,10 +398,13 @@ __global__ void __launch_bounds__(kTile) k_batch_reorder(const SceneDev* __restr
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     i = uniform(scene_of(P0, n_scenes, u, i));
     const SceneDev& S = sd[i];
-    const int p = (int)(u - P0[i]) * kTile + threadIdx.x;
-    if (p >= S.n) continue;
+    const int r0 = (int)(u - P0[i]) * kTile;
     const int cur = st[i].cur;
     const Particles src = S.set[cur], dst = S.set[1 - cur];
+    static_assert(kTile == 256, "reorder_records moves the 256 records of a 256-thread workgroup");
+    reorder_records(src.rec, dst.rec, sorted_vals(S), r0, S.n);
+    const int p = r0 + threadIdx.x;
+    if (p >= S.n) continue;
     const int64_t s = sorted_vals(S)[p];
     const int64_t cs = src.cap, cd = dst.cap;
 #pragma unroll
@@ -411,8 +414,6 @@ __global__ void __launch_bounds__(kTile) k_batch_reorder(const SceneDev* __restr
     }
 #pragma unroll
     for (int e = 0; e < 9; ++e) dst.F[e * cd + p] = src.F[e * cs + s];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) dst.rec[4 * (int64_t)p + e] = src.rec[4 * s + e];
     dst.Jp[p] = src.Jp[s];
     dst.mat[p] = src.mat[s];
     dst.pid[p] = src.pid[s];

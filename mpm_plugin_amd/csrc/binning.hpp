@@ -334,9 +334,24 @@ __global__ void __launch_bounds__(kBrickGroup) k_brick_order(int n_bricks, const
   slot[b] = (int32_t)pos;
 }
 
+// The 64-byte records of the 256 slots [r0, r0 + 256) handled by a 256-thread workgroup, dst[i] =
+// src[perm[i]] for i < n: four lanes per record, one float4 each, so that each load and store instruction
+// covers 16 whole records (with one float4 of 64 records per instruction, 64 records 64 bytes apart, the
+// gather took 79 instead of 53 us on the 1M mix).
+__device__ __forceinline__ void reorder_records(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                const uint32_t* __restrict__ perm, int r0, int n) {
+  const int e = (int)threadIdx.x & 3;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = r0 + q * 64 + ((int)threadIdx.x >> 2);
+    if (i < n) dst[4 * (int64_t)i + e] = src[4 * (int64_t)perm[i] + e];
+  }
+}
+
 // gather every per-slot particle array into sorted order: dst[i] = src[perm[i]]
 __global__ void __launch_bounds__(256) k_reorder(Particles src, Particles dst, const uint32_t* __restrict__ perm,
                                                  int n) {
+  reorder_records(src.rec, dst.rec, perm, blockIdx.x * blockDim.x, n);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t s = perm[i];
@@ -348,8 +363,6 @@ __global__ void __launch_bounds__(256) k_reorder(Particles src, Particles dst, c
   }
 #pragma unroll
   for (int e = 0; e < 9; ++e) dst.F[e * cd + i] = src.F[e * cs + s];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) dst.rec[4 * (int64_t)i + e] = src.rec[4 * s + e];
   dst.Jp[i] = src.Jp[s];
   dst.mat[i] = src.mat[s];
   dst.pid[i] = src.pid[s];
