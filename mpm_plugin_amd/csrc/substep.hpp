@@ -858,13 +858,14 @@ __global__ void __launch_bounds__(256) k_fp64_fixup(Particles P, const MatDev* _
 // c_stash: this thread's LDS words, `stride` apart (one column per thread of the workgroup: no bank
 // conflicts). The new C waits there across the SVD instead of in registers, so the 64-byte record is written
 // whole at the end: written a float4 at a time at two points of the kernel, its lines left the L2 partly
-// written and cost about a quarter more HBM write traffic. stage_record: the whole record (C, stress,
+// written and cost about a quarter more HBM write traffic. kStageRecord: the whole record (C, stress,
 // mass: 16 words) is left in the stash for the caller to store, instead of stored here.
+template <bool kStageRecord = false>
 __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restrict__ mats, const GridGeom& g,
                                              const StepConsts& sc, float dt, const float4* __restrict__ vel,
                                              int64_t p, const ColliderSet& cs,
                                              const ColliderFrame* frames, float& speed, float* c_stash,
-                                             int stride, bool stage_record = false) {
+                                             int stride) {
   const int64_t cap = P.cap;
   const float xo[3] = {P.x[p], P.x[cap + p], P.x[2 * cap + p]};
   float fx[3];
@@ -972,7 +973,7 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
   } else {
     material_update(m, Fn, U, sig, V, P, t, p);
   }
-  if (stage_record) {
+  if constexpr (kStageRecord) {
 #pragma unroll
     for (int e = 0; e < 6; ++e) c_stash[(9 + e) * stride] = t[e];
     c_stash[15 * stride] = m.p_mass;
@@ -989,7 +990,7 @@ __device__ __forceinline__ bool g2p_particle(Particles& P, const MatDev* __restr
 }
 
 // The records of a wave's 64 consecutive particles, slots p0 + 0..63 (those below n), from a stash of 16 rows
-// `stride` words apart whose column c0 + l holds particle p0 + l's record (g2p_particle with stage_record):
+// `stride` words apart whose column c0 + l holds particle p0 + l's record (g2p_particle<true>):
 // four lanes per record, one float4 each, so that each store instruction writes 16 whole records (one
 // float4 of 64 records per instruction: G2P 45.2 instead of 41.9 us on the 1M mix). A stride of 257 puts
 // the four words a lane reads in four banks. The whole wave calls it.
@@ -1033,7 +1034,7 @@ __global__ void __launch_bounds__(256) k_g2p(Particles P, const MatDev* __restri
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float speed = 0.0f;
   bool defer = false;
-  if (p < n) defer = g2p_particle(P, mats, g, sc, dt, vel, p, cs, frames, speed, &s_c[0][threadIdx.x], 257, true);
+  if (p < n) defer = g2p_particle<true>(P, mats, g, sc, dt, vel, p, cs, frames, speed, &s_c[0][threadIdx.x], 257);
   const int w0 = threadIdx.x & ~63;
   store_staged_records(P, &s_c[0][0], 257, w0, (int64_t)blockIdx.x * blockDim.x + w0, n);
   if (sc.precision != kPrecisionFp32) wave_append(fix, defer, (uint32_t)p);
